@@ -1,0 +1,124 @@
+"""ctypes binding of libfastbart_hip.so (the C-ABI declared in include/fastbart_hip.h).
+
+The library is loaded from the package directory. If it is missing it is built once with hipcc; if it cannot be built
+or loaded, the import fails: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfastbart_hip.so")
+
+NODES = 64
+STATS_REF16 = 0
+STATS_FX64 = 1
+
+
+class FbhError(RuntimeError):
+    pass
+
+
+class ChainIn(C.Structure):
+    _fields_ = [("n", C.c_int), ("p", C.c_int), ("q", C.c_void_p), ("y", C.c_void_p), ("wt", C.c_void_p),
+                ("numcut", C.c_void_p), ("ybin", C.c_void_p), ("tau", C.c_float), ("lambda_", C.c_float),
+                ("nu", C.c_float), ("sigest", C.c_float), ("offset", C.c_float), ("seed", C.c_uint)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("m", C.c_int), ("nskip", C.c_int), ("ndpost", C.c_int), ("probit", C.c_int),
+                ("keep_trees", C.c_int), ("pgtab", C.c_void_p), ("stats_mode", C.c_int), ("node_cap", C.c_int),
+                ("device", C.c_int), ("sweeps_per_launch", C.c_int), ("block_threads", C.c_int),
+                ("tile_rows", C.c_int)]
+
+
+class ChainOut(C.Structure):
+    _fields_ = [("sigma", C.c_void_p), ("trmean", C.c_void_p), ("trees", C.c_void_p), ("stats", C.c_void_p)]
+
+
+class TraceRec(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("it", "j", "mode", "nx", "v", "c", "nl", "nr", "accepted", "ngood",
+                                          "nnogs", "cap")] + \
+               [(k, C.c_double) for k in ("pr", "lalpha", "lu", "sl0", "sl1", "sr0", "sr1")] + \
+               [("rng_after", C.c_uint32 * 4)]
+
+
+# every symbol include/fastbart_hip.h declares (checked by the CPU tests)
+EXPORTS = ["bart_fit_cpu", "bart_wfit", "bart_set_rng", "fbh_last_status", "fbh_last_error", "fbh_batch_create",
+           "fbh_batch_run", "fbh_batch_sync", "fbh_batch_iter", "fbh_batch_fetch", "fbh_batch_state",
+           "fbh_batch_yhat", "fbh_batch_timing", "fbh_batch_destroy", "fbh_fit_batch", "fbh_tree_text", "fbh_yhat",
+           "fbh_compile_trees", "fbh_trees_info", "fbh_trees_arrays", "fbh_trees_free", "fbh_predict_draw",
+           "fbh_predict_draws", "fbh_rows_upload", "fbh_rows_free", "fbh_predict_rows", "fbh_predict_timing",
+           "fbh_debug_math", "fbh_debug_libm_host", "fbh_batch_trace", "fbh_version"]
+
+_lock = threading.Lock()
+_lib = None
+
+
+def _declare(L):
+    vp, i, d, f = C.c_void_p, C.c_int, C.c_double, C.c_float
+    L.bart_fit_cpu.argtypes = [vp] * 8 + [i] * 5 + [f] * 4 + [C.c_uint, i, vp, i, vp, f]
+    L.bart_fit_cpu.restype = None
+    L.bart_wfit.argtypes = [vp] * 8 + [i] * 5 + [f] * 4 + [C.c_uint, i, vp]
+    L.bart_wfit.restype = None
+    L.bart_set_rng.argtypes = [vp] * 4
+    L.bart_set_rng.restype = None
+    L.fbh_last_status.restype = i
+    L.fbh_last_error.restype = C.c_char_p
+    L.fbh_version.restype = C.c_char_p
+    L.fbh_batch_create.argtypes = [C.POINTER(ChainIn), i, C.POINTER(Opts), C.POINTER(vp)]
+    L.fbh_batch_run.argtypes = [vp, i]
+    L.fbh_batch_sync.argtypes = [vp]
+    L.fbh_batch_iter.argtypes = [vp]
+    L.fbh_batch_fetch.argtypes = [vp, i, C.POINTER(ChainOut)]
+    L.fbh_batch_state.argtypes = [vp, i, vp, vp, vp, vp, vp, vp]
+    L.fbh_batch_yhat.argtypes = [vp, i, vp]
+    L.fbh_batch_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i), C.POINTER(d)]
+    L.fbh_batch_destroy.argtypes = [vp]
+    L.fbh_batch_destroy.restype = None
+    L.fbh_batch_trace.argtypes = [vp, i, vp, i]
+    L.fbh_fit_batch.argtypes = [C.POINTER(ChainIn), i, C.POINTER(Opts), C.POINTER(ChainOut)]
+    L.fbh_tree_text.argtypes = [vp, i, i, i, vp, C.c_size_t, vp]
+    L.fbh_tree_text.restype = C.c_size_t
+    L.fbh_yhat.argtypes = [vp, vp, i, i, i, i, i, vp]
+    L.fbh_compile_trees.argtypes = [C.c_char_p, C.c_size_t, vp, vp, i, i, C.POINTER(vp)]
+    L.fbh_trees_info.argtypes = [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)]
+    L.fbh_trees_arrays.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.fbh_trees_free.argtypes = [vp]
+    L.fbh_trees_free.restype = None
+    L.fbh_predict_draw.argtypes = [vp, vp, i, i, i, d, vp]
+    L.fbh_predict_draws.argtypes = [vp, vp, i, i, vp, i, d, vp]
+    L.fbh_rows_upload.argtypes = [vp, i, i, i, C.POINTER(vp)]
+    L.fbh_rows_free.argtypes = [vp]
+    L.fbh_rows_free.restype = None
+    L.fbh_predict_rows.argtypes = [vp, vp, vp, i, d, vp]
+    L.fbh_predict_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i)]
+    L.fbh_debug_math.argtypes = [i, vp, vp, C.c_long, vp, vp, i]
+    L.fbh_debug_libm_host.argtypes = [i, C.c_long, C.c_long, vp]
+    return L
+
+
+def lib():
+    """The loaded library (built on first use if absent)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                from . import build as _b
+                _b.build()
+            _lib = _declare(C.CDLL(LIB_PATH))
+        return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().fbh_last_error().decode(errors="replace")
+        raise FbhError(f"{what or 'fastbart_hip'} failed ({rc}): {msg}")
+
+
+def ptr(a: np.ndarray | None):
+    return None if a is None else C.c_void_p(a.ctypes.data)

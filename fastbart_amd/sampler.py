@@ -1,0 +1,191 @@
+"""Host-side interface over the C-ABI: batched chains, the single-chain drop-in, and the output formats.
+
+The names and argument meanings follow the reference's C-ABI (``bart_fit_cpu``, src/bart_kernel.h:63-66) and its glue
+(``fbart_cpp``, src/glue.cpp:10-62). Inputs are *problems* in the C-ABI form (see fastbart_amd.workloads).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import NODES, ChainIn, ChainOut, FbhError, Opts, check, lib, ptr
+
+
+def _chain_in(prob, seed, probit, keep):
+    q = np.ascontiguousarray(prob["q"], dtype=np.uint8)
+    p, n = q.shape
+    y = np.ascontiguousarray(prob["y"], dtype=np.float32)
+    wt = np.ascontiguousarray(prob["wt"], dtype=np.float32)
+    nc = np.ascontiguousarray(prob["numcut"], dtype=np.int32)
+    yb = prob.get("ybin")
+    yb = np.ascontiguousarray(yb, dtype=np.int32) if yb is not None else None
+    keep.extend([q, y, wt, nc, yb])
+    ci = ChainIn()
+    ci.n, ci.p = n, p
+    ci.q, ci.y, ci.wt, ci.numcut = ptr(q), ptr(y), ptr(wt), ptr(nc)
+    ci.ybin = ptr(yb) if probit else None
+    ci.tau = prob["tau"]
+    ci.lambda_ = prob["lambda"]
+    ci.nu = prob["nu"]
+    ci.sigest = prob["sigest"]
+    ci.offset = prob.get("offset", 0.0)
+    ci.seed = int(seed) & 0xFFFFFFFF
+    return ci
+
+
+class BatchSampler:
+    """Many independent chains sampled together on one GPU (fbh_batch_*)."""
+
+    def __init__(self, problems, m, nskip, ndpost, seeds, probit=0, keep_trees=1, stats_mode=0, node_cap=0,
+                 device=0, sweeps_per_launch=0, pgtab=None):
+        L = lib()
+        self.problems = problems
+        self.C = len(problems)
+        self.m, self.nskip, self.ndpost, self.probit = m, nskip, ndpost, probit
+        self.keep_trees = keep_trees
+        self._keep = []
+        arr = (ChainIn * self.C)()
+        for c, pb in enumerate(problems):
+            arr[c] = _chain_in(pb, seeds[c], probit, self._keep)
+        pg = np.ascontiguousarray(pgtab if pgtab is not None else problems[0]["pgtab"], dtype=np.float32)
+        self._keep.append(pg)
+        o = Opts(m, nskip, ndpost, probit, keep_trees, ptr(pg), stats_mode, node_cap, device, sweeps_per_launch, 0, 0)
+        h = C.c_void_p()
+        check(L.fbh_batch_create(arr, self.C, C.byref(o), C.byref(h)), "fbh_batch_create")
+        self.h = h
+        self.n = [int(pb["q"].shape[1]) for pb in problems]
+        self.p = [int(pb["q"].shape[0]) for pb in problems]
+
+    def run(self, nsweeps):
+        check(lib().fbh_batch_run(self.h, int(nsweeps)), "fbh_batch_run")
+
+    def sync(self):
+        check(lib().fbh_batch_sync(self.h), "fbh_batch_sync")
+
+    @property
+    def it(self):
+        return lib().fbh_batch_iter(self.h)
+
+    def fetch(self, c):
+        n = self.n[c]
+        sig = np.zeros(self.nskip + self.ndpost, np.float32)
+        trm = np.zeros(n, np.float32)
+        trees = np.zeros((self.ndpost, self.m, NODES), np.uint64) if self.keep_trees else None
+        st = np.zeros(3, np.int32)
+        out = ChainOut(ptr(sig), ptr(trm), ptr(trees), ptr(st))
+        check(lib().fbh_batch_fetch(self.h, c, C.byref(out)), "fbh_batch_fetch")
+        res = dict(sigma=sig, trmean=trm, stats=st)
+        if trees is not None:
+            res["trees"] = trees
+        return res
+
+    def state(self, c):
+        n = self.n[c]
+        rng = np.zeros(4, np.uint32)
+        sig = C.c_double()
+        af = np.zeros(n)
+        lat = np.zeros(n)
+        leaf = np.zeros((self.m, n), np.uint8)
+        words = np.zeros((self.m, NODES), np.uint64)
+        check(lib().fbh_batch_state(self.h, c, ptr(rng), C.byref(sig), ptr(af), ptr(lat), ptr(leaf), ptr(words)),
+              "fbh_batch_state")
+        return dict(rng=rng, sigma=sig.value, af=af, latent=lat, leaf=leaf, trees=words)
+
+    def yhat(self, c):
+        out = np.zeros((self.ndpost, self.n[c]), np.float64)
+        check(lib().fbh_batch_yhat(self.h, c, ptr(out)), "fbh_batch_yhat")
+        return out
+
+    def timing(self):
+        avg, k, by = C.c_double(), C.c_int(), C.c_double()
+        check(lib().fbh_batch_timing(self.h, C.byref(avg), C.byref(k), C.byref(by)), "fbh_batch_timing")
+        return avg.value, k.value, by.value
+
+    def enable_trace(self, cap):
+        check(lib().fbh_batch_trace(self.h, int(cap), None, 0), "fbh_batch_trace")
+        self._trace_cap = cap
+
+    def trace(self, c):
+        buf = (_lib.TraceRec * self._trace_cap)()
+        check(lib().fbh_batch_trace(self.h, 0, C.cast(buf, C.c_void_p), c), "fbh_batch_trace")
+        return buf
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fbh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def bart_fit(prob, m, nskip, ndpost, seed, keep_trees=1, probit=0):
+    """The drop-in entry ``bart_fit_cpu`` (src/bart_kernel.h:63-66) on one chain; returns its output buffers."""
+    L = lib()
+    q = np.ascontiguousarray(prob["q"], dtype=np.uint8)
+    p, n = q.shape
+    y = np.ascontiguousarray(prob["y"], dtype=np.float32)
+    wt = np.ascontiguousarray(prob["wt"], dtype=np.float32)
+    nc = np.ascontiguousarray(prob["numcut"], dtype=np.int32)
+    pg = np.ascontiguousarray(prob["pgtab"], dtype=np.float32)
+    yb = prob.get("ybin")
+    yb = np.ascontiguousarray(yb, dtype=np.int32) if yb is not None else None
+    sig = np.zeros(nskip + ndpost, np.float32)
+    trm = np.zeros(n, np.float32)
+    trees = np.zeros((ndpost, m, NODES), np.uint64) if keep_trees else None
+    st = np.zeros(8, np.int32)
+    L.bart_set_rng(None, None, None, None)
+    L.bart_fit_cpu(ptr(q), ptr(y), ptr(wt), ptr(nc), ptr(pg), ptr(sig), ptr(trm), ptr(trees), n, p, m, nskip, ndpost,
+                   prob["tau"], prob["lambda"], prob["nu"], prob["sigest"], int(seed) & 0xFFFFFFFF, keep_trees,
+                   ptr(st), probit, ptr(yb) if probit else None, prob.get("offset", 0.0))
+    if L.fbh_last_status() != 0:
+        raise FbhError("bart_fit_cpu: " + L.fbh_last_error().decode(errors="replace"))
+    res = dict(sigma=sig, trmean=trm, stats=st[:3].copy())
+    if keep_trees:
+        res["trees"] = trees
+    return res
+
+
+def tree_text(trees, p):
+    """Treedraws text and varcount of [ndpost][m][64] output words (src/glue.cpp:32-51)."""
+    trees = np.ascontiguousarray(trees, dtype=np.uint64)
+    nd, m = trees.shape[0], trees.shape[1]
+    L = lib()
+    need = L.fbh_tree_text(ptr(trees), nd, m, p, None, 0, None)
+    buf = C.create_string_buffer(need + 1)
+    vc = np.zeros((nd, p), np.int32)
+    L.fbh_tree_text(ptr(trees), nd, m, p, C.cast(buf, C.c_void_p), need + 1, ptr(vc))
+    return buf.raw[:need].decode(), vc
+
+
+def yhat(trees, q, device=0):
+    """Per-draw training fits from output words (src/glue.cpp:52-56), on the device."""
+    trees = np.ascontiguousarray(trees, dtype=np.uint64)
+    q = np.ascontiguousarray(q, dtype=np.uint8)
+    nd, m = trees.shape[0], trees.shape[1]
+    p, n = q.shape
+    out = np.zeros((nd, n), np.float64)
+    check(lib().fbh_yhat(ptr(trees), ptr(q), n, p, nd, m, device, ptr(out)), "fbh_yhat")
+    return out
+
+
+def debug_math(kind, u=None, d=None, device=0):
+    u = None if u is None else np.ascontiguousarray(u, dtype=np.uint32)
+    d = None if d is None else np.ascontiguousarray(d, dtype=np.float64)
+    count = {0: len(u) // 2 if u is not None else 0, 1: len(u) if u is not None else 0}.get(
+        kind, (len(d) // 2 if kind == 5 else len(d)) if d is not None else 0)
+    of = np.zeros(count, np.float32) if kind <= 1 else None
+    od = np.zeros(count, np.float64) if kind >= 2 else None
+    check(lib().fbh_debug_math(kind, ptr(u), ptr(d), count, ptr(of), ptr(od), device), "fbh_debug_math")
+    return of if kind <= 1 else od

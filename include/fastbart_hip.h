@@ -1,0 +1,175 @@
+/* fastbart_hip.h -- C-ABI of the MI355X (gfx950) BART sampler library libfastbart_hip.so.
+ *
+ * Plain C types and caller-owned host buffers only. Every entry point returns an int status (0 = success,
+ * negative = error; fbh_last_error() describes the last error of the calling thread) except the three drop-in
+ * symbols, which keep the reference's void signatures.
+ *
+ * Drop-in boundary (reference interfaces this library replaces):
+ *   bart_fit_cpu / bart_wfit / bart_set_rng   src/bart_kernel.h:49-70 (called from src/glue.cpp:24-27)
+ *   fbh_fit_batch / fbh_batch_*               new: many independent chains in one persistent launch (no reference
+ *                                             counterpart; the reference runs one chain per R process)
+ *   fbh_tree_text / fbh_yhat                  the output half of fbart_cpp, src/glue.cpp:29-57
+ *   fbh_compile_trees / fbh_predict_draw(s)   compile_trees_cpp / predict_draw_cpp, src/predict.cpp:12-73
+ */
+#ifndef FASTBART_HIP_H
+#define FASTBART_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FBH_NODES 64       /* node slots per tree (src/bart_kernel.h:22-24) */
+#define FBH_NMAX 65536     /* rows per chain (src/bart_kernel.h:15-16) */
+#define FBH_PMAX 32        /* predictors (src/bart_kernel.h:20) */
+#define FBH_MMAX 200       /* trees (src/bart_kernel.h:21) */
+
+/* error codes */
+#define FBH_OK 0
+#define FBH_EINVAL -1      /* bad argument / shape outside the limits above */
+#define FBH_ENODEV -2      /* no usable gfx950 device */
+#define FBH_EHIP -3        /* a HIP call failed */
+#define FBH_ENOMEM -4      /* device or host allocation failed */
+#define FBH_ERNG -5        /* an external (R) generator is installed: the device cannot reproduce its stream */
+#define FBH_EPARSE -6      /* malformed tree text */
+
+/* statistics modes */
+#define FBH_STATS_REF16 0  /* double sums in 16 row lanes, bit-identical to the reference CPU sampler */
+#define FBH_STATS_FX64 1   /* int64 fixed point (w, w*r at 2^32; rss at 2^24): order independent */
+
+/* ---------------------------------------------------------------- drop-in single chain */
+/* Same arguments and semantics as src/bart_kernel.h:63-66 / src/bart_kernel.cpp:304-307, run on the GPU with the
+ * internal xoshiro128** generator. If bart_set_rng installed an external generator, nothing is computed,
+ * stats_out[0..2] are set to -1 and fbh_last_status() returns FBH_ERNG. */
+void bart_fit_cpu(const uint8_t* qin, const float* yin, const float* wtin, const int* numcut, const float* pgtab,
+                  float* sigma_out, float* trmean_out, uint64_t* trees_out,
+                  int n, int p, int m, int nskip, int ndpost, float tau, float lambda, float nu, float sigest,
+                  unsigned seed, int keep_trees, int* stats_out, int probit, const int* ybin, float offset);
+/* src/bart_kernel.h:49-58: bart_fit_cpu with probit = 0 */
+void bart_wfit(const uint8_t* qin, const float* yin, const float* wtin, const int* numcut, const float* pgtab,
+               float* sigma_out, float* trmean_out, uint64_t* trees_out,
+               int n, int p, int m, int nskip, int ndpost, float tau, float lambda, float nu, float sigest,
+               unsigned seed, int keep_trees, int* stats_out);
+/* src/bart_kernel.h:69: all-null selects the internal generator (FASTBART_RNG=internal); any non-null pointer marks
+ * an external generator, which the device path refuses (FBH_ERNG). */
+void bart_set_rng(double (*u)(void), double (*nrm)(void), double (*e)(void), double (*chisq)(double));
+/* status of the last drop-in call on this thread */
+int fbh_last_status(void);
+const char* fbh_last_error(void);
+
+/* ---------------------------------------------------------------- batched chains */
+typedef struct fbh_chain_in {
+  int n, p;                 /* rows, predictors */
+  const uint8_t* q;         /* [p][n] cutpoint ranks: x < xi[v][c] <=> q <= c */
+  const float* y;           /* [n] centred outcome (ignored for probit) */
+  const float* wt;          /* [n] 1/w^2 */
+  const int* numcut;        /* [p] */
+  const int* ybin;          /* [n] 0/1 outcomes, probit only */
+  float tau, lambda, nu, sigest, offset;
+  unsigned seed;            /* xoshiro128** seed of this chain */
+} fbh_chain_in;
+
+typedef struct fbh_opts {
+  int m, nskip, ndpost;     /* trees, burn-in sweeps, kept sweeps (shared by the batch) */
+  int probit;               /* 0 wbart, 1 pbart */
+  int keep_trees;           /* store the kept trees on the device */
+  const float* pgtab;       /* [64] base/(1+d)^power as float */
+  int stats_mode;           /* FBH_STATS_REF16 / FBH_STATS_FX64 */
+  int node_cap;             /* node slots a tree may use (3..64), 0 = 64 */
+  int device;               /* HIP device ordinal */
+  int sweeps_per_launch;    /* 0 = automatic */
+  int block_threads;        /* 0 = automatic (tuned); otherwise 64..1024, a multiple of 64 */
+  int tile_rows;            /* 0 = automatic; rows staged in LDS per tile, multiple of 16 */
+} fbh_opts;
+
+typedef struct fbh_chain_out {
+  float* sigma;             /* [nskip+ndpost] (may be null) */
+  float* trmean;            /* [n] (may be null) */
+  uint64_t* trees;          /* [ndpost][m][64] output words (may be null) */
+  int* stats;               /* [3] births, deaths, node-cap hits (may be null) */
+} fbh_chain_out;
+
+typedef struct fbh_batch fbh_batch;
+
+/* Upload the inputs and initialise every chain (root-only trees, seeded generator, probit latents). */
+int fbh_batch_create(const fbh_chain_in* chains, int nchains, const fbh_opts* opts, fbh_batch** out);
+/* Enqueue the next nsweeps sweeps of every chain on the batch's stream (asynchronous). */
+int fbh_batch_run(fbh_batch* b, int nsweeps);
+int fbh_batch_sync(fbh_batch* b);
+/* sweeps done so far */
+int fbh_batch_iter(const fbh_batch* b);
+/* Copy chain c's outputs to host buffers (synchronises). */
+int fbh_batch_fetch(fbh_batch* b, int chain, const fbh_chain_out* out);
+/* Copy chain c's sampler state (parity checks, checkpoints): rng[4], sigma, af[n], latent[n], leaf[m][n],
+ * tree words [m][64]; any pointer may be null. */
+int fbh_batch_state(fbh_batch* b, int chain, uint32_t* rng, double* sigma, double* af, double* latent,
+                    uint8_t* leaf, uint64_t* words);
+/* yhat[dr][i] = sum over the m trees of draw dr, in tree order, of the leaf value row i reaches (src/glue.cpp:52-56),
+ * computed on the device from the stored trees; yhat is [ndpost][n] host memory. */
+int fbh_batch_yhat(fbh_batch* b, int chain, double* yhat);
+/* Average duration (ms) of the sweep-kernel launches so far, from HIP events on the batch stream, and the launch count.
+ * Algorithmic bytes of those launches (37 bytes per row per tree update, (m+1)/m for the closing pass). */
+int fbh_batch_timing(const fbh_batch* b, double* avg_ms, int* nlaunch, double* bytes_per_launch);
+void fbh_batch_destroy(fbh_batch* b);
+
+/* create + run all sweeps + fetch every chain + destroy */
+int fbh_fit_batch(const fbh_chain_in* chains, int nchains, const fbh_opts* opts, const fbh_chain_out* outs);
+
+/* ---------------------------------------------------------------- output formats (src/glue.cpp:29-57) */
+/* treedraws text of ndpost x m packed trees: "ndpost m p" then per tree the node count and pre-order lines
+ * "id v c theta" (%.10g). Returns the full length; writes at most cap bytes (NUL-terminated if it fits).
+ * varcount [ndpost][p] may be null. */
+size_t fbh_tree_text(const uint64_t* trees, int ndpost, int m, int p, char* buf, size_t cap, int* varcount);
+/* yhat[dr][i] from host tree words and q [p][n], evaluated on the device */
+int fbh_yhat(const uint64_t* trees, const uint8_t* q, int n, int p, int ndpost, int m, int device, double* yhat);
+
+/* ---------------------------------------------------------------- prediction (src/predict.cpp) */
+typedef struct fbh_trees fbh_trees;
+/* Parse treedraws text into flat breadth-first child-pair arrays (children adjacent, cut = cutpoints[v][c]) and upload
+ * them. cutpoints is the concatenation of the per-variable cutpoint arrays, cut_off[v] the start of variable v
+ * (cut_off has pcut+1 entries). */
+int fbh_compile_trees(const char* text, size_t len, const double* cutpoints, const int* cut_off, int pcut, int device,
+                      fbh_trees** out);
+/* ndpost, ntree, p, node count; and a host copy of the flat arrays (any pointer may be null) */
+int fbh_trees_info(const fbh_trees* t, int* ndpost, int* ntree, int* p, int* nnodes);
+int fbh_trees_arrays(const fbh_trees* t, int* var, int* child, double* cut, double* theta, int* root);
+void fbh_trees_free(fbh_trees* t);
+/* One draw (m_draw 1-based) for n rows given as p host columns: out[i] = mu + sum over trees in order of theta at the
+ * leaf reached with x >= cut going right (predict_draw_cpp, src/predict.cpp:111-135). */
+int fbh_predict_draw(const fbh_trees* t, const double* const* cols, int n, int p, int m_draw, double mu, double* out);
+/* Batched pwbart: draws[k] (1-based) for k < ndraws over the same rows; out [ndraws][n]. x is row data uploaded
+ * once per call as p columns. */
+int fbh_predict_draws(const fbh_trees* t, const double* const* cols, int n, int p, const int* draws, int ndraws,
+                      double mu, double* out);
+
+/* device-resident rows for repeated prediction: upload once, predict many draws */
+typedef struct fbh_rows fbh_rows;
+int fbh_rows_upload(const double* const* cols, int n, int p, int device, fbh_rows** out);
+void fbh_rows_free(fbh_rows* r);
+/* out_dev may be null (results copied to out_host [ndraws][n]) */
+int fbh_predict_rows(const fbh_trees* t, const fbh_rows* r, const int* draws, int ndraws, double mu, double* out_host);
+/* average predict-kernel duration (ms) and launch count since the trees were compiled */
+int fbh_predict_timing(const fbh_trees* t, double* avg_ms, int* nlaunch);
+
+/* ---------------------------------------------------------------- diagnostics */
+/* Evaluate the device normal draw, logf/cosf tables and double math on host-given inputs (parity tests):
+ *   kind 0: out_f[i] = normal from raw generator outputs (in_u[2i], in_u[2i+1])
+ *   kind 1: out_f[i] = sqrtf(-2*logf(u)) for u index in_u[i] (top 24 bits)
+ *   kind 2: out_d[i] = log(in_d[i]); kind 3: out_d[i] = exp(in_d[i]); kind 4: out_d[i] = sqrt(in_d[i])
+ *   kind 5: out_d[i] = in_d[2i] / in_d[2i+1] */
+int fbh_debug_math(int kind, const uint32_t* in_u, const double* in_d, long count, float* out_f, double* out_d,
+                   int device);
+/* host libm image entries [k0, k0+count) of table kind 0 (logf), 1 (cosf of 6.28318531f*u), 2 (log of the double u)
+ * on the generator grid u_k = (k + 1/2) 2^-24; out is float for kinds 0-1, double for kind 2 (no device needed) */
+int fbh_debug_libm_host(int kind, long k0, long count, void* out);
+/* per-tree-update trace of the sampler: cap > 0 allocates cap records per chain for the following launches;
+ * cap = 0 copies chain's records (struct layout: fbh::TraceRec) into host_out */
+int fbh_batch_trace(fbh_batch* b, int cap, void* host_out, int chain);
+/* library build information */
+const char* fbh_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,62 @@
+"""GPU arithmetic the sampler's bit-identity rests on: the float normal draw, correctly rounded sqrt and division, and
+the agreement rate of the device double log/exp with the host libm (used only inside comparisons)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import same_bits
+from fastbart_amd import sampler
+
+pytestmark = pytest.mark.gpu
+
+
+def test_gpu_sqrt_of_logf_image_exhaustive(oracle):
+    """sqrtf(-2 logf(u)) on the device for all 2^24 grid points equals the host's."""
+    k = (np.arange(1 << 24, dtype=np.uint64) << np.uint64(8)).astype(np.uint32)
+    got = sampler.debug_math(1, u=k)
+    assert same_bits(got, oracle.libm_image(3))
+
+
+def test_gpu_normal_pairs(oracle):
+    rng = np.random.default_rng(1)
+    raw = rng.integers(0, 2 ** 32, size=2 * 200000, dtype=np.uint64).astype(np.uint32)
+    got = sampler.debug_math(0, u=raw)
+    exp = oracle.normal_from(raw[0::2][:20000], raw[1::2][:20000])
+    assert same_bits(got[:20000], exp)
+
+
+def test_gpu_double_sqrt_and_division_are_correctly_rounded():
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.uniform(0, 10, 200000), np.exp(rng.uniform(-700, 700, 200000)),
+                        rng.uniform(1e-300, 1e-290, 1000)])
+    assert same_bits(sampler.debug_math(4, d=x), np.sqrt(x))
+    a = rng.standard_normal(400000) * np.exp(rng.uniform(-30, 30, 400000))
+    b = rng.standard_normal(400000) * np.exp(rng.uniform(-30, 30, 400000))
+    ab = np.empty(800000)
+    ab[0::2], ab[1::2] = a, b
+    assert same_bits(sampler.debug_math(5, d=ab), a / b)
+
+
+def test_gpu_double_log_exp_agreement_with_host_libm():
+    """log/exp on the device differ from glibc only in rare last-bit cases; the sampler uses them only in comparisons
+    (lalpha against a log-uniform on a 2^24 grid, the gamma and truncated-normal acceptance tests), where a last-bit
+    difference changes a decision only if the two sides are within an ulp."""
+    libm = C.CDLL("libm.so.6")
+    libm.log.argtypes = [C.c_double]
+    libm.log.restype = C.c_double
+    libm.exp.argtypes = [C.c_double]
+    libm.exp.restype = C.c_double
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(1.0, 50.0, 50000), np.exp(rng.uniform(-20, 5, 50000))])
+    gl = np.array([libm.log(float(v)) for v in x])
+    dl = sampler.debug_math(2, d=x)
+    e = -0.5 * rng.uniform(0, 8, 50000) ** 2
+    ge = np.array([libm.exp(float(v)) for v in e])
+    de = sampler.debug_math(3, d=e)
+    mis_log = np.count_nonzero(gl != dl) / len(x)
+    mis_exp = np.count_nonzero(ge != de) / len(e)
+    print(f"log mismatch rate {mis_log:.2e}, exp mismatch rate {mis_exp:.2e}")
+    assert np.max(np.abs(gl - dl) / np.abs(gl + 1e-300)) < 4e-16
+    assert np.max(np.abs(ge - de) / ge) < 4e-16
+    assert mis_log < 0.01 and mis_exp < 0.01

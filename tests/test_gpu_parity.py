@@ -1,0 +1,115 @@
+"""GPU parity: the HIP sampler against the reference's trajectories and the oracle, bit for bit."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import same_bits
+from fastbart_amd import sampler
+from fastbart_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+GOLDEN = ["w37_p3", "w200_p3_extreme", "w1000_p16_ones", "w333_p1", "p300_probit", "w517_m1_deep",
+          "w4096_p16_short", "n8_w800_cap"]
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_gpu_matches_reference_trajectory(golden_cases, name):
+    g = golden_cases[name]
+    _say("case", name)
+    with sampler.BatchSampler([g["problem"]], g["m"], g["nskip"], g["ndpost"], [g["seed"]], probit=g["probit"],
+                              node_cap=g["nodes"]) as b:
+        b.run(g["nskip"] + g["ndpost"])
+        out = b.fetch(0)
+    e = g["expected"]
+    assert same_bits(out["sigma"], e["sigma"])
+    assert same_bits(out["trmean"], e["trmean"])
+    assert np.array_equal(out["stats"], e["stats"])
+    assert same_bits(out["trees"][:, :, :g["nodes"]], e["trees"])
+
+
+def test_gpu_dropin_bart_fit_cpu(golden_cases):
+    """The reference's own entry point (src/bart_kernel.h:63-66) on the device."""
+    g = golden_cases["w1000_p16_ones"]
+    out = sampler.bart_fit(g["problem"], g["m"], g["nskip"], g["ndpost"], g["seed"])
+    e = g["expected"]
+    for k in ("sigma", "trmean", "trees"):
+        assert same_bits(out[k], e[k]), k
+    assert np.array_equal(out["stats"], e["stats"])
+
+
+def test_gpu_batch_of_uneven_chains_matches_oracle(oracle):
+    """Chains of the 512-chain workload's shapes (uneven n and p) in one launch, each against the oracle."""
+    idx = [0, 5, 17, 200, 333, 511]
+    probs = [W.batch512_chain(c) for c in idx]
+    seeds = [1 + c for c in idx]
+    m, nskip, nd = 200, 5, 3
+    _say("batch chains", idx)
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds) as b:
+        b.run(nskip + nd)
+        outs = [b.fetch(c) for c in range(len(idx))]
+        yh = b.yhat(2)
+    for c, pb in enumerate(probs):
+        e = oracle.fit(pb, m, nskip, nd, seeds[c])
+        for k in ("sigma", "trmean", "trees", "stats"):
+            assert same_bits(outs[c][k], e[k]), (idx[c], k)
+    # per-draw training fits of one chain, summed in tree order (src/glue.cpp:52-56)
+    assert same_bits(yh, oracle.yhat(outs[2]["trees"], probs[2]["q"]))
+
+
+def test_gpu_state_after_partial_run_and_resume(oracle):
+    """Sampler state after k sweeps equals the oracle's (generator, sigma, running fit, leaf ids, trees); then the
+    remaining sweeps run in further launches and the outputs still agree."""
+    pb = W.small_problem(1500, 5, 30, 31, "uniform", ntree=30)
+    m, nskip, nd = 30, 12, 6
+    ch = oracle.chain(pb, m, nskip, nd, 11)
+    with sampler.BatchSampler([pb], m, nskip, nd, [11], sweeps_per_launch=4) as b:
+        b.run(7)
+        ch.sweeps(7)
+        s, e = b.state(0), ch.state()
+        assert np.array_equal(s["rng"], e["rng"])
+        assert s["sigma"] == e["sigma"]
+        assert same_bits(s["af"], e["af"])
+        assert np.array_equal(s["leaf"], e["leaf"])
+        assert np.array_equal(s["trees"], e["trees"])
+        b.run(nskip + nd - 7)
+        ch.sweeps(nskip + nd - 7)
+        out = b.fetch(0)
+    assert same_bits(out["sigma"], ch.sigma_out)
+    assert same_bits(out["trmean"], ch.trmean())
+    assert same_bits(out["trees"].reshape(-1), ch.trees_out)
+
+
+def test_gpu_probit_latents_match_oracle(oracle):
+    pb = W.small_problem(700, 3, 12, 41, probit=True, ntree=12)
+    m, nskip, nd = 12, 10, 5
+    ch = oracle.chain(pb, m, nskip, nd, 3, probit=1)
+    with sampler.BatchSampler([pb], m, nskip, nd, [3], probit=1) as b:
+        s0 = b.state(0)
+        assert same_bits(s0["latent"], ch.state()["latent"])     # initial latents
+        b.run(nskip + nd)
+        ch.sweeps(nskip + nd)
+        s = b.state(0)
+        assert same_bits(s["latent"], ch.state()["latent"])
+        out = b.fetch(0)
+    assert same_bits(out["trees"].reshape(-1), ch.trees_out)
+
+
+def test_gpu_fx64_mode_matches_oracle(oracle):
+    """Fixed-point statistics (order-independent sums) against the oracle's fixed-point twin."""
+    probs = [W.small_problem(900, 4, 25, 51 + k, "uniform", ntree=20) for k in range(3)]
+    m, nskip, nd = 20, 15, 5
+    with sampler.BatchSampler(probs, m, nskip, nd, [7, 8, 9], stats_mode=1) as b:
+        b.run(nskip + nd)
+        outs = [b.fetch(c) for c in range(3)]
+    for c, pb in enumerate(probs):
+        e = oracle.fit(pb, m, nskip, nd, 7 + c, stats_mode=1)
+        for k in ("sigma", "trmean", "trees", "stats"):
+            assert same_bits(outs[c][k], e[k]), (c, k)

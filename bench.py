@@ -1,0 +1,197 @@
+"""Benchmark: tree-updates/sec/GPU of the batched plain sampler, plus min-ESS/sec.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--chains 512]
+
+Workload: the batched fwbart configuration -- 512 independent chains in one persistent launch, chain c with
+n_c ~ U{3091..4573} rows, p_c ~ U{15..19} predictors, x ~ U(0,1) with 255 cutpoints, y = 10 sin(pi x1 x2) +
+20 (x3-0.5)^2 + 10 x4 + 5 x5 + N(0, s_c^2), s_c ~ U(0.5, 2), w ~ U(0.5, 2); ntree 200, nskip 1000, ndpost 100.
+A step is one sweep of every chain (200 tree updates per chain plus the closing pass and the sigma draw). W warmup
+sweeps run untimed, then exactly K sweeps are timed; W + K = 1100 covers the whole fit by default, so the kept draws of
+the timed region give the min-ESS figure. Inputs are resident in HBM before the timed region starts.
+
+One JSON line on stdout. Statistics mode REF16 (double sums in 16 row lanes, bit-identical to the reference CPU
+sampler). The CPU baseline is the reference's own core (oracle/_ref, built from the reference sources) timed on this
+host, one chain per process, on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "tree-updates/sec/GPU (batched chains) + min-ESS/sec, learned vs plain proposal"
+HBM_PEAK_GBS = 8000.0
+M, NSKIP, NDPOST = 200, 1000, 100
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------------ CPU baseline (runs before the GPU is touched)
+def _cpu_worker(args):
+    chain, sweeps = args
+    from fastbart_amd import workloads as W
+    from oracle import pyoracle
+    pb = W.batch512_chain(chain)
+    if pyoracle.RefCore.available():
+        core = pyoracle.RefCore()
+        t0 = time.perf_counter()
+        core.fit(pb, M, sweeps, 0, 1 + chain, keep_trees=0)
+    else:
+        core = pyoracle.Oracle()
+        t0 = time.perf_counter()
+        core.fit(pb, M, sweeps, 0, 1 + chain, keep_trees=0)
+    return M * sweeps, time.perf_counter() - t0
+
+
+def cpu_baseline(cores: int, chains_per_core: int = 3, sweeps: int = 40):
+    from oracle import pyoracle
+    kind = "reference" if pyoracle.RefCore.available() else "port"
+    if kind == "port" and not os.path.exists(pyoracle.ORACLE_SO):
+        pyoracle.build(ref=False)
+    jobs = [(c, sweeps) for c in range(cores * chains_per_core)]
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_cpu_worker, jobs, chunksize=1)
+    wall = time.perf_counter() - t0
+    updates = sum(r[0] for r in res)
+    cpu_s = sum(r[1] for r in res)
+    return dict(value=updates / wall, unit="tree-updates/s", cores=cores, kind=kind,
+                sample=f"{len(jobs)} chains of the 512-chain workload x {sweeps} sweeps x {M} trees, one chain per "
+                       f"process on {cores} cores ({cpu_s:.1f} CPU-s, {wall:.1f} s wall; "
+                       f"{updates / cpu_s:.0f} tree-updates/s per core)")
+
+
+# ------------------------------------------------------------------ ESS (Geyer initial monotone sequence)
+def ess_geyer(x: np.ndarray) -> float:
+    x = np.asarray(x, np.float64)
+    n = len(x)
+    if n < 4 or np.var(x) == 0:
+        return float(n)
+    xc = x - x.mean()
+    f = np.fft.rfft(xc, 2 * n)
+    acov = np.fft.irfft(f * np.conj(f))[:n] / n
+    rho = acov / acov[0]
+    pairs = []
+    for k in range(0, n - 1, 2):
+        s = rho[k] + rho[k + 1]
+        if s <= 0:
+            break
+        pairs.append(s)
+    pairs = np.minimum.accumulate(np.array(pairs)) if pairs else np.array([1.0])
+    tau = -1.0 + 2.0 * pairs.sum()
+    return float(n / max(tau, 1e-12))
+
+
+def min_ess_chain(sigma_kept: np.ndarray, yhat: np.ndarray, max_rows: int = 0) -> float:
+    cols = range(yhat.shape[1]) if not max_rows else np.linspace(0, yhat.shape[1] - 1, max_rows).astype(int)
+    e = [ess_geyer(sigma_kept)] + [ess_geyer(yhat[:, i]) for i in cols]
+    return float(min(e))
+
+
+# ------------------------------------------------------------------ main
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed sweeps (default: 1100 - warmup)")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--chains", type=int, default=512)
+    ap.add_argument("--sweeps-per-launch", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ess-rows", type=int, default=0, help="rows per chain used for ESS (0 = all)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build runs every chain on a single MI355X "
+              "(chains shard across GPUs with no exchange; see the design notes). Use --gpus 1.", file=sys.stderr)
+        return 2
+    W_ = max(0, args.warmup)
+    K = args.steps if args.steps is not None else NSKIP + NDPOST - W_
+    total = W_ + K
+    nskip = max(0, total - NDPOST) if total >= NDPOST else 0
+    ndpost = total - nskip
+
+    cpu = None
+    if not args.no_cpu:
+        cores = min(16, len(os.sched_getaffinity(0)))
+        log(f"cpu baseline on {cores} cores ...")
+        cpu = cpu_baseline(cores)
+        log("cpu baseline:", cpu)
+
+    from fastbart_amd import sampler
+    from fastbart_amd import workloads as Wk
+
+    t0 = time.perf_counter()
+    probs = [Wk.batch512_chain(c) for c in range(args.chains)]
+    seeds = [1 + c for c in range(args.chains)]
+    log(f"generated {args.chains} chains in {time.perf_counter() - t0:.1f} s; total rows "
+        f"{sum(p['q'].shape[1] for p in probs)}")
+    b = sampler.BatchSampler(probs, M, nskip, ndpost, seeds, keep_trees=1,
+                             sweeps_per_launch=args.sweeps_per_launch)
+    b.sync()
+    t_w = time.perf_counter()
+    if W_:
+        b.run(W_)
+        b.sync()
+    t_w = time.perf_counter() - t_w
+    log(f"warmup {W_} sweeps: {t_w:.2f} s")
+    _, nl0, _ = b.timing()
+    t1 = time.perf_counter()
+    b.run(K)
+    b.sync()
+    T = time.perf_counter() - t1
+    avg_ms, nl, by = b.timing()
+    ntu = args.chains * M * K
+    value = ntu / T
+    log(f"timed {K} sweeps: {T:.3f} s -> {value / 1e6:.3f} M tree-updates/s; kernel avg {avg_ms:.3f} ms over {nl} "
+        f"launches")
+
+    # roofline of the sweep kernel: algorithmic bytes (37 per row per tree update, x (m+1)/m) / kernel time
+    rows_total = sum(p["q"].shape[1] for p in probs)
+    bytes_per_sweep = 37.0 * rows_total * (M + 1)
+    spl = b.sweeps_per_launch if hasattr(b, "sweeps_per_launch") else None
+    achieved = bytes_per_sweep * total / (avg_ms * nl / 1e3) / 1e9 if nl else 0.0
+
+    ess = None
+    if ndpost >= 10:
+        t2 = time.perf_counter()
+        tot = 0.0
+        for c in range(args.chains):
+            out = b.fetch(c)
+            yh = b.yhat(c)
+            tot += min_ess_chain(out["sigma"][nskip:], yh, args.ess_rows)
+        ess = tot / (t_w + T)
+        log(f"min-ESS: sum over chains {tot:.1f} ({time.perf_counter() - t2:.1f} s to compute)")
+    b.close()
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "tree-updates/s", "n_gpus": 1, "steps": K, "warmup": W_,
+        "ms_per_step": T / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batched fwbart, {args.chains} chains, n_i~U{{3091..4573}}, p_i~U{{15..19}}, "
+                               f"255 cutpoints, ntree {M}, nskip {nskip}, ndpost {ndpost}, REF16 statistics",
+                   "chains": args.chains, "ntree": M, "rows_total": rows_total, "parallelism": "chains"},
+        "min_ess_per_sec": ess,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_sweeps", "kernel_avg_ms": avg_ms, "launches": nl,
+                     "bytes_per_tree_update_row": 37},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

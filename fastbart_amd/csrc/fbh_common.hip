@@ -98,13 +98,13 @@ __global__ void k_debug_math(int kind, const uint32_t* in_u, const double* in_d,
   switch (kind) {
     case 0: {
       const uint32_t a = in_u[2 * i] >> 8, b = in_u[2 * i + 1] >> 8;
-      out_f[i] = __fsqrt_rn(-2.f * tb.logf_t[a]) * tb.cosf_t[b];
+      out_f[i] = __builtin_sqrtf(-2.f * tb.logf_t[a]) * tb.cosf_t[b];
       break;
     }
-    case 1: out_f[i] = __fsqrt_rn(-2.f * tb.logf_t[in_u[i] >> 8]); break;
+    case 1: out_f[i] = __builtin_sqrtf(-2.f * tb.logf_t[in_u[i] >> 8]); break;
     case 2: out_d[i] = log(in_d[i]); break;
     case 3: out_d[i] = exp(in_d[i]); break;
-    case 4: out_d[i] = __dsqrt_rn(in_d[i]); break;
+    case 4: out_d[i] = __builtin_sqrt(in_d[i]); break;
     case 5: out_d[i] = in_d[2 * i] / in_d[2 * i + 1]; break;
     default: break;
   }

@@ -2,15 +2,21 @@
 //
 // Per tree update (src/bart_kernel.cpp:400-535): proposal, one data pass over the chain's rows (apply the previous
 // tree's accepted change to the leaf ids and the running fit, then this tree's sufficient statistics), the
-// Metropolis-Hastings decision, and the leaf draws. After the last tree, a closing pass (apply, residual sum of squares,
+// Metropolis-Hastings decision and the leaf draws. After the last tree, a closing pass (apply, residual sum of squares,
 // posterior-mean sum) and the sigma draw or the probit latent redraw (:536-558).
 //
-// Data pass (the hot loop, :133-213): rows are staged through LDS in tiles. Phase A is row-parallel over all threads
-// (coalesced reads of the u8 leaf-id and cutpoint-rank columns, the running fit, y and w; leaf-id rewrite) and writes
-// per-row w, w*r and a tag (leaf rank | birth-split side). Phase B reduces per (lane, leaf): in REF16 mode each lane
-// i % 16 is summed in row order by one thread with a branch-free fused multiply-add by 1.0 or 0.0 (adding +0.0 to a sum
-// that started at +0.0 leaves it unchanged, so the order and rounding are exactly the reference's lane sums), then the
-// 16 lanes are added 0..15. In FX64 mode per-row int64 fixed-point values are added with LDS atomics (order free).
+// Workgroup = 1 control wave + 4 data waves.
+//  * Data wave w owns summation lanes 4w..4w+3 (row i belongs to lane i % 16, src/bart_kernel.cpp:354-356). It runs
+//    phase A (apply + per-row w, w*r and a tag = leaf rank | birth-split side) for the rows of its own lanes and then
+//    phase B, the ordered per-(lane, leaf) sums: one thread per (lane, task) walks its lane's rows in row order with a
+//    fused multiply-add by 1.0 or 0.0 (adding +0.0 to a sum that started at +0.0 changes nothing, so these are exactly
+//    the reference's lane sums). A wave only ever reads rows it wrote itself, so the pass needs no workgroup barrier.
+//    In FX64 mode the same walk adds int64 fixed-point images instead (order free, exact).
+//  * The control wave overlaps the data pass of tree j with: the 1 + 2*NODES generator outputs of tree j's MH test and
+//    leaf normals, the libm-image gathers for the slots that can be leaves after the MH step, and the proposal of tree
+//    j+1 (whose draws follow those in the stream). After the pass it adds the 16 lanes, runs MH and the leaf draws.
+//  * The running fit of a chain stays in LDS across the whole launch when it fits (RES); leaf ids, cutpoint ranks, y and
+//    w stream from HBM / L2 with all of a tile's loads issued before use.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -20,16 +26,30 @@
 
 namespace fbh {
 
-constexpr int kThreads = 256;
-constexpr int kTile = 2048;  // rows per LDS tile (multiple of 16)
-constexpr int kGroups = (kMaxRank + kThreads / kLanes - 1) / (kThreads / kLanes);  // rank groups per phase-B thread
+constexpr int kDataWaves = 4;
+constexpr int kBlock = 64 * (1 + kDataWaves);
+constexpr int kLanesPerWave = kLanes / kDataWaves;   // 4 summation lanes per data wave
+constexpr int kTaskSlots = 64 / kLanesPerWave;       // 16 task threads per lane
+constexpr int kMaxTasks = 2 * kMaxRank;              // (sum of w, sum of w*r) for every leaf and split half
+constexpr int kGroups = (kMaxTasks + kTaskSlots - 1) / kTaskSlots;
+
+// Tile geometry: R rows per phase-A thread per tile, 16 R rows per lane, 256 R rows per tile.
+template <int R>
+struct Geo {
+  static constexpr int kTW4 = 16 * R;                         // rows of one lane in a tile
+  static constexpr int kTile = kLanes * kTW4;                 // rows in a tile
+  static constexpr int kStride = kTW4 + 1;                    // doubles per lane row block (bank offset)
+  static constexpr int kTagStride = (kTW4 + 4) & ~3;          // bytes per lane tag block
+  static constexpr int kScratchBytes = 2 * kLanes * kStride * 8 + kLanes * kTagStride;
+};
 
 struct KArgs {
   const ChainDesc* desc;
   ChainState* state;
   TreeDev* trees;
   const uint8_t* q;
-  double* Y;
+  double* Y;           // probit latents
+  const float* yf;     // wbart outcome
   const float* W;
   double* AF;
   double* TM;
@@ -43,11 +63,14 @@ struct KArgs {
   int trace_cap;  // records per chain
   int m, nskip, ndpost, probit, keep_trees, stats_mode, node_cap;
   int it0, it1;
+  int af_off;     // byte offset of the resident running fit in dynamic LDS
 };
 
 struct Rng {
   uint32_t s0, s1, s2, s3;
 };
+
+__device__ __forceinline__ uint32_t ufl(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
 // xoshiro128** (Blackman & Vigna), state order of src/bart_kernel.cpp:54-58
 __device__ __forceinline__ uint32_t rng_next(Rng& g) {
@@ -64,9 +87,11 @@ __device__ __forceinline__ uint32_t rng_next(Rng& g) {
 }
 // (k + 1/2) 2^-24 from the top 24 bits (:59)
 __device__ __forceinline__ float unif_f(uint32_t x) { return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f); }
-// sqrtf(-2 logf(u1)) cosf(2 pi u2) in float with the host libm images (:60)
+// sqrtf(-2 logf(u1)) cosf(2 pi u2) in float with the host libm images (:60). __builtin_sqrtf / __builtin_sqrt lower to
+// correctly rounded sequences (v_sqrt + FMA residual correction); HIP's __fsqrt_rn is the native approximation unless
+// OCML_BASIC_ROUNDED_OPERATIONS is defined, so it is not used anywhere.
 __device__ __forceinline__ float normal_f(uint32_t x1, uint32_t x2, const LibmTables& tb) {
-  return __fsqrt_rn(-2.f * tb.logf_t[x1 >> 8]) * tb.cosf_t[x2 >> 8];
+  return __builtin_sqrtf(-2.f * tb.logf_t[x1 >> 8]) * tb.cosf_t[x2 >> 8];
 }
 __device__ __forceinline__ double unif_d(Rng& g) { return (double)unif_f(rng_next(g)); }
 __device__ __forceinline__ double rnorm_d(Rng& g, const LibmTables& tb) {
@@ -80,7 +105,7 @@ __device__ __forceinline__ double log_unif(Rng& g, const LibmTables& tb) { retur
 // Marsaglia-Tsang gamma, shape >= 1 (:73-83)
 __device__ double rgamma_dev(Rng& g, double a, const LibmTables& tb) {
   const double d = a - 1.0 / 3.0;
-  const double c = 1.0 / __dsqrt_rn(9.0 * d);
+  const double c = 1.0 / __builtin_sqrt(9.0 * d);
   for (int tries = 0; tries < 32; tries++) {
     const double x = rnorm_d(g, tb);
     double v = 1.0 + c * x;
@@ -102,7 +127,7 @@ __device__ double rtnorm_dev(Rng& g, double mean, double tau, const LibmTables& 
       z = rnorm_d(g, tb);
     } while (z < tau);
   } else {
-    const double lam = 0.5 * (tau + __dsqrt_rn(tau * tau + 4.0));
+    const double lam = 0.5 * (tau + __builtin_sqrt(tau * tau + 4.0));
     for (;;) {
       const double e = -log_unif(g, tb);
       z = e / lam + tau;
@@ -113,42 +138,50 @@ __device__ double rtnorm_dev(Rng& g, double mean, double tau, const LibmTables& 
   return z + mean;
 }
 
-// ------------------------------------------------------------------ shared memory of one chain's workgroup
-struct Smem {
-  TreeDev T;
+// ------------------------------------------------------------------ control state of one chain's workgroup
+struct Ctl {
+  TreeDev tree[2];               // tree j (being updated) and tree j+1 (proposal prepared during pass j)
   ChainDesc cd;
   double PG[kNodes];
-  double thc[kNodes], tho[kNodes], thn[kNodes];
+  double thc[2][kNodes];         // leaf values of the tree whose statistics the pass computes
+  double tho[kNodes], thn[kNodes];  // previous tree's values before / after its update (applied by the next pass)
   double S0[kNodes], S1[kNodes];
-  double part0[kMaxRank][kLanes], part1[kMaxRank][kLanes];
-  unsigned long long fx0[kNodes], fx1[kNodes], fxs[4], fxrss;
-  uint32_t rnd[132];
-  uint8_t rank_of[kNodes], slot_of[kNodes];
+  double part[kMaxTasks][kLanes];   // per-lane sums (double bits, or int64 in FX64 mode)
+  uint64_t cand[2];              // slots that can be leaves after the MH step
+  double prr[2];
+  int cmode[2], cnx[2], cv[2], cc[2], nleaf[2], ngood[2], nnogs[2];
+  uint8_t rank_of[2][kNodes], slot_of[2][kNodes];
+  uint32_t rnd[2 * kNodes + 4];  // generator outputs of the MH uniform and the leaf normals
+  uint32_t rng_mark[4];          // generator state right after them (trace records)
+  float z[kNodes];
+  double lu;
   int pmode, pnx, pv, pc, pL, pR, pl, pr;
-  int cmode, cnx, cv, cc;
-  int nleaf, nL, nR;
-  double prr, inv_s2, sL0, sL1, sR0, sR1, rss;
-  double wsc[kTile], wrsc[kTile];
-  uint8_t tag[kTile];
+  int nL, nR;
+  double sL0, sL1, sR0, sR1, rss;
+  double inv_s2;
 };
 
 #define BITM(mask, i) (((mask) >> (i)) & 1ull)
 
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// index of the k-th set bit (0-based); 0 if there are not k+1 bits (:94-101)
 __device__ __forceinline__ int select_kth(uint64_t m, int k) {
-  int seen = 0;
-  for (int i = 0; i < kNodes; i++)
-    if (BITM(m, i)) {
-      if (seen == k) return i;
-      seen++;
-    }
-  return 0;
+  for (int i = 0; i < k && m; i++) m &= m - 1;
+  return m ? __builtin_ctzll(m) : 0;
 }
+// lowest unused slot in [1, cap) other than skip, or -1 (:102-109)
 __device__ __forceinline__ int first_free(uint64_t used, int skip, int cap) {
-  for (int i = 1; i < cap; i++)
-    if (!BITM(used, i) && i != skip) return i;
-  return -1;
+  uint64_t f = ~used & ~1ull;
+  if (cap < 64) f &= (1ull << cap) - 1ull;
+  if (skip >= 0) f &= ~(1ull << skip);
+  return f ? __builtin_ctzll(f) : -1;
 }
-// BART's tree::rg (:238-251)
+// BART's tree::rg (:238-251), uniform walk
 __device__ void node_range(const TreeDev& t, int x, int v, int nc, int* L, int* U) {
   int lo = 0, hi = nc - 1, node = x;
   for (int s = 0; s < kMaxDepth && node != 0; s++) {
@@ -165,282 +198,26 @@ __device__ void node_range(const TreeDev& t, int x, int v, int nc, int* L, int* 
   *L = lo;
   *U = hi;
 }
-// variables with a non-empty range at node x (:253-276)
-__device__ int count_goodvars(const TreeDev& t, int x, const int* nc, int p) {
-  int lo[FBH_PMAX], hi[FBH_PMAX];
-  for (int v = 0; v < FBH_PMAX; v++) {
-    lo[v] = 0;
-    hi[v] = nc[v] - 1;
-  }
+// variables with a non-empty range at node x (:253-276): lane v tracks variable v along the shared ancestor walk
+__device__ int count_goodvars_wave(const TreeDev& t, int x, const int* nc, int p, int lane) {
+  int lo = 0, hi = (lane < FBH_PMAX ? nc[lane] : 0) - 1;
   int node = x;
   for (int s = 0; s < kMaxDepth && node != 0; s++) {
     const int pa = t.par[node], v = t.v[pa], c = t.c[pa];
-    if ((int)t.l[pa] == node) {
-      if (c - 1 < hi[v]) hi[v] = c - 1;
-    } else {
-      if (c + 1 > lo[v]) lo[v] = c + 1;
+    if (v == lane) {
+      if ((int)t.l[pa] == node) {
+        if (c - 1 < hi) hi = c - 1;
+      } else {
+        if (c + 1 > lo) lo = c + 1;
+      }
     }
     node = pa;
   }
-  int gcount = 0;
-  for (int v = 0; v < p; v++) gcount += hi[v] >= lo[v];
-  return gcount;
+  return __popcll(__ballot(lane < p && hi >= lo));
 }
 __device__ __forceinline__ double lh(double b, double M, double t2) {
   const double k = b * t2 + 1;
   return -0.5 * log(k) + 0.5 * M * M * t2 / k;
-}
-
-// Birth/death proposal for the tree in sm.T (:413-464). Runs on one thread.
-__device__ void propose(Smem& sm, Rng& g, int& ngood_out, int& nnogs_out) {
-  const TreeDev& t = sm.T;
-  const double pb = 0.5;
-  const int p = sm.cd.p;
-  uint64_t goodm = 0, nogm = 0;
-  for (int s = 0; s < kNodes; s++) {
-    const bool isleaf = BITM(t.leaf, s), isused = BITM(t.used, s);
-    if (isused && isleaf && t.ngv[s] > 0) goodm |= 1ull << s;
-    if (isused && !isleaf && BITM(t.leaf, t.l[s]) && BITM(t.leaf, t.r[s])) nogm |= 1ull << s;
-  }
-  const int ngood = __popcll(goodm), nnogs = __popcll(nogm);
-  ngood_out = ngood;
-  nnogs_out = nnogs;
-  const bool single = t.used == 1ull;
-  const double PBx = ngood == 0 ? 0 : (single ? 1 : pb);
-  int cmode = 0, cnx = 0, cv = 0, cc = 0;
-  double pr = 0;
-  if (unif_d(g) < PBx) {
-    const int nx = select_kth(goodm, (int)(unif_d(g) * ngood));
-    int v = (int)(unif_d(g) * p);
-    if (v >= p) v = p - 1;
-    int bL, bU;
-    node_range(t, nx, v, sm.cd.nc[v], &bL, &bU);
-    if (bU >= bL && (int)t.d[nx] + 1 < kMaxDepth) {
-      int c = bL + (int)(unif_d(g) * (bU - bL + 1));
-      if (c > bU) c = bU;
-      const int dnx = t.d[nx];
-      const double PGnx = sm.PG[dnx];
-      double PGly, PGry;
-      if (t.ngv[nx] > 1) {
-        PGly = PGry = sm.PG[dnx + 1];
-      } else {
-        PGly = (c - 1 < bL) ? 0 : sm.PG[dnx + 1];
-        PGry = (bU < c + 1) ? 0 : sm.PG[dnx + 1];
-      }
-      const double PDy = (ngood > 1) ? 1 - pb : ((PGry == 0 && PGly == 0) ? 1 : 1 - pb);
-      double Pnogy;
-      if (nx == 0) {
-        Pnogy = 1;
-      } else {
-        const int pa = t.par[nx];
-        const bool panog = BITM(t.leaf, t.l[pa]) && BITM(t.leaf, t.r[pa]);
-        Pnogy = panog ? 1.0 / nnogs : 1.0 / (nnogs + 1);
-      }
-      const double Pbotx = 1.0 / ngood;
-      pr = (PGnx * (1 - PGly) * (1 - PGry) * PDy * Pnogy) / ((1 - PGnx) * Pbotx * PBx);
-      cmode = 1;
-      cnx = nx;
-      cv = v;
-      cc = c;
-    }
-  } else {
-    const int nx = select_kth(nogm, (int)(unif_d(g) * nnogs));
-    const int dny = t.d[nx], l = t.l[nx], r = t.r[nx];
-    const double PGny = sm.PG[dny];
-    const double PGlx = t.ngv[l] > 0 ? sm.PG[dny + 1] : 0, PGrx = t.ngv[r] > 0 ? sm.PG[dny + 1] : 0;
-    const double PBy = (nx == 0) ? 1 : pb;
-    const int ng2 = ngood - (t.ngv[l] > 0) - (t.ngv[r] > 0) + 1;
-    const double Pboty = 1.0 / ng2, PDx = 1 - PBx, Pnogx = 1.0 / nnogs;
-    pr = ((1 - PGny) * PBy * Pboty) / (PGny * (1 - PGlx) * (1 - PGrx) * PDx * Pnogx);
-    cmode = 2;
-    cnx = nx;
-  }
-  sm.cmode = cmode;
-  sm.cnx = cnx;
-  sm.cv = cv;
-  sm.cc = cc;
-  sm.prr = pr;
-}
-
-__device__ __forceinline__ long long fx_round(double x, double scale) { return (long long)rint(x * scale); }
-
-// One pass over the chain's rows (:133-213). All threads of the workgroup call it; it ends synchronised.
-__device__ void data_pass(const KArgs& a, Smem& sm, double* AF, double* TM, const double* Y, const float* W,
-                          const uint8_t* Q, uint8_t* LEAF, int jp, int j, bool apply, bool stats, bool rss,
-                          bool keep) {
-  const int tid = threadIdx.x;
-  const int n = sm.cd.n;
-  const bool fx = a.stats_mode == FBH_STATS_FX64;
-  const int pmode = apply ? sm.pmode : 0, pnx = sm.pnx, pc = sm.pc, pL = sm.pL, pR = sm.pR, pl = sm.pl, pr = sm.pr;
-  const int cmode = stats ? sm.cmode : 0, cnx = sm.cnx, cc = sm.cc;
-  const int nleaf = sm.nleaf;
-  const int ntask = rss ? 1 : nleaf + (cmode == 1 ? 2 : 0);
-  uint8_t* lprev = apply ? LEAF + (size_t)jp * n : nullptr;
-  const uint8_t* lcur = stats ? LEAF + (size_t)j * n : nullptr;
-  const uint8_t* qp = (apply && pmode == 1) ? Q + (size_t)sm.pv * n : nullptr;
-  const uint8_t* qc = (stats && cmode == 1) ? Q + (size_t)sm.cv * n : nullptr;
-
-  if (tid < kNodes) {
-    sm.S0[tid] = 0;
-    sm.S1[tid] = 0;
-    sm.fx0[tid] = 0;
-    sm.fx1[tid] = 0;
-  }
-  if (tid < 4) sm.fxs[tid] = 0;
-  if (tid == 0) {
-    sm.nL = sm.nR = 0;
-    sm.fxrss = 0;
-    sm.sL0 = sm.sL1 = sm.sR0 = sm.sR1 = 0;
-    sm.rss = 0;
-  }
-  // phase-B task of this thread: lane k, ranks qs + 16 g
-  const int k = tid & (kLanes - 1), qs = tid / kLanes;
-  double a0[kGroups], a1[kGroups];
-  int qmask[kGroups], qval[kGroups];
-  bool qon[kGroups];
-#pragma unroll
-  for (int g = 0; g < kGroups; g++) {
-    const int q = qs + (kThreads / kLanes) * g;
-    a0[g] = 0;
-    a1[g] = 0;
-    qon[g] = !fx && q < ntask;
-    if (rss) {
-      qmask[g] = 0;
-      qval[g] = 0;
-    } else if (q < nleaf) {
-      qmask[g] = 63;
-      qval[g] = q;
-    } else {
-      qmask[g] = 0xC0;
-      qval[g] = q == nleaf ? 0x40 : 0x80;
-    }
-  }
-  __syncthreads();
-
-  for (int tb = 0; tb < n; tb += kTile) {
-    const int rows = min(kTile, n - tb);
-    int my_nl = 0, my_nr = 0;
-    // ---- phase A: row-parallel apply + per-row values
-    for (int rr = tid; rr < rows; rr += kThreads) {
-      const int i = tb + rr;
-      double af = AF[i];
-      if (apply) {
-        const int lf = lprev[i];
-        int lf2 = lf;
-        if (pmode == 1 && lf == pnx)
-          lf2 = qp[i] <= pc ? pL : pR;
-        else if (pmode == 2 && (lf == pl || lf == pr))
-          lf2 = pnx;
-        af = af - sm.tho[lf] + sm.thn[lf2];
-        lprev[i] = (uint8_t)lf2;
-        AF[i] = af;
-      }
-      if (rss) {
-        const double e = Y[i] - af;
-        const double c2 = ((double)W[i] * e) * e;
-        if (fx) {
-          atomicAdd(&sm.fxrss, (unsigned long long)fx_round(c2, 16777216.0));
-        } else {
-          sm.wsc[rr] = 0.0;
-          sm.wrsc[rr] = c2;
-          sm.tag[rr] = 0;
-        }
-        if (keep) TM[i] += af;
-      } else if (stats) {
-        const int lf = lcur[i];
-        const double r = Y[i] - af + sm.thc[lf];
-        const double w = (double)W[i];
-        const double wr = w * r;
-        int side = 0;
-        if (cmode == 1 && lf == cnx) {
-          side = qc[i] <= cc ? 1 : 2;
-          my_nl += side == 1;
-          my_nr += side == 2;
-        }
-        if (fx) {
-          const unsigned long long iw = (unsigned long long)fx_round(w, 4294967296.0);
-          const unsigned long long iwr = (unsigned long long)fx_round(wr, 4294967296.0);
-          atomicAdd(&sm.fx0[lf], iw);
-          atomicAdd(&sm.fx1[lf], iwr);
-          if (side) {
-            atomicAdd(&sm.fxs[side == 1 ? 0 : 2], iw);
-            atomicAdd(&sm.fxs[side == 1 ? 1 : 3], iwr);
-          }
-        } else {
-          sm.wsc[rr] = w;
-          sm.wrsc[rr] = wr;
-          sm.tag[rr] = (uint8_t)(sm.rank_of[lf] | (side << 6));
-        }
-      }
-    }
-    if (my_nl) atomicAdd(&sm.nL, my_nl);
-    if (my_nr) atomicAdd(&sm.nR, my_nr);
-    __syncthreads();
-    // ---- phase B: lane-ordered sums
-    if (!fx) {
-#pragma unroll
-      for (int g = 0; g < kGroups; g++) {
-        if (!qon[g]) continue;
-        const int mk = qmask[g], vl = qval[g];
-        double x0 = a0[g], x1 = a1[g];
-        for (int rr = k; rr < rows; rr += kLanes) {
-          const double mm = ((int)(sm.tag[rr] & mk) == vl) ? 1.0 : 0.0;
-          x0 = fma(mm, sm.wsc[rr], x0);
-          x1 = fma(mm, sm.wrsc[rr], x1);
-        }
-        a0[g] = x0;
-        a1[g] = x1;
-      }
-    }
-    __syncthreads();
-  }
-
-  if (!fx) {
-#pragma unroll
-    for (int g = 0; g < kGroups; g++)
-      if (qon[g]) {
-        const int q = qs + (kThreads / kLanes) * g;
-        sm.part0[q][k] = a0[g];
-        sm.part1[q][k] = a1[g];
-      }
-    __syncthreads();
-    if (tid < ntask) {
-      const int q = tid;
-      double x0 = 0, x1 = 0;
-      for (int kk = 0; kk < kLanes; kk++) {
-        x0 += sm.part0[q][kk];
-        x1 += sm.part1[q][kk];
-      }
-      if (rss) {
-        sm.rss = x1;
-      } else if (q < nleaf) {
-        const int s = sm.slot_of[q];
-        sm.S0[s] = x0;
-        sm.S1[s] = x1;
-      } else if (q == nleaf) {
-        sm.sL0 = x0;
-        sm.sL1 = x1;
-      } else {
-        sm.sR0 = x0;
-        sm.sR1 = x1;
-      }
-    }
-  } else {
-    if (tid < kNodes) {
-      sm.S0[tid] = (double)(long long)sm.fx0[tid] * (1.0 / 4294967296.0);
-      sm.S1[tid] = (double)(long long)sm.fx1[tid] * (1.0 / 4294967296.0);
-    }
-    if (tid == 0) {
-      if (cmode == 1) {
-        sm.sL0 = (double)(long long)sm.fxs[0] * (1.0 / 4294967296.0);
-        sm.sL1 = (double)(long long)sm.fxs[1] * (1.0 / 4294967296.0);
-        sm.sR0 = (double)(long long)sm.fxs[2] * (1.0 / 4294967296.0);
-        sm.sR1 = (double)(long long)sm.fxs[3] * (1.0 / 4294967296.0);
-      }
-      if (rss) sm.rss = (double)(long long)sm.fxrss * (1.0 / 16777216.0);
-    }
-  }
-  __syncthreads();
 }
 
 __device__ __forceinline__ uint64_t pack_word(const TreeDev& t, int s) {
@@ -449,32 +226,375 @@ __device__ __forceinline__ uint64_t pack_word(const TreeDev& t, int s) {
   return vv | ((uint64_t)t.c[s] << 8) | ((uint64_t)t.l[s] << 16) | ((uint64_t)t.r[s] << 24) | ((uint64_t)thb << 32);
 }
 
-__global__ __launch_bounds__(kThreads) void k_sweeps(KArgs a) {
-  __shared__ Smem sm;
-  const int tid = threadIdx.x;
+__device__ __forceinline__ void copy_tree(TreeDev* dst, const TreeDev* src, int lane) {
+  const uint64_t* s = reinterpret_cast<const uint64_t*>(src);
+  uint64_t* d = reinterpret_cast<uint64_t*>(dst);
+  for (int w = lane; w < (int)(sizeof(TreeDev) / 8); w += 64) d[w] = s[w];
+}
+
+// ------------------------------------------------------------------ control wave pieces
+// Proposal for tree buffer b (:413-464). All 64 lanes run it (lane s looks at slot s); the generator state and every
+// decision are wave-uniform.
+__device__ void ctl_propose(Ctl& c, int b, Rng& g, int cap, int lane) {
+  const TreeDev& t = c.tree[b];
+  const double pb = 0.5;
+  const int p = c.cd.p;
+  const uint64_t used = t.used, leafm = t.leaf;
+  const int s = lane;
+  const bool isleaf = BITM(leafm, s), isused = BITM(used, s);
+  const bool good = isused && isleaf && t.ngv[s] > 0;
+  const bool nog = isused && !isleaf && BITM(leafm, t.l[s] & 63) && BITM(leafm, t.r[s] & 63);
+  const uint64_t goodm = __ballot(good), nogm = __ballot(nog);
+  const uint64_t lm = used & leafm;
+  c.thc[b][s] = (double)t.th[s];
+  if (BITM(lm, s)) {
+    const int rk = __popcll(lm & ((1ull << s) - 1ull));
+    c.rank_of[b][s] = (uint8_t)rk;
+    c.slot_of[b][rk] = (uint8_t)s;
+  } else {
+    c.rank_of[b][s] = 0;
+  }
+  const int ngood = __popcll(goodm), nnogs = __popcll(nogm);
+  const bool single = used == 1ull;
+  const double PBx = ngood == 0 ? 0 : (single ? 1 : pb);
+  int cmode = 0, cnx = 0, cv = 0, cc = 0;
+  double pr = 0;
+  uint64_t cand = lm;
+  if (unif_d(g) < PBx) {
+    const int nx = select_kth(goodm, (int)(unif_d(g) * ngood));
+    int v = (int)(unif_d(g) * p);
+    if (v >= p) v = p - 1;
+    int bL, bU;
+    node_range(t, nx, v, c.cd.nc[v], &bL, &bU);
+    if (bU >= bL && (int)t.d[nx] + 1 < kMaxDepth) {
+      int cc_ = bL + (int)(unif_d(g) * (bU - bL + 1));
+      if (cc_ > bU) cc_ = bU;
+      const int dnx = t.d[nx];
+      const double PGnx = c.PG[dnx];
+      double PGly, PGry;
+      if (t.ngv[nx] > 1) {
+        PGly = PGry = c.PG[dnx + 1];
+      } else {
+        PGly = (cc_ - 1 < bL) ? 0 : c.PG[dnx + 1];
+        PGry = (bU < cc_ + 1) ? 0 : c.PG[dnx + 1];
+      }
+      const double PDy = (ngood > 1) ? 1 - pb : ((PGry == 0 && PGly == 0) ? 1 : 1 - pb);
+      double Pnogy;
+      if (nx == 0) {
+        Pnogy = 1;
+      } else {
+        const int pa = t.par[nx];
+        const bool panog = BITM(leafm, t.l[pa] & 63) && BITM(leafm, t.r[pa] & 63);
+        Pnogy = panog ? 1.0 / nnogs : 1.0 / (nnogs + 1);
+      }
+      const double Pbotx = 1.0 / ngood;
+      pr = (PGnx * (1 - PGly) * (1 - PGry) * PDy * Pnogy) / ((1 - PGnx) * Pbotx * PBx);
+      cmode = 1;
+      cnx = nx;
+      cv = v;
+      cc = cc_;
+      const int L = first_free(used, -1, cap), R = L < 0 ? -1 : first_free(used, L, cap);
+      if (L >= 0 && R >= 0) cand |= (1ull << L) | (1ull << R);
+    }
+  } else {
+    const int nx = select_kth(nogm, (int)(unif_d(g) * nnogs));
+    const int dny = t.d[nx], l = t.l[nx], r = t.r[nx];
+    const double PGny = c.PG[dny];
+    const double PGlx = t.ngv[l] > 0 ? c.PG[dny + 1] : 0, PGrx = t.ngv[r] > 0 ? c.PG[dny + 1] : 0;
+    const double PBy = (nx == 0) ? 1 : pb;
+    const int ng2 = ngood - (t.ngv[l] > 0) - (t.ngv[r] > 0) + 1;
+    const double Pboty = 1.0 / ng2, PDx = 1 - PBx, Pnogx = 1.0 / nnogs;
+    pr = ((1 - PGny) * PBy * Pboty) / (PGny * (1 - PGlx) * (1 - PGrx) * PDx * Pnogx);
+    cmode = 2;
+    cnx = nx;
+    cand |= 1ull << nx;
+  }
+  if (lane == 0) {
+    c.cmode[b] = cmode;
+    c.cnx[b] = cnx;
+    c.cv[b] = cv;
+    c.cc[b] = cc;
+    c.prr[b] = pr;
+    c.cand[b] = cand;
+    c.nleaf[b] = __popcll(lm);
+    c.ngood[b] = ngood;
+    c.nnogs[b] = nnogs;
+  }
+}
+
+// The generator outputs of this update's MH uniform and its NODES leaf normals (:482, :521-523), then the gathers of
+// the libm images for the slots that can be leaves afterwards.
+__device__ void ctl_rng_and_gather(Ctl& c, int b, Rng& g, int cap, const LibmTables& tb, int lane) {
+  const int nout = 1 + 2 * cap;
+  for (int k = 0; k < nout; k++) {
+    const uint32_t x = rng_next(g);
+    if (lane == 0) c.rnd[k] = x;
+  }
+  if (lane == 0) {
+    c.rng_mark[0] = g.s0;
+    c.rng_mark[1] = g.s1;
+    c.rng_mark[2] = g.s2;
+    c.rng_mark[3] = g.s3;
+  }
+  wave_sync();
+  if (lane == 0) c.lu = tb.logd_t[c.rnd[0] >> 8];
+  if (lane < cap && BITM(c.cand[b], lane)) c.z[lane] = normal_f(c.rnd[1 + 2 * lane], c.rnd[2 + 2 * lane], tb);
+}
+
+// Add the 16 lanes of every task in lane order from zero (:202-212).
+template <bool FX>
+__device__ void ctl_combine(Ctl& c, int b, bool rss, int cmode, int lane) {
+  const int nleaf = c.nleaf[b];
+  const int ntask = rss ? 2 : 2 * (nleaf + (cmode == 1 ? 2 : 0));
+  c.S0[lane] = 0;
+  c.S1[lane] = 0;
+  if (lane == 0) c.sL0 = c.sL1 = c.sR0 = c.sR1 = c.rss = 0;
+  wave_sync();
+  for (int task = lane; task < ntask; task += 64) {
+    double x;
+    if (FX) {
+      long long s = 0;
+      for (int k = 0; k < kLanes; k++) s += __double_as_longlong(c.part[task][k]);
+      x = (double)s * (rss ? (1.0 / 16777216.0) : (1.0 / 4294967296.0));
+    } else {
+      x = 0;
+      for (int k = 0; k < kLanes; k++) x += c.part[task][k];
+    }
+    const int q = task >> 1, kind = task & 1;
+    if (rss) {
+      if (kind) c.rss = x;
+    } else if (q < nleaf) {
+      const int s = c.slot_of[b][q];
+      if (kind) c.S1[s] = x;
+      else c.S0[s] = x;
+    } else if (q == nleaf) {
+      if (kind) c.sL1 = x;
+      else c.sL0 = x;
+    } else {
+      if (kind) c.sR1 = x;
+      else c.sR0 = x;
+    }
+  }
+  wave_sync();
+}
+
+// ------------------------------------------------------------------ data pass (data waves)
+struct RowPtrs {
+  double* AF;        // global running fit
+  double* AFl;       // LDS running fit (resident mode)
+  double* TM;
+  const double* Y;
+  const float* yf;
+  const float* W;
+  const uint8_t* Q;
+  uint8_t* LEAF;
+  int n, probit;
+};
+
+template <bool FX>
+__device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, const double* __restrict__ src, int cnt,
+                                           int mk, int vl, double acc) {
+  if (FX) {
+    long long s = __double_as_longlong(acc);
+    const long long* v = reinterpret_cast<const long long*>(src);
+    int i = 0;
+    for (; i + 4 <= cnt; i += 4) {
+      const uint32_t t4 = *reinterpret_cast<const uint32_t*>(tag + i);
+#pragma unroll
+      for (int u = 0; u < 4; u++) s += ((int)((t4 >> (8 * u)) & mk) == vl) ? v[i + u] : 0ll;
+    }
+    for (; i < cnt; i++) s += ((int)(tag[i] & mk) == vl) ? v[i] : 0ll;
+    return __longlong_as_double(s);
+  } else {
+    double s = acc;
+    int i = 0;
+    for (; i + 4 <= cnt; i += 4) {
+      const uint32_t t4 = *reinterpret_cast<const uint32_t*>(tag + i);
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const double mm = ((int)((t4 >> (8 * u)) & mk) == vl) ? 1.0 : 0.0;
+        s = fma(mm, src[i + u], s);
+      }
+    }
+    for (; i < cnt; i++) {
+      const double mm = ((int)(tag[i] & mk) == vl) ? 1.0 : 0.0;
+      s = fma(mm, src[i], s);
+    }
+    return s;
+  }
+}
+
+__device__ __forceinline__ long long fx_round(double x, double scale) { return (long long)rint(x * scale); }
+
+template <int R, bool RES, bool FX>
+__device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int dw, int lane, int jp, int j, int cb,
+                          bool apply, bool stats, bool rss, bool keep) {
+  using G = Geo<R>;
+  const int n = P.n;
+  const int pmode = apply ? c.pmode : 0, pnx = c.pnx, pc = c.pc, pL = c.pL, pR = c.pR, pl = c.pl, pr = c.pr;
+  const int cmode = stats ? c.cmode[cb] : 0, cnx = c.cnx[cb], cc = c.cc[cb];
+  const int nleaf = c.nleaf[cb];
+  const int ntask = rss ? 2 : 2 * (nleaf + (cmode == 1 ? 2 : 0));
+  uint8_t* __restrict__ lprev = apply ? P.LEAF + (size_t)jp * n : nullptr;
+  const uint8_t* __restrict__ lcur = stats ? P.LEAF + (size_t)j * n : nullptr;
+  const uint8_t* __restrict__ qp = (apply && pmode == 1) ? P.Q + (size_t)c.pv * n : nullptr;
+  const uint8_t* __restrict__ qc = (stats && cmode == 1) ? P.Q + (size_t)c.cv[cb] * n : nullptr;
+  const double* __restrict__ thc = c.thc[cb];
+  const uint8_t* __restrict__ rank_of = c.rank_of[cb];
+
+  // phase-A coordinates: local lane al, row slot iis; phase-B coordinates: local lane bl, task slot
+  const int al = lane & 3, iis = lane >> 2, ka = kLanesPerWave * dw + al;
+  const int bl = lane >> 4, slot = lane & 15, kb = kLanesPerWave * dw + bl;
+  double* v0 = reinterpret_cast<double*>(scratch);
+  double* v1 = v0 + kLanes * G::kStride;
+  uint8_t* tg = reinterpret_cast<uint8_t*>(v1 + kLanes * G::kStride);
+  double* v0a = v0 + (kLanesPerWave * dw + al) * G::kStride;
+  double* v1a = v1 + (kLanesPerWave * dw + al) * G::kStride;
+  uint8_t* tga = tg + (kLanesPerWave * dw + al) * G::kTagStride;
+  const double* v0b = v0 + kb * G::kStride;
+  const double* v1b = v1 + kb * G::kStride;
+  const uint8_t* tgb = tg + kb * G::kTagStride;
+
+  double acc[kGroups];
+#pragma unroll
+  for (int g = 0; g < kGroups; g++) acc[g] = 0;
+  const int nper_kb = kb < n ? (n - kb + kLanes - 1) / kLanes : 0;
+  int my_nl = 0, my_nr = 0;
+
+  for (int tb = 0; tb < n; tb += G::kTile) {
+    const int ii0 = tb / kLanes;
+    // ---- phase A: all loads of this thread's R rows first, then the arithmetic
+    uint32_t bytes4[R];  // leaf id of tree j-1 | leaf id of tree j << 8 | rank on v_prev << 16 | rank on v_cur << 24
+    float wv[R];
+    double yd[R], afv[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int i = (ii0 + iis + 16 * r) * kLanes + ka;
+      uint32_t b4 = 0;
+      wv[r] = 0.f;
+      yd[r] = afv[r] = 0.0;
+      if (i < n) {
+        if (apply) b4 |= (uint32_t)lprev[i];
+        if (stats) b4 |= (uint32_t)lcur[i] << 8;
+        if (qp) b4 |= (uint32_t)qp[i] << 16;
+        if (qc) b4 |= (uint32_t)qc[i] << 24;
+        yd[r] = P.probit ? P.Y[i] : (double)P.yf[i];
+        wv[r] = P.W[i];
+        afv[r] = RES ? P.AFl[i] : P.AF[i];
+      }
+      bytes4[r] = b4;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int idx = iis + 16 * r;
+      const int i = (ii0 + idx) * kLanes + ka;
+      if (i >= n) continue;
+      double af = afv[r];
+      const uint32_t b4 = bytes4[r];
+      if (apply) {
+        const int lf = (int)(b4 & 255);
+        int lf2 = lf;
+        if (pmode == 1 && lf == pnx)
+          lf2 = (int)((b4 >> 16) & 255) <= pc ? pL : pR;
+        else if (pmode == 2 && (lf == pl || lf == pr))
+          lf2 = pnx;
+        af = af - c.tho[lf] + c.thn[lf2];
+        lprev[i] = (uint8_t)lf2;
+        if (RES) P.AFl[i] = af;
+        else P.AF[i] = af;
+      }
+      const double y = yd[r];
+      const double w = (double)wv[r];
+      if (rss) {
+        const double e = y - af;
+        const double c2 = (w * e) * e;
+        v0a[idx] = 0.0;
+        v1a[idx] = FX ? __longlong_as_double(fx_round(c2, 16777216.0)) : c2;
+        tga[idx] = 0;
+        if (keep) P.TM[i] += af;
+      } else {
+        const int lf = (int)((b4 >> 8) & 255);
+        const double rr = y - af + thc[lf];
+        const double wr = w * rr;
+        int side = 0;
+        if (cmode == 1 && lf == cnx) {
+          side = (int)(b4 >> 24) <= cc ? 1 : 2;
+          my_nl += side == 1;
+          my_nr += side == 2;
+        }
+        v0a[idx] = FX ? __longlong_as_double(fx_round(w, 4294967296.0)) : w;
+        v1a[idx] = FX ? __longlong_as_double(fx_round(wr, 4294967296.0)) : wr;
+        tga[idx] = (uint8_t)(rank_of[lf] | (side << 6));
+      }
+    }
+    wave_sync();
+    // ---- phase B: this thread's tasks walk lane kb's rows of the tile in row order
+    const int cnt = min(G::kTW4, max(0, nper_kb - ii0));
+#pragma unroll
+    for (int g = 0; g < kGroups; g++) {
+      const int task = slot + kTaskSlots * g;
+      if (task < ntask) {
+        // task = 2 rank + kind: kind 0 sums w, kind 1 sums w*r; ranks below nleaf are leaves, then the split halves
+        const int q = task >> 1;
+        const int mk = rss ? 0 : (q < nleaf ? 63 : 0xC0);
+        const int vl = rss ? 0 : (q < nleaf ? q : (q == nleaf ? 0x40 : 0x80));
+        acc[g] = lane_sum<FX>(tgb, (task & 1) ? v1b : v0b, cnt, mk, vl, acc[g]);
+      }
+    }
+    wave_sync();
+  }
+  if (my_nl) atomicAdd(&c.nL, my_nl);
+  if (my_nr) atomicAdd(&c.nR, my_nr);
+#pragma unroll
+  for (int g = 0; g < kGroups; g++) {
+    const int task = slot + kTaskSlots * g;
+    if (task < ntask) c.part[task][kb] = acc[g];
+  }
+}
+
+// ------------------------------------------------------------------ the sweep kernel
+// Two chains per CU: 2 x 5 waves over 4 SIMDs needs 3 waves per SIMD, i.e. at most 168 VGPRs.
+template <int R, bool RES, bool FX>
+__global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(3))) void k_sweeps(
+    KArgs a) {
+  __shared__ Ctl c;
+  extern __shared__ __align__(16) unsigned char dyn[];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const bool ctl = wave == 0;
+  const int dw = wave - 1;
   const int chain = blockIdx.x;
-  if (tid == 0) sm.cd = a.desc[chain];
-  if (tid < kNodes) sm.PG[tid] = a.PG[tid];
+  if (ctl) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(a.desc + chain);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&c.cd);
+    for (int w = lane; w < (int)(sizeof(ChainDesc) / 4); w += 64) dst[w] = src[w];
+    c.PG[lane] = a.PG[lane];
+  }
   __syncthreads();
-  const int n = sm.cd.n, m = a.m;
-  double* Y = a.Y + sm.cd.row_off;
-  const float* W = a.W + sm.cd.row_off;
-  double* AF = a.AF + sm.cd.row_off;
-  double* TM = a.TM + sm.cd.row_off;
-  const uint8_t* YB = a.ybin + sm.cd.row_off;
-  const uint8_t* Q = a.q + sm.cd.q_off;
-  uint8_t* LEAF = a.leaf + sm.cd.leaf_off;
-  TreeDev* TR = a.trees + sm.cd.tree_off;
+  const int n = c.cd.n, m = a.m, cap = a.node_cap;
+  RowPtrs P;
+  P.AF = a.AF + c.cd.row_off;
+  P.AFl = reinterpret_cast<double*>(dyn + a.af_off);
+  P.TM = a.TM + c.cd.row_off;
+  P.Y = a.Y + c.cd.row_off;
+  P.yf = a.yf + c.cd.row_off;
+  P.W = a.W + c.cd.row_off;
+  P.Q = a.q + c.cd.q_off;
+  P.LEAF = a.leaf + c.cd.leaf_off;
+  P.n = n;
+  P.probit = a.probit;
+  TreeDev* TR = a.trees + c.cd.tree_off;
   ChainState* cs = a.state + chain;
   const LibmTables tb = a.tb;
-  const double t2 = (double)sm.cd.tau * sm.cd.tau;
+  const double t2 = (double)c.cd.tau * c.cd.tau;
   const double a_prior = 1 / t2;
+  if (RES && !ctl)
+    for (int i = tid - 64; i < n; i += kBlock - 64) P.AFl[i] = P.AF[i];
 
   Rng g{0, 0, 0, 0};
   double sigma = 0;
   int n_birth = 0, n_death = 0, n_cap = 0;
-  if (tid == 0) {
-    g = Rng{cs->rng[0], cs->rng[1], cs->rng[2], cs->rng[3]};
+  if (ctl) {
+    g = Rng{ufl(cs->rng[0]), ufl(cs->rng[1]), ufl(cs->rng[2]), ufl(cs->rng[3])};
     sigma = cs->sigma;
     n_birth = cs->n_birth;
     n_death = cs->n_death;
@@ -482,94 +602,101 @@ __global__ __launch_bounds__(kThreads) void k_sweeps(KArgs a) {
   }
 
   for (int it = a.it0; it < a.it1; it++) {
-    if (tid == 0) {
-      sm.inv_s2 = 1.0 / (sigma * sigma);
-      sm.pmode = 0;
+    const bool keep = it >= a.nskip;
+    if (ctl) {
+      if (lane == 0) {
+        c.inv_s2 = 1.0 / (sigma * sigma);
+        c.pmode = 0;
+        c.nL = c.nR = 0;  // the data waves add their split counts into these during each pass
+      }
+      copy_tree(&c.tree[0], &TR[0], lane);
+      wave_sync();
+      ctl_propose(c, 0, g, cap, lane);
     }
     __syncthreads();
+    int cb = 0;
     for (int j = 0; j < m; j++) {
-      // ---- tree j into LDS
-      {
-        const uint64_t* src = reinterpret_cast<const uint64_t*>(&TR[j]);
-        uint64_t* dst = reinterpret_cast<uint64_t*>(&sm.T);
-        for (int w = tid; w < (int)(sizeof(TreeDev) / 8); w += kThreads) dst[w] = src[w];
-      }
-      __syncthreads();
-      int ngood = 0, nnogs = 0;
-      if (tid == 0) propose(sm, g, ngood, nnogs);
-      if (tid < kNodes) {
-        const int s = tid;
-        sm.thc[s] = (double)sm.T.th[s];
-        const uint64_t lm = sm.T.used & sm.T.leaf;
-        if (BITM(lm, s)) {
-          const int rk = __popcll(lm & ((1ull << s) - 1ull));
-          sm.rank_of[s] = (uint8_t)rk;
-          sm.slot_of[rk] = (uint8_t)s;
-        } else {
-          sm.rank_of[s] = 0;
+      if (!ctl) {
+        data_pass<R, RES, FX>(c, dyn, P, dw, lane, j - 1, j, cb, j > 0, true, false, false);
+      } else {
+        ctl_rng_and_gather(c, cb, g, cap, tb, lane);
+        if (j + 1 < m) {
+          copy_tree(&c.tree[cb ^ 1], &TR[j + 1], lane);
+          wave_sync();
+          ctl_propose(c, cb ^ 1, g, cap, lane);
         }
-        if (s == 0) sm.nleaf = __popcll(lm);
       }
       __syncthreads();
-      // ---- data pass: apply tree j-1's change, statistics of tree j
-      data_pass(a, sm, AF, TM, Y, W, Q, LEAF, j - 1, j, j > 0, true, false, false);
-
-      // ---- Metropolis-Hastings and the generator outputs of this update's 64 normals (:469-518)
-      if (tid == 0) {
-        TreeDev& t = sm.T;
-        const double inv_s2 = sm.inv_s2;
-        const double lu = log_unif(g, tb);
+      if (ctl) {
+        // ---- lane sums, Metropolis-Hastings (:469-518), leaf values (:520-528)
+        const int cmode = c.cmode[cb];
+        if (FX) ctl_combine<true>(c, cb, false, cmode, lane);
+        else ctl_combine<false>(c, cb, false, cmode, lane);
+        TreeDev& t = c.tree[cb];
+        const double inv_s2 = c.inv_s2;
+        const double lu = c.lu;
+        const double pr = c.prr[cb];
         double lalpha = 0;
-        int accepted = 0, capped = 0;
-        int nmode = 0, nL_ = 0, nR_ = 0, nl_ = 0, nr_ = 0;
-        if (sm.cmode == 1) {
-          if (sm.nL >= 5 && sm.nR >= 5) {
-            const double bl = sm.sL0 * inv_s2, Ml = sm.sL1 * inv_s2, br = sm.sR0 * inv_s2, Mr = sm.sR1 * inv_s2;
-            lalpha = log(sm.prr) + lh(bl, Ml, t2) + lh(br, Mr, t2) - lh(bl + br, Ml + Mr, t2);
+        int accepted = 0, capped = 0, nmode = 0, nL_ = 0, nR_ = 0, nl_ = 0, nr_ = 0;
+        const int cnx = c.cnx[cb];
+        const int nLc = c.nL, nRc = c.nR;
+        if (cmode == 1) {
+          if (nLc >= 5 && nRc >= 5) {
+            const double bl = c.sL0 * inv_s2, Ml = c.sL1 * inv_s2, br = c.sR0 * inv_s2, Mr = c.sR1 * inv_s2;
+            lalpha = log(pr) + lh(bl, Ml, t2) + lh(br, Mr, t2) - lh(bl + br, Ml + Mr, t2);
             if (lalpha > 0) lalpha = 0;
             if (lu < lalpha) {
-              const int L = first_free(t.used, -1, a.node_cap);
-              const int R = L < 0 ? -1 : first_free(t.used, L, a.node_cap);
-              if (L < 0 || R < 0) {
+              const int L = first_free(t.used, -1, cap);
+              const int Rr = L < 0 ? -1 : first_free(t.used, L, cap);
+              if (L < 0 || Rr < 0) {
                 n_cap++;
                 capped = 1;
               } else {
-                const int nx = sm.cnx;
-                t.v[nx] = (uint8_t)sm.cv;
-                t.c[nx] = (uint8_t)sm.cc;
-                t.l[nx] = (uint8_t)L;
-                t.r[nx] = (uint8_t)R;
-                t.par[L] = t.par[R] = (uint8_t)nx;
-                t.l[L] = t.r[L] = t.l[R] = t.r[R] = 0;
-                t.d[L] = t.d[R] = (uint8_t)(t.d[nx] + 1);
-                t.used |= (1ull << L) | (1ull << R);
-                t.leaf = (t.leaf & ~(1ull << nx)) | (1ull << L) | (1ull << R);
-                t.ngv[L] = (uint8_t)count_goodvars(t, L, sm.cd.nc, sm.cd.p);
-                t.ngv[R] = (uint8_t)count_goodvars(t, R, sm.cd.nc, sm.cd.p);
-                sm.S0[L] = sm.sL0;
-                sm.S1[L] = sm.sL1;
-                sm.S0[R] = sm.sR0;
-                sm.S1[R] = sm.sR1;
+                if (lane == 0) {
+                  t.v[cnx] = (uint8_t)c.cv[cb];
+                  t.c[cnx] = (uint8_t)c.cc[cb];
+                  t.l[cnx] = (uint8_t)L;
+                  t.r[cnx] = (uint8_t)Rr;
+                  t.par[L] = t.par[Rr] = (uint8_t)cnx;
+                  t.l[L] = t.r[L] = t.l[Rr] = t.r[Rr] = 0;
+                  t.d[L] = t.d[Rr] = (uint8_t)(t.d[cnx] + 1);
+                  t.used |= (1ull << L) | (1ull << Rr);
+                  t.leaf = (t.leaf & ~(1ull << cnx)) | (1ull << L) | (1ull << Rr);
+                  c.S0[L] = c.sL0;
+                  c.S1[L] = c.sL1;
+                  c.S0[Rr] = c.sR0;
+                  c.S1[Rr] = c.sR1;
+                }
+                wave_sync();
+                const int gL = count_goodvars_wave(t, L, c.cd.nc, c.cd.p, lane);
+                const int gR = count_goodvars_wave(t, Rr, c.cd.nc, c.cd.p, lane);
+                if (lane == 0) {
+                  t.ngv[L] = (uint8_t)gL;
+                  t.ngv[Rr] = (uint8_t)gR;
+                }
                 nmode = 1;
                 nL_ = L;
-                nR_ = R;
+                nR_ = Rr;
                 n_birth++;
                 accepted = 1;
               }
             }
           }
-        } else if (sm.cmode == 2) {
-          const int nx = sm.cnx, l = t.l[nx], r = t.r[nx];
-          const double bl = sm.S0[l] * inv_s2, Ml = sm.S1[l] * inv_s2, br = sm.S0[r] * inv_s2,
-                       Mr = sm.S1[r] * inv_s2;
-          lalpha = log(sm.prr) + lh(bl + br, Ml + Mr, t2) - lh(bl, Ml, t2) - lh(br, Mr, t2);
+        } else if (cmode == 2) {
+          const int l = t.l[cnx], r = t.r[cnx];
+          const double bl = c.S0[l] * inv_s2, Ml = c.S1[l] * inv_s2, br = c.S0[r] * inv_s2, Mr = c.S1[r] * inv_s2;
+          lalpha = log(pr) + lh(bl + br, Ml + Mr, t2) - lh(bl, Ml, t2) - lh(br, Mr, t2);
           if (lalpha > 0) lalpha = 0;
           if (lu < lalpha) {
-            sm.S0[nx] = sm.S0[l] + sm.S0[r];
-            sm.S1[nx] = sm.S1[l] + sm.S1[r];
-            t.used &= ~((1ull << l) | (1ull << r));
-            t.leaf = (t.leaf & ~((1ull << l) | (1ull << r))) | (1ull << nx);
-            t.l[nx] = t.r[nx] = t.v[nx] = t.c[nx] = 0;
+            const double m0 = c.S0[l] + c.S0[r], m1 = c.S1[l] + c.S1[r];
+            wave_sync();
+            if (lane == 0) {
+              c.S0[cnx] = m0;
+              c.S1[cnx] = m1;
+              t.used &= ~((1ull << l) | (1ull << r));
+              t.leaf = (t.leaf & ~((1ull << l) | (1ull << r))) | (1ull << cnx);
+              t.l[cnx] = t.r[cnx] = t.v[cnx] = t.c[cnx] = 0;
+            }
             nmode = 2;
             nl_ = l;
             nr_ = r;
@@ -577,99 +704,88 @@ __global__ __launch_bounds__(kThreads) void k_sweeps(KArgs a) {
             accepted = 1;
           }
         }
-        // node_cap is the reference's NODES: that many normals per update (:521-523)
-        for (int s = 0; s < 2 * a.node_cap; s++) sm.rnd[s] = rng_next(g);
-        // the previous-update record for the next pass (:470, :500, :515)
-        sm.pmode = nmode;
-        sm.pnx = sm.cnx;
-        sm.pv = sm.cv;
-        sm.pc = sm.cc;
-        sm.pL = nL_;
-        sm.pR = nR_;
-        sm.pl = nl_;
-        sm.pr = nr_;
-        if (a.trace) {
-          const long idx = (long)(it - a.it0) * m + j;
-          if (idx < a.trace_cap) {
-            TraceRec& tr = a.trace[(long)chain * a.trace_cap + idx];
-            tr.it = it;
-            tr.j = j;
-            tr.mode = sm.cmode;
-            tr.nx = sm.cnx;
-            tr.v = sm.cv;
-            tr.c = sm.cc;
-            tr.nl = sm.nL;
-            tr.nr = sm.nR;
-            tr.accepted = accepted;
-            tr.ngood = ngood;
-            tr.nnogs = nnogs;
-            tr.cap = capped;
-            tr.pr = sm.prr;
-            tr.lalpha = lalpha;
-            tr.lu = lu;
-            tr.sl0 = sm.sL0;
-            tr.sl1 = sm.sL1;
-            tr.sr0 = sm.sR0;
-            tr.sr1 = sm.sR1;
-            // generator state after this update's 64 normals
-            Rng h = g;
-            tr.rng_after[0] = h.s0;
-            tr.rng_after[1] = h.s1;
-            tr.rng_after[2] = h.s2;
-            tr.rng_after[3] = h.s3;
+        wave_sync();
+        // leaf values: the normals of the used leaves (gathered during the pass)
+        const int s = lane;
+        c.tho[s] = c.thc[cb][s];
+        if (s < cap && BITM(t.leaf, s) && BITM(t.used, s)) {
+          const double b = c.S0[s] * inv_s2, M = c.S1[s] * inv_s2;
+          t.th[s] = (float)(M / (a_prior + b) + (double)c.z[s] / __builtin_sqrt(a_prior + b));
+        }
+        wave_sync();
+        c.thn[s] = (double)t.th[s];
+        if (keep && a.keep_trees)
+          a.trees_out[c.cd.tout_off + ((size_t)(it - a.nskip) * m + j) * kNodes + s] = pack_word(t, s);
+        copy_tree(&TR[j], &t, lane);
+        if (lane == 0) {
+          c.pmode = nmode;
+          c.pnx = cnx;
+          c.pv = c.cv[cb];
+          c.pc = c.cc[cb];
+          c.pL = nL_;
+          c.pR = nR_;
+          c.pl = nl_;
+          c.pr = nr_;
+          c.nL = c.nR = 0;
+          if (a.trace) {
+            const long idx = (long)(it - a.it0) * m + j;
+            if (idx < a.trace_cap) {
+              TraceRec& tr = a.trace[(long)chain * a.trace_cap + idx];
+              tr.it = it;
+              tr.j = j;
+              tr.mode = cmode;
+              tr.nx = cnx;
+              tr.v = c.cv[cb];
+              tr.c = c.cc[cb];
+              tr.nl = nLc;
+              tr.nr = nRc;
+              tr.accepted = accepted;
+              tr.ngood = c.ngood[cb];
+              tr.nnogs = c.nnogs[cb];
+              tr.cap = capped;
+              tr.pr = pr;
+              tr.lalpha = lalpha;
+              tr.lu = lu;
+              tr.sl0 = c.sL0;
+              tr.sl1 = c.sL1;
+              tr.sr0 = c.sR0;
+              tr.sr1 = c.sR1;
+              for (int w = 0; w < 4; w++) tr.rng_after[w] = c.rng_mark[w];
+            }
           }
         }
       }
       __syncthreads();
-      // ---- leaf values: one normal per slot, used leaves keep theirs (:520-528)
-      if (tid < kNodes) {
-        const int s = tid;
-        if (s < a.node_cap && BITM(sm.T.leaf, s) && BITM(sm.T.used, s)) {
-          const float z = normal_f(sm.rnd[2 * s], sm.rnd[2 * s + 1], tb);
-          const double inv_s2 = sm.inv_s2;
-          const double b = sm.S0[s] * inv_s2, M = sm.S1[s] * inv_s2;
-          sm.T.th[s] = (float)(M / (a_prior + b) + (double)z / __dsqrt_rn(a_prior + b));
-        }
-        sm.tho[s] = sm.thc[s];
-      }
-      __syncthreads();
-      if (tid < kNodes) sm.thn[tid] = (double)sm.T.th[tid];
-      {
-        const uint64_t* src = reinterpret_cast<const uint64_t*>(&sm.T);
-        uint64_t* dst = reinterpret_cast<uint64_t*>(&TR[j]);
-        for (int w = tid; w < (int)(sizeof(TreeDev) / 8); w += kThreads) dst[w] = src[w];
-      }
-      __syncthreads();
+      cb ^= 1;
     }
     // ---- closing pass (:536-541)
-    const bool keep = it >= a.nskip;
-    data_pass(a, sm, AF, TM, Y, W, Q, LEAF, m - 1, m, true, false, true, keep);
-    if (tid == 0) {
+    if (!ctl) data_pass<R, RES, FX>(c, dyn, P, dw, lane, m - 1, m, cb, true, false, true, keep);
+    __syncthreads();
+    if (ctl) {
+      if (FX) ctl_combine<true>(c, cb, true, 0, lane);
+      else ctl_combine<false>(c, cb, true, 0, lane);
       if (a.probit) {
         // latent redraw, rows in order (:543-547)
-        const double off = (double)sm.cd.offset;
+        const double off = (double)c.cd.offset;
+        const uint8_t* YB = a.ybin + c.cd.row_off;
+        double* Y = a.Y + c.cd.row_off;
         for (int i = 0; i < n; i++) {
-          const double f = AF[i];
-          Y[i] = YB[i] == 0 ? -rtnorm_dev(g, -f, off, tb) : rtnorm_dev(g, f, -off, tb);
+          const double f = RES ? P.AFl[i] : P.AF[i];
+          const double z = YB[i] == 0 ? -rtnorm_dev(g, -f, off, tb) : rtnorm_dev(g, f, -off, tb);
+          if (lane == 0) Y[i] = z;
         }
       } else {
-        const float nul = sm.cd.nu * sm.cd.lambda;
-        const float nnu = (float)n + sm.cd.nu;
-        sigma = __dsqrt_rn(((double)nul + sm.rss) / (2.0 * rgamma_dev(g, (double)nnu / 2.0, tb)));
+        const float nul = c.cd.nu * c.cd.lambda;
+        const float nnu = (float)n + c.cd.nu;
+        sigma = __builtin_sqrt(((double)nul + c.rss) / (2.0 * rgamma_dev(g, (double)nnu / 2.0, tb)));
       }
-      a.sigma_out[sm.cd.sig_off + it] = (float)sigma;
-      sm.pmode = 0;
-    }
-    if (keep && a.keep_trees) {
-      uint64_t* out = a.trees_out + sm.cd.tout_off + (size_t)(it - a.nskip) * m * kNodes;
-      for (int idx = tid; idx < m * kNodes; idx += kThreads) {
-        const int jj = idx / kNodes, s = idx % kNodes;
-        out[idx] = pack_word(TR[jj], s);
-      }
+      if (lane == 0) a.sigma_out[c.cd.sig_off + it] = (float)sigma;
     }
     __syncthreads();
   }
-  if (tid == 0) {
+  if (RES && !ctl)
+    for (int i = tid - 64; i < n; i += kBlock - 64) P.AF[i] = P.AFl[i];
+  if (ctl && lane == 0) {
     cs->rng[0] = g.s0;
     cs->rng[1] = g.s1;
     cs->rng[2] = g.s2;
@@ -682,23 +798,21 @@ __global__ __launch_bounds__(kThreads) void k_sweeps(KArgs a) {
   }
 }
 
-// Chain initialisation: root-only trees, generator seed + 16 discarded outputs, y or the initial probit latents
+// Chain initialisation: root-only trees, generator seed + 16 discarded outputs, the initial probit latents
 // (src/bart_kernel.cpp:367-386).
-__global__ __launch_bounds__(kThreads) void k_init(KArgs a, const float* yin, const unsigned* seeds) {
+__global__ __launch_bounds__(256) void k_init(KArgs a, const unsigned* seeds) {
   const int tid = threadIdx.x;
   const int chain = blockIdx.x;
   const ChainDesc cd = a.desc[chain];
   TreeDev* TR = a.trees + cd.tree_off;
-  for (int w = tid; w < a.m * (int)(sizeof(TreeDev) / 8); w += kThreads)
-    reinterpret_cast<uint64_t*>(TR)[w] = 0;
-  double* Y = a.Y + cd.row_off;
-  for (int i = tid; i < cd.n; i += kThreads) {
-    Y[i] = (double)yin[cd.row_off + i];
+  for (int w = tid; w < a.m * (int)(sizeof(TreeDev) / 8); w += 256) reinterpret_cast<uint64_t*>(TR)[w] = 0;
+  for (int i = tid; i < cd.n; i += 256) {
     a.AF[cd.row_off + i] = 0;
     a.TM[cd.row_off + i] = 0;
+    a.Y[cd.row_off + i] = (double)a.yf[cd.row_off + i];
   }
   __syncthreads();
-  for (int j = tid; j < a.m; j += kThreads) {
+  for (int j = tid; j < a.m; j += 256) {
     TR[j].ngv[0] = (uint8_t)cd.root_ngv;
     TR[j].used = 1;
     TR[j].leaf = 1;
@@ -710,6 +824,7 @@ __global__ __launch_bounds__(kThreads) void k_init(KArgs a, const float* yin, co
     if (a.probit) {
       const double off = (double)cd.offset;
       const uint8_t* YB = a.ybin + cd.row_off;
+      double* Y = a.Y + cd.row_off;
       for (int i = 0; i < cd.n; i++) Y[i] = YB[i] == 0 ? -rtnorm_dev(g, 0., off, a.tb) : rtnorm_dev(g, 0., -off, a.tb);
     }
     ChainState* cs = a.state + chain;
@@ -729,8 +844,29 @@ __global__ void k_trmean(const double* TM, float* out, int n, int ndpost) {
   if (i < n) out[i] = (float)TM[i] / (float)ndpost;
 }
 
-}  // namespace fbh
+// ------------------------------------------------------------------ launch variants
+using SweepKernel = void (*)(KArgs);
+constexpr int kRowsRes = 6;    // 1536-row tiles when the running fit is resident in LDS
+constexpr int kRowsStream = 4; // 1024-row tiles otherwise
+constexpr int kCtlBytes = sizeof(Ctl);
+constexpr size_t kLdsBudgetPerWG = 80 * 1024;
+inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
+struct Variant {
+  SweepKernel fn;
+  int scratch;
+  bool resident;
+};
+
+inline Variant pick_variant(bool resident, bool fx) {
+  if (resident)
+    return fx ? Variant{k_sweeps<kRowsRes, true, true>, Geo<kRowsRes>::kScratchBytes, true}
+              : Variant{k_sweeps<kRowsRes, true, false>, Geo<kRowsRes>::kScratchBytes, true};
+  return fx ? Variant{k_sweeps<kRowsStream, false, true>, Geo<kRowsStream>::kScratchBytes, false}
+            : Variant{k_sweeps<kRowsStream, false, false>, Geo<kRowsStream>::kScratchBytes, false};
+}
+
+}  // namespace fbh
 // ==================================================================== host side
 using namespace fbh;
 
@@ -758,6 +894,10 @@ struct fbh_batch {
   float* d_trm = nullptr;
   TraceRec* d_trace = nullptr;
   int trace_cap = 0;
+  Variant var{};
+  size_t dyn_bytes = 0;
+  int af_off = 0;
+  int nmax = 0;
   LibmTables tb{};
   size_t rows_total = 0, q_total = 0, leaf_total = 0;
   std::vector<hipEvent_t> ev;  // start/stop pairs per launch
@@ -794,6 +934,7 @@ static KArgs make_args(const fbh_batch* b) {
   a.trees = b->d_trees;
   a.q = b->d_q;
   a.Y = b->d_Y;
+  a.yf = b->d_yin;
   a.W = b->d_W;
   a.AF = b->d_AF;
   a.TM = b->d_TM;
@@ -813,6 +954,7 @@ static KArgs make_args(const fbh_batch* b) {
   a.stats_mode = b->stats_mode;
   a.node_cap = b->node_cap;
   a.it0 = a.it1 = 0;
+  a.af_off = b->af_off;
   return a;
 }
 
@@ -882,8 +1024,9 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     d.tau = ci.tau;
     d.lambda = ci.lambda;
     d.nu = ci.nu;
-    d.sigest = o->probit ? ci.sigest : ci.sigest;
+    d.sigest = ci.sigest;
     d.offset = ci.offset;
+    b->nmax = std::max(b->nmax, ci.n);
     int rn = 0;
     for (int v = 0; v < FBH_PMAX; v++) {
       d.nc[v] = v < ci.p ? ci.numcut[v] : 0;
@@ -904,6 +1047,30 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
   b->q_total = qo;
   b->rows_total = ro;
   b->leaf_total = lo;
+  // ---- kernel variant and LDS: the running fit stays resident in LDS when control state + tile scratch + 8 bytes a row
+  // fit the per-workgroup budget that lets two chains share a CU (160 KiB of LDS per CU on gfx950); otherwise it streams.
+  {
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device) != hipSuccess)
+      lds_max = 65536;
+    const size_t budget = std::min<size_t>((size_t)lds_max, kLdsBudgetPerWG);
+    const size_t need_res = kCtlBytes + align16(Geo<kRowsRes>::kScratchBytes) + 8 * (size_t)b->nmax;
+    const bool res = need_res <= budget;
+    b->var = pick_variant(res, b->stats_mode == FBH_STATS_FX64);
+    b->af_off = (int)align16(b->var.scratch);
+    b->dyn_bytes = (size_t)b->af_off + (res ? 8 * (size_t)b->nmax : 0);
+    if (kCtlBytes + b->dyn_bytes > (size_t)lds_max) {
+      set_error(FBH_EINVAL, "fbh_batch_create: LDS need exceeds the device limit");
+      free_batch(b);
+      return FBH_EINVAL;
+    }
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(b->var.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)b->dyn_bytes) != hipSuccess) {
+      set_error(FBH_EHIP, "fbh_batch_create: hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
+      free_batch(b);
+      return FBH_EHIP;
+    }
+  }
   const size_t tout = b->keep_trees ? (size_t)nchains * b->ndpost * b->m * kNodes : 1;
   if ((rc = dalloc(&b->d_desc, nchains)) || (rc = dalloc(&b->d_state, nchains)) ||
       (rc = dalloc(&b->d_trees, (size_t)nchains * b->m)) || (rc = dalloc(&b->d_q, qo)) ||
@@ -951,7 +1118,7 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     if (e == hipSuccess) e = hipMemsetAsync(b->d_leaf, 0, lo, b->stream);
     if (e == hipSuccess) {
       KArgs a = make_args(b);
-      hipLaunchKernelGGL(k_init, dim3(nchains), dim3(kThreads), 0, b->stream, a, b->d_yin, d_seeds);
+      hipLaunchKernelGGL(k_init, dim3(nchains), dim3(256), 0, b->stream, a, d_seeds);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
@@ -985,8 +1152,9 @@ extern "C" int fbh_batch_run(fbh_batch* b, int nsweeps) {
     b->ev.push_back(e0);
     b->ev.push_back(e1);
     FBH_HIP_CHECK(hipEventRecord(e0, b->stream));
-    hipLaunchKernelGGL(k_sweeps, dim3(b->C), dim3(kThreads), 0, b->stream, a);
-    FBH_HIP_CHECK(hipGetLastError());
+    void* args[] = {&a};
+    FBH_HIP_CHECK(hipLaunchKernel(reinterpret_cast<const void*>(b->var.fn), dim3(b->C), dim3(kBlock), args,
+                                  b->dyn_bytes, b->stream));
     FBH_HIP_CHECK(hipEventRecord(e1, b->stream));
     b->it = it1;
   }

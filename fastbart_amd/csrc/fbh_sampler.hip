@@ -32,6 +32,7 @@ constexpr int kLanesPerWave = kLanes / kDataWaves;   // 4 summation lanes per da
 constexpr int kTaskSlots = 64 / kLanesPerWave;       // 16 task threads per lane
 constexpr int kMaxTasks = 2 * kMaxRank;              // (sum of w, sum of w*r) for every leaf and split half
 constexpr int kGroups = (kMaxTasks + kTaskSlots - 1) / kTaskSlots;
+constexpr int kTreeWords = sizeof(TreeDev) / 8;  // 90
 
 // Tile geometry: R rows per phase-A thread per tile, 16 R rows per lane, 256 R rows per tile.
 template <int R>
@@ -322,25 +323,6 @@ __device__ void ctl_propose(Ctl& c, int b, Rng& g, int cap, int lane) {
   }
 }
 
-// The generator outputs of this update's MH uniform and its NODES leaf normals (:482, :521-523), then the gathers of
-// the libm images for the slots that can be leaves afterwards.
-__device__ void ctl_rng_and_gather(Ctl& c, int b, Rng& g, int cap, const LibmTables& tb, int lane) {
-  const int nout = 1 + 2 * cap;
-  for (int k = 0; k < nout; k++) {
-    const uint32_t x = rng_next(g);
-    if (lane == 0) c.rnd[k] = x;
-  }
-  if (lane == 0) {
-    c.rng_mark[0] = g.s0;
-    c.rng_mark[1] = g.s1;
-    c.rng_mark[2] = g.s2;
-    c.rng_mark[3] = g.s3;
-  }
-  wave_sync();
-  if (lane == 0) c.lu = tb.logd_t[c.rnd[0] >> 8];
-  if (lane < cap && BITM(c.cand[b], lane)) c.z[lane] = normal_f(c.rnd[1 + 2 * lane], c.rnd[2 + 2 * lane], tb);
-}
-
 // Add the 16 lanes of every task in lane order from zero (:202-212).
 template <bool FX>
 __device__ void ctl_combine(Ctl& c, int b, bool rss, int cmode, int lane) {
@@ -391,37 +373,47 @@ struct RowPtrs {
   int n, probit;
 };
 
+// Walk one lane's rows in row order, adding src[i] to the task's sum where (tag[i] & mk) == vl. Rows go in blocks of 16
+// whose LDS reads are all issued before the (strictly sequential) adds.
 template <bool FX>
 __device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, const double* __restrict__ src, int cnt,
                                            int mk, int vl, double acc) {
+  int i = 0;
   if (FX) {
     long long s = __double_as_longlong(acc);
     const long long* v = reinterpret_cast<const long long*>(src);
-    int i = 0;
-    for (; i + 4 <= cnt; i += 4) {
-      const uint32_t t4 = *reinterpret_cast<const uint32_t*>(tag + i);
+    for (; i + 16 <= cnt; i += 16) {
+      uint32_t t4[4];
+      long long x[16];
 #pragma unroll
-      for (int u = 0; u < 4; u++) s += ((int)((t4 >> (8 * u)) & mk) == vl) ? v[i + u] : 0ll;
+      for (int u = 0; u < 4; u++) t4[u] = *reinterpret_cast<const uint32_t*>(tag + i + 4 * u);
+#pragma unroll
+      for (int u = 0; u < 16; u++) x[u] = v[i + u];
+#pragma unroll
+      for (int u = 0; u < 16; u++) s += ((int)((t4[u >> 2] >> (8 * (u & 3))) & mk) == vl) ? x[u] : 0ll;
     }
     for (; i < cnt; i++) s += ((int)(tag[i] & mk) == vl) ? v[i] : 0ll;
     return __longlong_as_double(s);
-  } else {
-    double s = acc;
-    int i = 0;
-    for (; i + 4 <= cnt; i += 4) {
-      const uint32_t t4 = *reinterpret_cast<const uint32_t*>(tag + i);
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const double mm = ((int)((t4 >> (8 * u)) & mk) == vl) ? 1.0 : 0.0;
-        s = fma(mm, src[i + u], s);
-      }
-    }
-    for (; i < cnt; i++) {
-      const double mm = ((int)(tag[i] & mk) == vl) ? 1.0 : 0.0;
-      s = fma(mm, src[i], s);
-    }
-    return s;
   }
+  double s = acc;
+  for (; i + 16 <= cnt; i += 16) {
+    uint32_t t4[4];
+    double x[16];
+#pragma unroll
+    for (int u = 0; u < 4; u++) t4[u] = *reinterpret_cast<const uint32_t*>(tag + i + 4 * u);
+#pragma unroll
+    for (int u = 0; u < 16; u++) x[u] = src[i + u];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const double mm = ((int)((t4[u >> 2] >> (8 * (u & 3))) & mk) == vl) ? 1.0 : 0.0;
+      s = fma(mm, x[u], s);
+    }
+  }
+  for (; i < cnt; i++) {
+    const double mm = ((int)(tag[i] & mk) == vl) ? 1.0 : 0.0;
+    s = fma(mm, src[i], s);
+  }
+  return s;
 }
 
 __device__ __forceinline__ long long fx_round(double x, double scale) { return (long long)rint(x * scale); }
@@ -439,6 +431,11 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
   const uint8_t* __restrict__ lcur = stats ? P.LEAF + (size_t)j * n : nullptr;
   const uint8_t* __restrict__ qp = (apply && pmode == 1) ? P.Q + (size_t)c.pv * n : nullptr;
   const uint8_t* __restrict__ qc = (stats && cmode == 1) ? P.Q + (size_t)c.cv[cb] * n : nullptr;
+  // unused streams read a valid dummy column so that every load is unconditional
+  const uint8_t* __restrict__ la = lprev ? lprev : P.LEAF;
+  const uint8_t* __restrict__ lb = lcur ? lcur : P.LEAF;
+  const uint8_t* __restrict__ qa = qp ? qp : P.Q;
+  const uint8_t* __restrict__ qb = qc ? qc : P.Q;
   const double* __restrict__ thc = c.thc[cb];
   const uint8_t* __restrict__ rank_of = c.rank_of[cb];
 
@@ -463,26 +460,17 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
 
   for (int tb = 0; tb < n; tb += G::kTile) {
     const int ii0 = tb / kLanes;
-    // ---- phase A: all loads of this thread's R rows first, then the arithmetic
+    // ---- phase A: all of the tile's loads first (rows past n re-read row n-1), then the arithmetic
     uint32_t bytes4[R];  // leaf id of tree j-1 | leaf id of tree j << 8 | rank on v_prev << 16 | rank on v_cur << 24
     float wv[R];
     double yd[R], afv[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      const int i = (ii0 + iis + 16 * r) * kLanes + ka;
-      uint32_t b4 = 0;
-      wv[r] = 0.f;
-      yd[r] = afv[r] = 0.0;
-      if (i < n) {
-        if (apply) b4 |= (uint32_t)lprev[i];
-        if (stats) b4 |= (uint32_t)lcur[i] << 8;
-        if (qp) b4 |= (uint32_t)qp[i] << 16;
-        if (qc) b4 |= (uint32_t)qc[i] << 24;
-        yd[r] = P.probit ? P.Y[i] : (double)P.yf[i];
-        wv[r] = P.W[i];
-        afv[r] = RES ? P.AFl[i] : P.AF[i];
-      }
-      bytes4[r] = b4;
+      const int i = min((ii0 + iis + 16 * r) * kLanes + ka, n - 1);
+      bytes4[r] = (uint32_t)la[i] | ((uint32_t)lb[i] << 8) | ((uint32_t)qa[i] << 16) | ((uint32_t)qb[i] << 24);
+      yd[r] = P.Y[i];
+      wv[r] = P.W[i];
+      afv[r] = RES ? P.AFl[i] : P.AF[i];
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -619,12 +607,41 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       if (!ctl) {
         data_pass<R, RES, FX>(c, dyn, P, dw, lane, j - 1, j, cb, j > 0, true, false, false);
       } else {
-        ctl_rng_and_gather(c, cb, g, cap, tb, lane);
-        if (j + 1 < m) {
-          copy_tree(&c.tree[cb ^ 1], &TR[j + 1], lane);
+        // tree j+1's words are requested first so that their latency overlaps the generator stepping
+        const bool nxt = j + 1 < m;
+        const uint64_t* tsrc = reinterpret_cast<const uint64_t*>(&TR[nxt ? j + 1 : j]);
+        const uint64_t tw0 = tsrc[lane];
+        const uint64_t tw1 = tsrc[lane < kTreeWords - 64 ? 64 + lane : lane];
+        // this update's generator outputs: the MH uniform (:482), then two per slot for the leaf normals (:521-523)
+        const uint32_t x0 = rng_next(g);
+        uint32_t my1 = 0, my2 = 0;
+        for (int s = 0; s < cap; s++) {
+          const uint32_t a1 = rng_next(g);
+          const uint32_t a2 = rng_next(g);
+          my1 = lane == s ? a1 : my1;
+          my2 = lane == s ? a2 : my2;
+        }
+        if (lane == 0) {
+          c.rng_mark[0] = g.s0;
+          c.rng_mark[1] = g.s1;
+          c.rng_mark[2] = g.s2;
+          c.rng_mark[3] = g.s3;
+        }
+        // libm-image gathers for the MH uniform and for the slots that can be leaves after the MH step; consumed after
+        // the next proposal
+        const bool want = lane < cap && BITM(c.cand[cb], lane);
+        const double luv = tb.logd_t[x0 >> 8];
+        const float lf = tb.logf_t[want ? (my1 >> 8) : 0u];
+        const float cf = tb.cosf_t[want ? (my2 >> 8) : 0u];
+        if (nxt) {
+          uint64_t* tdst = reinterpret_cast<uint64_t*>(&c.tree[cb ^ 1]);
+          tdst[lane] = tw0;
+          if (lane < kTreeWords - 64) tdst[64 + lane] = tw1;
           wave_sync();
           ctl_propose(c, cb ^ 1, g, cap, lane);
         }
+        if (want) c.z[lane] = __builtin_sqrtf(-2.f * lf) * cf;
+        if (lane == 0) c.lu = luv;
       }
       __syncthreads();
       if (ctl) {

@@ -149,7 +149,7 @@ struct Ctl {
   double S0[kNodes], S1[kNodes];
   double part[kMaxTasks][kLanes];   // per-lane sums (double bits, or int64 in FX64 mode)
   uint64_t cand[2];              // slots that can be leaves after the MH step
-  double prr[2];
+  double prr[2], lpr[2];           // proposal ratio and its log (taken while the data waves run)
   int cmode[2], cnx[2], cv[2], cc[2], nleaf[2], ngood[2], nnogs[2];
   uint8_t rank_of[2][kNodes], slot_of[2][kNodes];
   uint32_t rnd[2 * kNodes + 4];  // generator outputs of the MH uniform and the leaf normals
@@ -310,8 +310,10 @@ __device__ void ctl_propose(Ctl& c, int b, Rng& g, int cap, int lane) {
     cnx = nx;
     cand |= 1ull << nx;
   }
+  const double lpr = cmode ? log(pr) : 0.0;
   if (lane == 0) {
     c.cmode[b] = cmode;
+    c.lpr[b] = lpr;
     c.cnx[b] = cnx;
     c.cv[b] = cv;
     c.cc[b] = cc;
@@ -607,6 +609,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       if (!ctl) {
         data_pass<R, RES, FX>(c, dyn, P, dw, lane, j - 1, j, cb, j > 0, true, false, false);
       } else {
+        // the previous update's tree (buffer cb ^ 1) is stored before that buffer receives tree j+1
+        if (j > 0) {
+          const TreeDev& tp = c.tree[cb ^ 1];
+          if (keep && a.keep_trees)
+            a.trees_out[c.cd.tout_off + ((size_t)(it - a.nskip) * m + (j - 1)) * kNodes + lane] = pack_word(tp, lane);
+          copy_tree(&TR[j - 1], &tp, lane);
+        }
         // tree j+1's words are requested first so that their latency overlaps the generator stepping
         const bool nxt = j + 1 < m;
         const uint64_t* tsrc = reinterpret_cast<const uint64_t*>(&TR[nxt ? j + 1 : j]);
@@ -660,7 +669,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         if (cmode == 1) {
           if (nLc >= 5 && nRc >= 5) {
             const double bl = c.sL0 * inv_s2, Ml = c.sL1 * inv_s2, br = c.sR0 * inv_s2, Mr = c.sR1 * inv_s2;
-            lalpha = log(pr) + lh(bl, Ml, t2) + lh(br, Mr, t2) - lh(bl + br, Ml + Mr, t2);
+            // the three leaf likelihoods on lanes 0-2, added in the reference's order
+            const double h = lane == 0 ? lh(bl, Ml, t2) : (lane == 1 ? lh(br, Mr, t2) : lh(bl + br, Ml + Mr, t2));
+            lalpha = c.lpr[cb] + __shfl(h, 0) + __shfl(h, 1) - __shfl(h, 2);
             if (lalpha > 0) lalpha = 0;
             if (lu < lalpha) {
               const int L = first_free(t.used, -1, cap);
@@ -702,7 +713,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         } else if (cmode == 2) {
           const int l = t.l[cnx], r = t.r[cnx];
           const double bl = c.S0[l] * inv_s2, Ml = c.S1[l] * inv_s2, br = c.S0[r] * inv_s2, Mr = c.S1[r] * inv_s2;
-          lalpha = log(pr) + lh(bl + br, Ml + Mr, t2) - lh(bl, Ml, t2) - lh(br, Mr, t2);
+          const double h = lane == 0 ? lh(bl + br, Ml + Mr, t2) : (lane == 1 ? lh(bl, Ml, t2) : lh(br, Mr, t2));
+          lalpha = c.lpr[cb] + __shfl(h, 0) - __shfl(h, 1) - __shfl(h, 2);
           if (lalpha > 0) lalpha = 0;
           if (lu < lalpha) {
             const double m0 = c.S0[l] + c.S0[r], m1 = c.S1[l] + c.S1[r];
@@ -731,9 +743,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         }
         wave_sync();
         c.thn[s] = (double)t.th[s];
-        if (keep && a.keep_trees)
-          a.trees_out[c.cd.tout_off + ((size_t)(it - a.nskip) * m + j) * kNodes + s] = pack_word(t, s);
-        copy_tree(&TR[j], &t, lane);
+        // tree j goes back to global memory (and to the kept draws) during the next pass, off the critical path
         if (lane == 0) {
           c.pmode = nmode;
           c.pnx = cnx;
@@ -775,7 +785,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       __syncthreads();
       cb ^= 1;
     }
-    // ---- closing pass (:536-541)
+    // ---- closing pass (:536-541); meanwhile the control wave stores the last tree (buffer cb ^ 1)
+    if (ctl) {
+      const TreeDev& tp = c.tree[cb ^ 1];
+      if (keep && a.keep_trees)
+        a.trees_out[c.cd.tout_off + ((size_t)(it - a.nskip) * m + (m - 1)) * kNodes + lane] = pack_word(tp, lane);
+      copy_tree(&TR[m - 1], &tp, lane);
+    }
     if (!ctl) data_pass<R, RES, FX>(c, dyn, P, dw, lane, m - 1, m, cb, true, false, true, keep);
     __syncthreads();
     if (ctl) {

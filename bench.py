@@ -73,30 +73,54 @@ def cpu_baseline(cores: int, chains_per_core: int = 3, sweeps: int = 40):
 
 
 # ------------------------------------------------------------------ ESS (Geyer initial monotone sequence)
-def ess_geyer(x: np.ndarray) -> float:
+def ess_geyer_cols(x: np.ndarray) -> np.ndarray:
+    """ESS of every column of x [draws][series]: autocovariances by FFT, Geyer's initial monotone sequence of pair
+    sums (Geyer 1992), ESS = n / (-1 + 2 sum of the monotone pair sums)."""
     x = np.asarray(x, np.float64)
-    n = len(x)
-    if n < 4 or np.var(x) == 0:
-        return float(n)
-    xc = x - x.mean()
-    f = np.fft.rfft(xc, 2 * n)
-    acov = np.fft.irfft(f * np.conj(f))[:n] / n
-    rho = acov / acov[0]
-    pairs = []
-    for k in range(0, n - 1, 2):
-        s = rho[k] + rho[k + 1]
-        if s <= 0:
-            break
-        pairs.append(s)
-    pairs = np.minimum.accumulate(np.array(pairs)) if pairs else np.array([1.0])
-    tau = -1.0 + 2.0 * pairs.sum()
-    return float(n / max(tau, 1e-12))
+    n, k = x.shape
+    xc = x - x.mean(axis=0)
+    var = (xc * xc).sum(axis=0) / n
+    f = np.fft.rfft(xc, 2 * n, axis=0)
+    acov = np.fft.irfft(f * np.conj(f), axis=0)[:n] / n
+    ok = var > 0
+    rho = np.where(ok, acov / np.where(ok, var, 1.0), 0.0)
+    m = (n - 1) // 2
+    pairs = rho[0:2 * m:2] + rho[1:2 * m:2]               # [m][k]
+    pos = pairs > 0
+    first_neg = np.where(pos.all(axis=0), m, np.argmin(pos, axis=0))
+    mono = np.minimum.accumulate(pairs, axis=0)
+    keep = np.arange(m)[:, None] < first_neg[None, :]
+    tau = -1.0 + 2.0 * (mono * keep).sum(axis=0)
+    ess = n / np.maximum(tau, 1e-12)
+    return np.where(ok, ess, float(n))
+
+
+def ess_geyer(x: np.ndarray) -> float:
+    return float(ess_geyer_cols(np.asarray(x, np.float64)[:, None])[0])
 
 
 def min_ess_chain(sigma_kept: np.ndarray, yhat: np.ndarray, max_rows: int = 0) -> float:
-    cols = range(yhat.shape[1]) if not max_rows else np.linspace(0, yhat.shape[1] - 1, max_rows).astype(int)
-    e = [ess_geyer(sigma_kept)] + [ess_geyer(yhat[:, i]) for i in cols]
-    return float(min(e))
+    cols = yhat if not max_rows else yhat[:, np.linspace(0, yhat.shape[1] - 1, max_rows).astype(int)]
+    return float(min(ess_geyer(sigma_kept), float(ess_geyer_cols(cols).min())))
+
+
+# ------------------------------------------------------------------ measured HBM traffic
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sweeps_batch512_traffic.txt")
+
+
+def measured_traffic(sweeps_per_launch: float):
+    """HBM bytes per launch of the sweep kernel from the committed counter summary (FETCH_SIZE + WRITE_SIZE per sweep,
+    each scaled by the ratio the same counter reads for an elementwise copy of known size), or None."""
+    t = {}
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for ln in f:
+                k, sep, v = ln.partition(":")
+                if sep and k.strip() in ("fetch_bytes_per_sweep", "write_bytes_per_sweep"):
+                    t[k.strip()] = float(v.split()[0])
+        return (t["fetch_bytes_per_sweep"] + t["write_bytes_per_sweep"]) * sweeps_per_launch
+    except (OSError, KeyError, ValueError, IndexError):
+        return None
 
 
 # ------------------------------------------------------------------ main
@@ -106,9 +130,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=None, help="timed sweeps (default: 1100 - warmup)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--chains", type=int, default=512)
-    ap.add_argument("--sweeps-per-launch", type=int, default=0)
+    ap.add_argument("--sweeps-per-launch", type=int, default=20,
+                    help="sweeps per persistent launch (the warmup and the timed sweeps split into equal launches)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ess-rows", type=int, default=0, help="rows per chain used for ESS (0 = all)")
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="also run an elementwise device copy of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs every chain on a single MI355X "
@@ -144,7 +171,7 @@ def main() -> int:
         b.sync()
     t_w = time.perf_counter() - t_w
     log(f"warmup {W_} sweeps: {t_w:.2f} s")
-    _, nl0, _ = b.timing()
+    avg0, nl0, _ = b.timing()
     t1 = time.perf_counter()
     b.run(K)
     b.sync()
@@ -155,11 +182,15 @@ def main() -> int:
     log(f"timed {K} sweeps: {T:.3f} s -> {value / 1e6:.3f} M tree-updates/s; kernel avg {avg_ms:.3f} ms over {nl} "
         f"launches")
 
-    # roofline of the sweep kernel: algorithmic bytes (37 per row per tree update, x (m+1)/m) / kernel time
+    # roofline of the sweep kernel over the timed launches: algorithmic bytes (37 per row per tree update, times
+    # (m+1)/m for the closing pass) per launch / average launch duration (HIP events on the sampler's stream)
     rows_total = sum(p["q"].shape[1] for p in probs)
     bytes_per_sweep = 37.0 * rows_total * (M + 1)
-    spl = b.sweeps_per_launch if hasattr(b, "sweeps_per_launch") else None
-    achieved = bytes_per_sweep * total / (avg_ms * nl / 1e3) / 1e9 if nl else 0.0
+    nlt = nl - nl0
+    ms_launch = (avg_ms * nl - avg0 * nl0) / nlt if nlt else 0.0
+    sweeps_launch = K / nlt if nlt else 0.0
+    achieved = bytes_per_sweep * sweeps_launch / (ms_launch / 1e3) / 1e9 if nlt else 0.0
+    traffic = measured_traffic(sweeps_launch)
 
     ess = None
     if ndpost >= 10:
@@ -172,6 +203,10 @@ def main() -> int:
         ess = tot / (t_w + T)
         log(f"min-ESS: sum over chains {tot:.1f} ({time.perf_counter() - t2:.1f} s to compute)")
     b.close()
+    if args.calib_copy:
+        # a plain copy kernel that reads and writes calib_copy MiB once each
+        ms = sampler.debug_copy(args.calib_copy)
+        log(f"calibration copy of {args.calib_copy} MiB: {ms:.3f} ms, {2 * (args.calib_copy << 20) / ms / 1e6:.0f} GB/s")
 
     line = {
         "metric": METRIC, "value": value, "unit": "tree-updates/s", "n_gpus": 1, "steps": K, "warmup": W_,
@@ -182,9 +217,10 @@ def main() -> int:
                    "chains": args.chains, "ntree": M, "rows_total": rows_total, "parallelism": "chains"},
         "min_ess_per_sec": ess,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_sweeps", "kernel_avg_ms": avg_ms, "launches": nl,
-                     "bytes_per_tree_update_row": 37},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_sweeps", "kernel_avg_ms": ms_launch, "launches": nlt,
+                     "sweeps_per_launch": sweeps_launch, "bytes_per_tree_update_row": 37,
+                     "algorithmic_bytes_per_launch": bytes_per_sweep * sweeps_launch},
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -53,7 +53,7 @@ EXPORTS = ["bart_fit_cpu", "bart_wfit", "bart_set_rng", "fbh_last_status", "fbh_
            "fbh_batch_yhat", "fbh_batch_timing", "fbh_batch_destroy", "fbh_fit_batch", "fbh_tree_text", "fbh_yhat",
            "fbh_compile_trees", "fbh_trees_info", "fbh_trees_arrays", "fbh_trees_free", "fbh_predict_draw",
            "fbh_predict_draws", "fbh_rows_upload", "fbh_rows_free", "fbh_predict_rows", "fbh_predict_timing",
-           "fbh_debug_math", "fbh_debug_libm_host", "fbh_batch_trace", "fbh_version"]
+           "fbh_debug_math", "fbh_debug_libm_host", "fbh_debug_copy", "fbh_batch_trace", "fbh_version"]
 
 _lock = threading.Lock()
 _lib = None
@@ -99,6 +99,7 @@ def _declare(L):
     L.fbh_predict_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i)]
     L.fbh_debug_math.argtypes = [i, vp, vp, C.c_long, vp, vp, i]
     L.fbh_debug_libm_host.argtypes = [i, C.c_long, C.c_long, vp]
+    L.fbh_debug_copy.argtypes = [C.c_long, i, C.POINTER(d)]
     return L
 
 

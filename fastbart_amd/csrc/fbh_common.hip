@@ -110,6 +110,13 @@ __global__ void k_debug_math(int kind, const uint32_t* in_u, const double* in_d,
   }
 }
 
+// plain copy of 16-byte words, grid-stride (counter calibration: reads and writes exactly `words` * 16 bytes)
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                     long words) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) dst[i] = src[i];
+}
+
 }  // namespace fbh
 
 using namespace fbh;
@@ -169,5 +176,38 @@ extern "C" int fbh_debug_math(int kind, const uint32_t* in_u, const double* in_d
   (void)hipFree(dd);
   (void)hipFree(of);
   (void)hipFree(od);
+  return FBH_OK;
+}
+
+extern "C" int fbh_debug_copy(long mib, int device, double* ms) {
+  if (mib < 1 || mib > 16384) {
+    set_error(FBH_EINVAL, "fbh_debug_copy: size outside 1..16384 MiB");
+    return FBH_EINVAL;
+  }
+  int rc = ensure_device(device);
+  if (rc) return rc;
+  const long words = (mib << 20) / 16;
+  uint4 *src = nullptr, *dst = nullptr;
+  FBH_HIP_CHECK(hipMalloc(&src, words * 16));
+  if (hipMalloc(&dst, words * 16) != hipSuccess) {
+    (void)hipFree(src);
+    set_error(FBH_ENOMEM, "fbh_debug_copy: hipMalloc failed");
+    return FBH_ENOMEM;
+  }
+  hipEvent_t e0, e1;
+  FBH_HIP_CHECK(hipEventCreate(&e0));
+  FBH_HIP_CHECK(hipEventCreate(&e1));
+  FBH_HIP_CHECK(hipEventRecord(e0, 0));
+  hipLaunchKernelGGL(k_stream_copy, dim3(8192), dim3(256), 0, 0, src, dst, words);
+  FBH_HIP_CHECK(hipGetLastError());
+  FBH_HIP_CHECK(hipEventRecord(e1, 0));
+  FBH_HIP_CHECK(hipEventSynchronize(e1));
+  float t = 0;
+  FBH_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(src);
+  (void)hipFree(dst);
   return FBH_OK;
 }

@@ -189,3 +189,10 @@ def debug_math(kind, u=None, d=None, device=0):
     od = np.zeros(count, np.float64) if kind >= 2 else None
     check(lib().fbh_debug_math(kind, ptr(u), ptr(d), count, ptr(of), ptr(od), device), "fbh_debug_math")
     return of if kind <= 1 else od
+
+
+def debug_copy(mib, device=0):
+    """One plain device copy of mib MiB (counter calibration); returns its duration in ms."""
+    ms = C.c_double()
+    check(lib().fbh_debug_copy(int(mib), device, C.byref(ms)), "fbh_debug_copy")
+    return ms.value

@@ -166,6 +166,9 @@ int fbh_debug_libm_host(int kind, long k0, long count, void* out);
 /* per-tree-update trace of the sampler: cap > 0 allocates cap records per chain for the following launches;
  * cap = 0 copies chain's records (struct layout: fbh::TraceRec) into host_out */
 int fbh_batch_trace(fbh_batch* b, int cap, void* host_out, int chain);
+/* one launch of a plain copy kernel over two fresh device buffers of mib MiB (1..16384); its duration in ms goes to
+ * *ms. Used to calibrate hardware byte counters (FETCH_SIZE / WRITE_SIZE) against a transfer of known size. */
+int fbh_debug_copy(long mib, int device, double* ms);
 /* library build information */
 const char* fbh_version(void);
 

@@ -999,8 +999,11 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
   *out = nullptr;
   if (o->m < 1 || o->m > FBH_MMAX || o->nskip < 0 || o->ndpost < 0 || !o->pgtab ||
       (o->stats_mode != FBH_STATS_REF16 && o->stats_mode != FBH_STATS_FX64) ||
-      (o->node_cap != 0 && (o->node_cap < 3 || o->node_cap > kNodes))) {
-    set_error(FBH_EINVAL, "fbh_batch_create: bad options (m, nskip, ndpost, pgtab, stats_mode or node_cap)");
+      (o->node_cap != 0 && (o->node_cap < 3 || o->node_cap > kNodes)) ||
+      (o->block_threads != 0 && o->block_threads != kBlock) ||
+      (o->tile_rows != 0 && o->tile_rows != Geo<kRowsStream>::kTile && o->tile_rows != Geo<kRowsRes>::kTile)) {
+    set_error(FBH_EINVAL, "fbh_batch_create: bad options (m, nskip, ndpost, pgtab, stats_mode, node_cap, "
+                          "block_threads or tile_rows)");
     return FBH_EINVAL;
   }
   for (int c = 0; c < nchains; c++) {
@@ -1088,7 +1091,12 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
       lds_max = 65536;
     const size_t budget = std::min<size_t>((size_t)lds_max, kLdsBudgetPerWG);
     const size_t need_res = kCtlBytes + align16(Geo<kRowsRes>::kScratchBytes) + 8 * (size_t)b->nmax;
-    const bool res = need_res <= budget;
+    const bool res = need_res <= budget && o->tile_rows != Geo<kRowsStream>::kTile;
+    if (o->tile_rows == Geo<kRowsRes>::kTile && !res) {
+      set_error(FBH_EINVAL, "fbh_batch_create: 1536-row tiles need the running fit resident in LDS (n too large)");
+      free_batch(b);
+      return FBH_EINVAL;
+    }
     b->var = pick_variant(res, b->stats_mode == FBH_STATS_FX64);
     b->af_off = (int)align16(b->var.scratch);
     b->dyn_bytes = (size_t)b->af_off + (res ? 8 * (size_t)b->nmax : 0);

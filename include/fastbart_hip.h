@@ -79,8 +79,9 @@ typedef struct fbh_opts {
   int node_cap;             /* node slots a tree may use (3..64), 0 = 64 */
   int device;               /* HIP device ordinal */
   int sweeps_per_launch;    /* 0 = automatic */
-  int block_threads;        /* 0 = automatic (tuned); otherwise 64..1024, a multiple of 64 */
-  int tile_rows;            /* 0 = automatic; rows staged in LDS per tile, multiple of 16 */
+  int block_threads;        /* 0 or 320: one control wave + four data waves per chain (the only shape built) */
+  int tile_rows;            /* 0 = automatic; 1024 or 1536 rows staged in LDS per tile (1536 needs the running fit
+                               resident in LDS, 1024 is the only size when it streams) */
 } fbh_opts;
 
 typedef struct fbh_chain_out {
@@ -136,7 +137,7 @@ int fbh_trees_info(const fbh_trees* t, int* ndpost, int* ntree, int* p, int* nno
 int fbh_trees_arrays(const fbh_trees* t, int* var, int* child, double* cut, double* theta, int* root);
 void fbh_trees_free(fbh_trees* t);
 /* One draw (m_draw 1-based) for n rows given as p host columns: out[i] = mu + sum over trees in order of theta at the
- * leaf reached with x >= cut going right (predict_draw_cpp, src/predict.cpp:111-135). */
+ * leaf reached with x >= cut going right (predict_draw_cpp, src/predict.cpp:49-73). */
 int fbh_predict_draw(const fbh_trees* t, const double* const* cols, int n, int p, int m_draw, double mu, double* out);
 /* Batched pwbart: draws[k] (1-based) for k < ndraws over the same rows; out [ndraws][n]. x is row data uploaded
  * once per call as p columns. */

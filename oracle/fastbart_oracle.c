@@ -831,7 +831,7 @@ void orc_ct_free(orc_ct* ct) {
   memset(ct, 0, sizeof *ct);
 }
 
-/* tree-outer, row-inner; each row adds the trees in order, then mu; x >= cut goes right. src/predict.cpp:111-135 */
+/* tree-outer, row-inner; each row adds the trees in order, then mu; x >= cut goes right. src/predict.cpp:49-73 */
 int orc_predict_draw(const orc_ct* ct, const double* const* cols, int n, int p, int m_draw, double mu, double* out) {
   if (p != ct->p) return -1;
   if (m_draw < 1 || m_draw > ct->ndpost) return -2;

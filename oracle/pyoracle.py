@@ -176,8 +176,8 @@ class Oracle:
         return out
 
     # -- chains advanced sweep by sweep (state snapshots for parity)
-    def chain(self, prob, m, nskip, ndpost, seed, probit=0, stats_mode=0, node_cap=0):
-        return OracleChain(self, prob, m, nskip, ndpost, seed, probit, stats_mode, node_cap)
+    def chain(self, prob, m, nskip, ndpost, seed, probit=0, stats_mode=0, node_cap=0, trace_cap=0):
+        return OracleChain(self, prob, m, nskip, ndpost, seed, probit, stats_mode, node_cap, trace_cap)
 
     # -- glue
     def tree_text(self, trees, p):
@@ -237,13 +237,18 @@ class Oracle:
 
 
 class OracleChain:
-    def __init__(self, orc: Oracle, prob, m, nskip, ndpost, seed, probit=0, stats_mode=0, node_cap=0):
+    def __init__(self, orc: Oracle, prob, m, nskip, ndpost, seed, probit=0, stats_mode=0, node_cap=0, trace_cap=0):
         self.o = orc
         q, y, wt, nc, pg, yb = _as_inputs(prob)
         self.p, self.n = q.shape
         self.m, self.nskip, self.ndpost = m, nskip, ndpost
         self._keep = (q, y, wt, nc, pg, yb)
         opts = OrcOpts(stats_mode, node_cap, None, 0)
+        self._trace = None
+        if trace_cap:
+            self._trace = (OrcTrace * trace_cap)()
+            opts.trace = C.cast(self._trace, C.POINTER(OrcTrace))
+            opts.trace_cap = trace_cap
         self.h = orc.L.orc_chain_new(q, y, wt, nc, pg, self.n, self.p, m, nskip, ndpost, prob["tau"], prob["lambda"],
                                      prob["nu"], prob["sigest"], seed, probit,
                                      yb.ctypes.data if yb is not None else None, prob.get("offset", 0.0),
@@ -281,6 +286,11 @@ class OracleChain:
         out = np.zeros(self.n, np.float32)
         self.o.L.orc_chain_trmean(self.h, out)
         return out
+
+    def trace(self):
+        """Per-tree-update records so far (OrcTrace structures)."""
+        k = self.o.L.orc_chain_trace_len(self.h) if self._trace is not None else 0
+        return [self._trace[i] for i in range(k)]
 
     def close(self):
         if self.h:

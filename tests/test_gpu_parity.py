@@ -113,3 +113,32 @@ def test_gpu_fx64_mode_matches_oracle(oracle):
         e = oracle.fit(pb, m, nskip, nd, 7 + c, stats_mode=1)
         for k in ("sigma", "trmean", "trees", "stats"):
             assert same_bits(outs[c][k], e[k]), (c, k)
+
+
+@pytest.mark.parametrize("stats_mode", [0, 1])
+def test_gpu_streaming_variant_matches_oracle(oracle, stats_mode):
+    """1024-row tiles with the running fit streamed from HBM (the variant large chains use), forced on small chains."""
+    probs = [W.small_problem(n, 4, 20, 81 + n, "extreme", ntree=15) for n in (1000, 2600, 4133)]
+    m, nskip, nd = 15, 10, 4
+    _say("streaming variant, stats mode", stats_mode)
+    with sampler.BatchSampler(probs, m, nskip, nd, [4, 5, 6], stats_mode=stats_mode, tile_rows=1024,
+                              sweeps_per_launch=5) as b:
+        b.run(nskip + nd)
+        outs = [b.fetch(c) for c in range(3)]
+    for c, pb in enumerate(probs):
+        e = oracle.fit(pb, m, nskip, nd, 4 + c, stats_mode=stats_mode)
+        for k in ("sigma", "trmean", "trees", "stats"):
+            assert same_bits(outs[c][k], e[k]), (c, k)
+
+
+def test_gpu_maxshape_chain_fx64_matches_oracle(oracle):
+    """A chain of the max-shape workload (n = 65,536, p = 32, node cap 64, FX64), a few sweeps, against the oracle."""
+    pb = W.maxshape64_chain(0)
+    m, nskip, nd = 200, 2, 1
+    _say("max-shape chain n", pb["q"].shape[1])
+    with sampler.BatchSampler([pb], m, nskip, nd, [1], stats_mode=1, node_cap=64) as b:
+        b.run(nskip + nd)
+        out = b.fetch(0)
+    e = oracle.fit(pb, m, nskip, nd, 1, stats_mode=1, node_cap=64)
+    for k in ("sigma", "trmean", "trees", "stats"):
+        assert same_bits(out[k], e[k]), k

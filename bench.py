@@ -1,17 +1,23 @@
 """Benchmark: tree-updates/sec/GPU of the batched plain sampler, plus min-ESS/sec.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--chains 512]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch512|fwbart_n4096|fpbart_n24576|maxshape64]
 
-Workload: the batched fwbart configuration -- 512 independent chains in one persistent launch, chain c with
+Default workload (the headline): batched fwbart -- 512 independent chains in one persistent launch, chain c with
 n_c ~ U{3091..4573} rows, p_c ~ U{15..19} predictors, x ~ U(0,1) with 255 cutpoints, y = 10 sin(pi x1 x2) +
 20 (x3-0.5)^2 + 10 x4 + 5 x5 + N(0, s_c^2), s_c ~ U(0.5, 2), w ~ U(0.5, 2); ntree 200, nskip 1000, ndpost 100.
-A step is one sweep of every chain (200 tree updates per chain plus the closing pass and the sigma draw). W warmup
-sweeps run untimed, then exactly K sweeps are timed; W + K = 1100 covers the whole fit by default, so the kept draws of
+A step is one sweep of every chain (m tree updates per chain plus the closing pass and the sigma / latent draw). W
+warmup sweeps run untimed, then exactly K sweeps are timed; W + K covers the whole fit by default, so the kept draws of
 the timed region give the min-ESS figure. Inputs are resident in HBM before the timed region starts.
 
+Other workloads (extra lines, not the headline):
+  fwbart_n4096   one chain, n = 4,096, p = 16, 100 cutpoints, ntree 200, nskip 1000, ndpost 100
+  fpbart_n24576  one probit chain, n = 24,576, p = 3, ntree 50, nskip 1000, ndpost 100
+  maxshape64     64 chains, n = 65,536, p = 32 (24 continuous + 8 integer columns), node cap 64, FX64 statistics,
+                 ntree 200, nskip 500, ndpost 100 (default here: 5 warmup + 40 timed sweeps)
+
 One JSON line on stdout. Statistics mode REF16 (double sums in 16 row lanes, bit-identical to the reference CPU
-sampler). The CPU baseline is the reference's own core (oracle/_ref, built from the reference sources) timed on this
-host, one chain per process, on a bounded sample of the same workload.
+sampler) unless the workload says FX64. The CPU baseline is the reference's own core (oracle/_ref, built from the
+reference sources) timed on this host, one chain per process, on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -29,36 +35,70 @@ sys.path.insert(0, ROOT)
 
 METRIC = "tree-updates/sec/GPU (batched chains) + min-ESS/sec, learned vs plain proposal"
 HBM_PEAK_GBS = 8000.0
-M, NSKIP, NDPOST = 200, 1000, 100
+
+# name -> (chains, ntree, nskip, ndpost, probit, stats_mode, node_cap, default warmup, default timed steps or None)
+WORKLOADS = {
+    "batch512": (512, 200, 1000, 100, 0, 0, 0, 20, None),
+    "fwbart_n4096": (1, 200, 1000, 100, 0, 0, 0, 20, None),
+    "fpbart_n24576": (1, 50, 1000, 100, 1, 0, 0, 20, None),
+    "maxshape64": (64, 200, 500, 100, 0, 1, 64, 5, 40),
+}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def make_chain(workload: str, c: int) -> dict:
+    from fastbart_amd import workloads as Wk
+    if workload == "batch512":
+        return Wk.batch512_chain(c)
+    if workload == "fwbart_n4096":
+        return Wk.workload_fwbart_n4096()
+    if workload == "fpbart_n24576":
+        return Wk.workload_fpbart_n24576()
+    if workload == "maxshape64":
+        return Wk.maxshape64_chain(c)
+    raise ValueError(workload)
+
+
+def describe(workload: str, chains: int, nskip: int, ndpost: int) -> str:
+    return {
+        "batch512": f"batched fwbart, {chains} chains, n_i~U{{3091..4573}}, p_i~U{{15..19}}, 255 cutpoints, ntree 200, "
+                    f"nskip {nskip}, ndpost {ndpost}, REF16 statistics",
+        "fwbart_n4096": f"fwbart, 1 chain, n 4096, p 16, 100 cutpoints, ntree 200, nskip {nskip}, ndpost {ndpost}, "
+                        f"REF16 statistics",
+        "fpbart_n24576": f"fpbart probit, 1 chain, n 24576, p 3, ntree 50, nskip {nskip}, ndpost {ndpost}, "
+                         f"REF16 statistics",
+        "maxshape64": f"max-shape batched fwbart, {chains} chains, n 65536, p 32, 255 cutpoints, node cap 64, ntree 200, "
+                      f"nskip {nskip}, ndpost {ndpost}, FX64 statistics",
+    }[workload]
+
+
 # ------------------------------------------------------------------ CPU baseline (runs before the GPU is touched)
 def _cpu_worker(args):
-    chain, sweeps = args
-    from fastbart_amd import workloads as W
+    workload, chain, sweeps = args
     from oracle import pyoracle
-    pb = W.batch512_chain(chain)
-    if pyoracle.RefCore.available():
-        core = pyoracle.RefCore()
-        t0 = time.perf_counter()
-        core.fit(pb, M, sweeps, 0, 1 + chain, keep_trees=0)
-    else:
-        core = pyoracle.Oracle()
-        t0 = time.perf_counter()
-        core.fit(pb, M, sweeps, 0, 1 + chain, keep_trees=0)
-    return M * sweeps, time.perf_counter() - t0
+    m, probit = WORKLOADS[workload][1], WORKLOADS[workload][4]
+    pb = make_chain(workload, chain)
+    core = pyoracle.RefCore() if pyoracle.RefCore.available() else pyoracle.Oracle()
+    t0 = time.perf_counter()
+    core.fit(pb, m, sweeps, 0, 1 + chain, keep_trees=0, probit=probit)
+    return m * sweeps, time.perf_counter() - t0
 
 
-def cpu_baseline(cores: int, chains_per_core: int = 3, sweeps: int = 40):
+def cpu_baseline(workload: str, cores: int):
     from oracle import pyoracle
     kind = "reference" if pyoracle.RefCore.available() else "port"
     if kind == "port" and not os.path.exists(pyoracle.ORACLE_SO):
         pyoracle.build(ref=False)
-    jobs = [(c, sweeps) for c in range(cores * chains_per_core)]
+    nch = WORKLOADS[workload][0]
+    if nch == 1:
+        cores, jobs = 1, [(workload, 0, 400)]
+    elif workload == "maxshape64":
+        jobs = [(workload, c, 2) for c in range(cores)]
+    else:
+        jobs = [(workload, c, 40) for c in range(cores * 3)]
     ctx = mp.get_context("spawn")
     t0 = time.perf_counter()
     with ctx.Pool(cores) as pool:
@@ -66,10 +106,13 @@ def cpu_baseline(cores: int, chains_per_core: int = 3, sweeps: int = 40):
     wall = time.perf_counter() - t0
     updates = sum(r[0] for r in res)
     cpu_s = sum(r[1] for r in res)
-    return dict(value=updates / wall, unit="tree-updates/s", cores=cores, kind=kind,
-                sample=f"{len(jobs)} chains of the 512-chain workload x {sweeps} sweeps x {M} trees, one chain per "
-                       f"process on {cores} cores ({cpu_s:.1f} CPU-s, {wall:.1f} s wall; "
-                       f"{updates / cpu_s:.0f} tree-updates/s per core)")
+    m = WORKLOADS[workload][1]
+    note = " (REF16: the reference has no FX64 mode; same rows and trees)" if workload == "maxshape64" else ""
+    return dict(value=updates / cpu_s if nch == 1 else updates / wall, unit="tree-updates/s", cores=cores,
+                kind=kind,
+                sample=f"{len(jobs)} chain(s) of the workload x {jobs[0][2]} sweeps x {m} trees, one chain per process "
+                       f"on {cores} core(s) ({cpu_s:.1f} CPU-s, {wall:.1f} s wall; {updates / cpu_s:.0f} "
+                       f"tree-updates/s per core){note}")
 
 
 # ------------------------------------------------------------------ ESS (Geyer initial monotone sequence)
@@ -99,21 +142,20 @@ def ess_geyer(x: np.ndarray) -> float:
     return float(ess_geyer_cols(np.asarray(x, np.float64)[:, None])[0])
 
 
-def min_ess_chain(sigma_kept: np.ndarray, yhat: np.ndarray, max_rows: int = 0) -> float:
+def min_ess_chain(sigma_kept, yhat: np.ndarray, max_rows: int = 0) -> float:
+    """min over sigma (wbart) and every f(x_i) column of the chain's kept draws."""
     cols = yhat if not max_rows else yhat[:, np.linspace(0, yhat.shape[1] - 1, max_rows).astype(int)]
-    return float(min(ess_geyer(sigma_kept), float(ess_geyer_cols(cols).min())))
+    e = float(ess_geyer_cols(cols).min())
+    return e if sigma_kept is None else min(ess_geyer(sigma_kept), e)
 
 
 # ------------------------------------------------------------------ measured HBM traffic
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sweeps_batch512_traffic.txt")
-
-
-def measured_traffic(sweeps_per_launch: float):
+def measured_traffic(workload: str, sweeps_per_launch: float):
     """HBM bytes per launch of the sweep kernel from the committed counter summary (FETCH_SIZE + WRITE_SIZE per sweep,
-    each scaled by the ratio the same counter reads for an elementwise copy of known size), or None."""
+    each scaled by the ratio the same counter reads for a plain copy of known size), or None."""
     t = {}
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(os.path.join(ROOT, "profiles", f"k_sweeps_{workload}_traffic.txt")) as f:
             for ln in f:
                 k, sep, v = ln.partition(":")
                 if sep and k.strip() in ("fetch_bytes_per_sweep", "write_bytes_per_sweep"):
@@ -127,43 +169,47 @@ def measured_traffic(sweeps_per_launch: float):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed sweeps (default: 1100 - warmup)")
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--chains", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=None, help="timed sweeps (default: the rest of the fit)")
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="batch512", choices=sorted(WORKLOADS))
+    ap.add_argument("--chains", type=int, default=None, help="chains of a batched workload (default: all)")
     ap.add_argument("--sweeps-per-launch", type=int, default=20,
                     help="sweeps per persistent launch (the warmup and the timed sweeps split into equal launches)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ess-rows", type=int, default=0, help="rows per chain used for ESS (0 = all)")
     ap.add_argument("--calib-copy", type=int, default=0,
-                    help="also run an elementwise device copy of this many MiB (counter calibration under rocprofv3)")
+                    help="also run a plain device copy of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs every chain on a single MI355X "
               "(chains shard across GPUs with no exchange; see the design notes). Use --gpus 1.", file=sys.stderr)
         return 2
-    W_ = max(0, args.warmup)
-    K = args.steps if args.steps is not None else NSKIP + NDPOST - W_
+    wl = args.workload
+    nch, M, NSKIP, NDPOST, probit, stats_mode, node_cap, w_def, k_def = WORKLOADS[wl]
+    chains = min(nch, args.chains) if args.chains else nch
+    W_ = max(0, args.warmup if args.warmup is not None else w_def)
+    K = args.steps if args.steps is not None else (k_def if k_def is not None else NSKIP + NDPOST - W_)
     total = W_ + K
     nskip = max(0, total - NDPOST) if total >= NDPOST else 0
     ndpost = total - nskip
+    spl = max(1, min(args.sweeps_per_launch, max(W_, 1), K))
 
     cpu = None
     if not args.no_cpu:
         cores = min(16, len(os.sched_getaffinity(0)))
-        log(f"cpu baseline on {cores} cores ...")
-        cpu = cpu_baseline(cores)
+        log(f"cpu baseline on up to {cores} cores ...")
+        cpu = cpu_baseline(wl, cores)
         log("cpu baseline:", cpu)
 
     from fastbart_amd import sampler
-    from fastbart_amd import workloads as Wk
 
     t0 = time.perf_counter()
-    probs = [Wk.batch512_chain(c) for c in range(args.chains)]
-    seeds = [1 + c for c in range(args.chains)]
-    log(f"generated {args.chains} chains in {time.perf_counter() - t0:.1f} s; total rows "
-        f"{sum(p['q'].shape[1] for p in probs)}")
-    b = sampler.BatchSampler(probs, M, nskip, ndpost, seeds, keep_trees=1,
-                             sweeps_per_launch=args.sweeps_per_launch)
+    probs = [make_chain(wl, c) for c in range(chains)]
+    seeds = [1 + c for c in range(chains)]
+    rows_total = sum(p["q"].shape[1] for p in probs)
+    log(f"{wl}: generated {chains} chains in {time.perf_counter() - t0:.1f} s; total rows {rows_total}")
+    b = sampler.BatchSampler(probs, M, nskip, ndpost, seeds, probit=probit, keep_trees=1, stats_mode=stats_mode,
+                             node_cap=node_cap, sweeps_per_launch=spl)
     b.sync()
     t_w = time.perf_counter()
     if W_:
@@ -176,30 +222,28 @@ def main() -> int:
     b.run(K)
     b.sync()
     T = time.perf_counter() - t1
-    avg_ms, nl, by = b.timing()
-    ntu = args.chains * M * K
-    value = ntu / T
+    avg_ms, nl, _ = b.timing()
+    value = chains * M * K / T
     log(f"timed {K} sweeps: {T:.3f} s -> {value / 1e6:.3f} M tree-updates/s; kernel avg {avg_ms:.3f} ms over {nl} "
         f"launches")
 
     # roofline of the sweep kernel over the timed launches: algorithmic bytes (37 per row per tree update, times
     # (m+1)/m for the closing pass) per launch / average launch duration (HIP events on the sampler's stream)
-    rows_total = sum(p["q"].shape[1] for p in probs)
     bytes_per_sweep = 37.0 * rows_total * (M + 1)
     nlt = nl - nl0
     ms_launch = (avg_ms * nl - avg0 * nl0) / nlt if nlt else 0.0
     sweeps_launch = K / nlt if nlt else 0.0
     achieved = bytes_per_sweep * sweeps_launch / (ms_launch / 1e3) / 1e9 if nlt else 0.0
-    traffic = measured_traffic(sweeps_launch)
+    traffic = measured_traffic(wl, sweeps_launch)
 
     ess = None
     if ndpost >= 10:
         t2 = time.perf_counter()
         tot = 0.0
-        for c in range(args.chains):
+        for c in range(chains):
             out = b.fetch(c)
             yh = b.yhat(c)
-            tot += min_ess_chain(out["sigma"][nskip:], yh, args.ess_rows)
+            tot += min_ess_chain(None if probit else out["sigma"][nskip:], yh, args.ess_rows)
         ess = tot / (t_w + T)
         log(f"min-ESS: sum over chains {tot:.1f} ({time.perf_counter() - t2:.1f} s to compute)")
     b.close()
@@ -212,9 +256,8 @@ def main() -> int:
         "metric": METRIC, "value": value, "unit": "tree-updates/s", "n_gpus": 1, "steps": K, "warmup": W_,
         "ms_per_step": T / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"batched fwbart, {args.chains} chains, n_i~U{{3091..4573}}, p_i~U{{15..19}}, "
-                               f"255 cutpoints, ntree {M}, nskip {nskip}, ndpost {ndpost}, REF16 statistics",
-                   "chains": args.chains, "ntree": M, "rows_total": rows_total, "parallelism": "chains"},
+        "config": {"workload": describe(wl, chains, nskip, ndpost), "chains": chains, "ntree": M,
+                   "rows_total": rows_total, "parallelism": "chains"},
         "min_ess_per_sec": ess,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

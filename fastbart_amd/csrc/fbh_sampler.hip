@@ -71,7 +71,32 @@ struct Rng {
   uint32_t s0, s1, s2, s3;
 };
 
+// the sweep kernel's dynamic LDS: tile scratch (also the probit ring), then the resident running fit
+extern __shared__ __align__(16) unsigned char fbh_dyn[];
+
 __device__ __forceinline__ uint32_t ufl(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+// a wave-uniform 64-bit value (pointer or double) read from the first lane
+template <class T>
+__device__ __forceinline__ T ufl_t(T v) {
+  static_assert(sizeof(T) == 8, "64-bit values");
+  uint64_t u;
+  __builtin_memcpy(&u, &v, 8);
+  u = ((uint64_t)ufl((uint32_t)(u >> 32)) << 32) | ufl((uint32_t)u);
+  __builtin_memcpy(&v, &u, 8);
+  return v;
+}
+// loads and stores through pointers known to be in global memory (generic pointers passed to a function that is not
+// inlined would otherwise become flat accesses, which also count against the LDS wait counter)
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+  typedef const __attribute__((address_space(1))) T* gptr;
+  return *(gptr)p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  typedef __attribute__((address_space(1))) T* gptr;
+  *(gptr)p = v;
+}
 
 // xoshiro128** (Blackman & Vigna), state order of src/bart_kernel.cpp:54-58
 __device__ __forceinline__ uint32_t rng_next(Rng& g) {
@@ -86,13 +111,28 @@ __device__ __forceinline__ uint32_t rng_next(Rng& g) {
   g.s3 = (g.s3 << 11) | (g.s3 >> 21);
   return res;
 }
+// the two halves of rng_next: the output from s1 before the step, and the state step alone. k11 / k21 are 11 and 21
+// passed at run time so that the rotation is not matched as one (it then stays on the scalar unit).
+__device__ __forceinline__ uint32_t starstar(uint32_t s1) {
+  const uint32_t x = s1 * 5u;
+  return ((x << 7) | (x >> 25)) * 9u;
+}
+__device__ __forceinline__ void rng_step(Rng& g, uint32_t k11, uint32_t k21) {
+  const uint32_t t = g.s1 << 9;
+  g.s2 ^= g.s0;
+  g.s3 ^= g.s1;
+  g.s1 ^= g.s2;
+  g.s0 ^= g.s3;
+  g.s2 ^= t;
+  g.s3 = (g.s3 << k11) | (g.s3 >> k21);
+}
 // (k + 1/2) 2^-24 from the top 24 bits (:59)
 __device__ __forceinline__ float unif_f(uint32_t x) { return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f); }
 // sqrtf(-2 logf(u1)) cosf(2 pi u2) in float with the host libm images (:60). __builtin_sqrtf / __builtin_sqrt lower to
 // correctly rounded sequences (v_sqrt + FMA residual correction); HIP's __fsqrt_rn is the native approximation unless
 // OCML_BASIC_ROUNDED_OPERATIONS is defined, so it is not used anywhere.
 __device__ __forceinline__ float normal_f(uint32_t x1, uint32_t x2, const LibmTables& tb) {
-  return __builtin_sqrtf(-2.f * tb.logf_t[x1 >> 8]) * tb.cosf_t[x2 >> 8];
+  return __builtin_sqrtf(-2.f * gld(tb.logf_t + (x1 >> 8))) * gld(tb.cosf_t + (x2 >> 8));
 }
 __device__ __forceinline__ double unif_d(Rng& g) { return (double)unif_f(rng_next(g)); }
 __device__ __forceinline__ double rnorm_d(Rng& g, const LibmTables& tb) {
@@ -139,6 +179,98 @@ __device__ double rtnorm_dev(Rng& g, double mean, double tau, const LibmTables& 
   return z + mean;
 }
 
+// ------------------------------------------------------------------ probit latent redraw in windows
+// Every attempt of rtnorm (:282-292) consumes exactly two generator outputs -- one attempt *slot*: (u1, u2) for one
+// normal when tau' <= 0, or (u for -log u, u for the test) when tau' > 0. Row i's latent is rtnorm(f, -off) if y = 1,
+// -rtnorm(-f, off) if y = 0 (:543-547). Rows go in windows of kPR starting at slot S: for every (row l, offset t) of
+// the window it is decided whether an attempt of row l in slot S + l + t would accept; row l's attempts are the slots
+// S + l + r_l, S + l + r_l + 1, ... where r_l is the number of rejections of the rows before it in the window, so a
+// scan of the decision masks finds each row's accepting slot. Decisions are taken in float (double for tau' <= 0's
+// normal) with a guard band and fall back to the exact double / host-libm-image evaluation whenever the two sides of the
+// comparison are close enough that rounding could matter, so every decision is the reference's; the accepted draws are
+// then evaluated exactly from the images.
+constexpr int kPR = 64;             // rows per window
+constexpr int kBand = 16;           // attempt offsets decided per row: [b_l, b_l + kBand)
+constexpr int kRhoMax = 24;         // cap of the predicted rejections per row, in 1/16
+constexpr int kTMax = (kRhoMax * (kPR - 1)) / 16 + kBand;  // largest offset a band reaches
+constexpr int kNeed = kPR + kTMax;  // slots past a window's first slot its decisions read
+constexpr int kRing = 512;          // generated slots kept in LDS (generated 64 at a time)
+constexpr int kCkpt = 8;            // a generator-state checkpoint every kCkpt slots
+constexpr int kAhead = 256;         // slots generated past a window's first slot while its decisions run
+static_assert(kAhead >= kNeed && kRing >= kAhead + kNeed, "ring too small for the band");
+// first attempt offset decided for row l of a window: the predicted rejections before it (rho16 / 16 per row) less a
+// margin, so that the true offset falls inside [b_l, b_l + kBand) unless the window's rows deviate by more than kBand/2
+__device__ __forceinline__ int band_start(int l, int rho16) { return max(0, (rho16 * l) / 16 - kBand / 2); }
+
+struct ProbitRing {
+  uint32_t raw[kRing][2];         // s1 before each of slot k's two outputs (k mod kRing); the output is starstar(s1)
+  uint32_t ck[kRing / kCkpt][4];  // generator state before slot kCkpt * q
+  uint32_t dmask[kPR];            // row l: bit b set if the attempt in slot S + l + band_start(l) + b accepts
+  uint8_t acc[kPR];               // accepting offset of each resolved row
+  int p_done, p_used;             // rows resolved by the window, slots they used
+  uint32_t fin[4];                // generator state after the last row
+};
+static_assert(kBand == 16 && kPR == 64, "four rows of 16 offsets per wave-wide decision");
+
+struct LatentRow {
+  double mean, tau, lam;          // rtnorm's mean, tau - mean, and the exponential rate when tau - mean > 0
+  float tauf, lamf;
+  bool pos;                       // y = 1: z = rtnorm(f, -off); y = 0: z = -rtnorm(-f, off)
+};
+
+__device__ __forceinline__ LatentRow latent_row(double f, int yb, double off) {
+  LatentRow r;
+  r.pos = yb != 0;
+  r.mean = r.pos ? f : -f;
+  const double tau_in = r.pos ? -off : off;
+  r.tau = tau_in - r.mean;
+  r.lam = r.tau > 0. ? 0.5 * (r.tau + __builtin_sqrt(r.tau * r.tau + 4.0)) : 0.;
+  r.tauf = (float)r.tau;
+  r.lamf = (float)r.lam;
+  return r;
+}
+
+// exact decision of one attempt (the reference's arithmetic; image gathers)
+__device__ __forceinline__ bool latent_accepts_exact(const LatentRow& r, uint32_t x1, uint32_t x2,
+                                                     const LibmTables& tb) {
+  if (r.tau <= 0.) return (double)normal_f(x1, x2, tb) >= r.tau;  // while (z < tau) retry
+  const double u = (double)unif_f(x2);
+  const double e = -gld(tb.logd_t + (x1 >> 8));
+  const double z = e / r.lam + r.tau;
+  return !(u > exp(-0.5 * (z - r.lam) * (z - r.lam)));
+}
+
+// the same decision, from device math where it is unambiguous
+__device__ __forceinline__ bool latent_accepts(const LatentRow& r, uint32_t x1, uint32_t x2, const LibmTables& tb) {
+  if (r.tau <= 0.) {
+    // logf / cosf / sqrtf within a few ulp of the host's: |z - z_host| < 1e-6 |z|
+    const float za = sqrtf(-2.f * logf(unif_f(x1))) * cosf(6.28318531f * unif_f(x2));
+    const double d = (double)za - r.tau;
+    if (fabs(d) > 1e-5 * fabs((double)za) + 1e-30) return d >= 0.;
+  } else if (r.tau < 64.) {
+    // float evaluation of u > exp(-(z - lam)^2 / 2): relative error of the exponential below 1e-4 for tau < 64
+    const float u = unif_f(x2);
+    const float za = -logf(unif_f(x1)) / r.lamf + r.tauf;
+    const float dz = za - r.lamf;
+    const float ea = expf(-0.5f * dz * dz);
+    if (fabsf(u - ea) > 1e-3f * u) return !(u > ea);
+  }
+  return latent_accepts_exact(r, x1, x2, tb);
+}
+
+// the latent of an accepted attempt, as rtnorm returns it and the caller signs it
+__device__ __forceinline__ double latent_value(const LatentRow& r, uint32_t x1, uint32_t x2, const LibmTables& tb) {
+  double z;
+  if (r.tau <= 0.) {
+    z = (double)normal_f(x1, x2, tb);
+  } else {
+    const double e = -gld(tb.logd_t + (x1 >> 8));
+    z = e / r.lam + r.tau;
+  }
+  const double v = z + r.mean;
+  return r.pos ? v : -v;
+}
+
 // ------------------------------------------------------------------ control state of one chain's workgroup
 struct Ctl {
   TreeDev tree[2];               // tree j (being updated) and tree j+1 (proposal prepared during pass j)
@@ -160,6 +292,8 @@ struct Ctl {
   int nL, nR;
   double sL0, sL1, sR0, sR1, rss;
   double inv_s2;
+  uint32_t rot[2];               // 11 and 21 (see rng_step)
+  uint32_t pst_[4];              // generator state handed to the probit latent redraw
 };
 
 #define BITM(mask, i) (((mask) >> (i)) & 1ull)
@@ -542,13 +676,213 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
   }
 }
 
+// ------------------------------------------------------------------ probit latent redraw (all waves)
+// The control wave generates attempt slots into the LDS ring, running kAhead slots ahead of the current window; the
+// data waves decide the window's attempts and, meanwhile, evaluate and store the previous window's accepted latents;
+// then the control wave scans the masks. Starts from the generator state in c.pst and leaves there the state after
+// the last row's accepted attempt. Control flow is uniform over the workgroup.
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  return __longlong_as_double((long long)readlane_u64((uint64_t)__double_as_longlong(v), l));
+}
+
+// 64 slots from F (a multiple of 64): lane k receives slot F + k's two state words and, every kCkpt slots, a checkpoint
+__device__ __forceinline__ void ring_batch(ProbitRing& B, Rng& h, uint32_t& F, int lane, uint32_t k11, uint32_t k21) {
+  uint32_t r1 = 0, r2 = 0, c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+#pragma unroll 8
+  for (int k = 0; k < 64; k++) {
+    if (k % kCkpt == 0) {
+      const bool mine = lane == k / kCkpt;
+      c0 = mine ? h.s0 : c0;
+      c1 = mine ? h.s1 : c1;
+      c2 = mine ? h.s2 : c2;
+      c3 = mine ? h.s3 : c3;
+    }
+    const bool mine = lane == k;
+    r1 = mine ? h.s1 : r1;
+    rng_step(h, k11, k21);
+    r2 = mine ? h.s1 : r2;
+    rng_step(h, k11, k21);
+  }
+  const uint32_t base = F % kRing;
+  *reinterpret_cast<uint2*>(B.raw[base + lane]) = make_uint2(r1, r2);
+  if (lane < 64 / kCkpt) *reinterpret_cast<uint4*>(B.ck[base / kCkpt + lane]) = make_uint4(c0, c1, c2, c3);
+  F += 64;
+}
+__device__ __forceinline__ void ring_generate(ProbitRing& B, Rng& h, uint32_t& F, uint32_t upto, int lane,
+                                              uint32_t k11, uint32_t k21) {
+  while (F < upto) ring_batch(B, h, F, lane, k11, k21);
+}
+__device__ __forceinline__ uint32_t slot_x1(const ProbitRing& B, uint32_t slot) { return starstar(B.raw[slot % kRing][0]); }
+__device__ __forceinline__ uint32_t slot_x2(const ProbitRing& B, uint32_t slot) { return starstar(B.raw[slot % kRing][1]); }
+
+// Not inlined: it runs once per sweep, and with its own register budget it does not spill inside the scans.
+template <bool RES>
+__device__ __attribute__((noinline)) void probit_latents(int n_, double off_, const double* AFg_, int af_off_,
+                                                         const uint8_t* YB_, double* Y_, LibmTables tb_, uint32_t g0,
+                                                         uint32_t g1, uint32_t g2, uint32_t g3, uint32_t k11_,
+                                                         uint32_t k21_) {
+  // the arguments are the same in every lane: say so, so that loop control and the generator stay scalar
+  const int n = (int)ufl((uint32_t)n_);
+  const double off = ufl_t(off_);
+  const int af_off = (int)ufl((uint32_t)af_off_);
+  const uint8_t* __restrict__ YB = ufl_t(YB_);
+  double* __restrict__ Y = ufl_t(Y_);
+  const LibmTables tb{ufl_t(tb_.logf_t), ufl_t(tb_.cosf_t), ufl_t(tb_.logd_t)};
+  const uint32_t k11 = ufl(k11_), k21 = ufl(k21_);
+  ProbitRing& B = *reinterpret_cast<ProbitRing*>(fbh_dyn);
+  const double* __restrict__ AF = RES ? reinterpret_cast<const double*>(fbh_dyn + af_off) : ufl_t(AFg_);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  Rng h{0, 0, 0, 0};
+  uint32_t F = 0;  // slots generated so far (control wave)
+  if (wave == 0) {
+    h = Rng{ufl(g0), ufl(g1), ufl(g2), ufl(g3)};
+    ring_generate(B, h, F, kAhead, lane, k11, k21);
+  }
+  __syncthreads();
+  int row0 = 0, prow0 = 0, pdone = 0;
+  int rho16 = 8;  // predicted rejections per row, in 1/16, from the windows so far
+  uint32_t S = 0, pS = 0;
+  for (;;) {
+    const int reff = min(kPR, n - row0);
+    if (wave == 0) {
+      if (reff > 0) ring_generate(B, h, F, S + kAhead, lane, k11, k21);
+    } else {
+      // this window's rows (lane k holds row dw + 4k) and the previous window's accepted attempts: row loads first,
+      // then the image gathers of the accepted attempts, whose latency the decisions cover
+      const int dw = wave - 1;
+      const int lr = dw + kDataWaves * lane;
+      double fv = 0.;
+      int ybv = 0;
+      if (reff > 0 && lr < reff) {
+        fv = RES ? AF[row0 + lr] : gld(AF + row0 + lr);
+        ybv = gld(YB + row0 + lr);
+      }
+      const int j = tid - 64;
+      const bool fin = j < pdone;
+      double fj = 0.;
+      int ybj = 0;
+      uint32_t x1 = 0, x2 = 0;
+      float lfj = 0.f, cfj = 0.f;
+      double ldj = 0.;
+      if (fin) {
+        fj = RES ? AF[prow0 + j] : gld(AF + prow0 + j);
+        ybj = gld(YB + prow0 + j);
+        const uint32_t k = pS + (uint32_t)j + B.acc[j];
+        x1 = slot_x1(B, k);
+        x2 = slot_x2(B, k);
+        lfj = gld(tb.logf_t + (x1 >> 8));
+        cfj = gld(tb.cosf_t + (x2 >> 8));
+        ldj = gld(tb.logd_t + (x1 >> 8));
+      }
+      // decisions: data wave dw takes rows dw + 4i (i < 16), four rows per step; lane 16g + b decides offset
+      // band_start(l) + b of row l = dw + 4 (4k + g)
+      if (reff > 0) {
+        for (int k = 0; k < kPR / (4 * kDataWaves); k++) {
+          const int i = 4 * k + (lane >> 4);
+          const int l = dw + kDataWaves * i;
+          const double f = __shfl(fv, i);
+          const int yb = __shfl(ybv, i);
+          bool ok = false;
+          if (l < reff) {
+            const LatentRow r = latent_row(f, yb, off);
+            const uint32_t slot = S + (uint32_t)(l + band_start(l, rho16) + (lane & 15));
+            ok = latent_accepts(r, slot_x1(B, slot), slot_x2(B, slot), tb);
+          }
+          const uint64_t msk = __ballot(ok);
+          if (lane < 4) {
+            const int lg = dw + kDataWaves * (4 * k + lane);
+            if (lg < reff) B.dmask[lg] = (uint32_t)(msk >> (16 * lane)) & 0xFFFFu;
+          }
+        }
+      }
+      if (fin) {
+        const LatentRow rj = latent_row(fj, ybj, off);
+        double z;
+        if (rj.tau <= 0.) {
+          z = (double)(__builtin_sqrtf(-2.f * lfj) * cfj);
+        } else {
+          z = -ldj / rj.lam + rj.tau;
+        }
+        const double v = z + rj.mean;
+        gst(Y + prow0 + j, rj.pos ? v : -v);
+      }
+    }
+    __syncthreads();
+    if (reff <= 0) break;
+    if (wave == 0) {
+      // row l starts at offset r (the rejections so far) and accepts at the first set bit at or above it. Lane l packs
+      // its row's band start and mask so that the dependent scalar chain per row is a few instructions.
+      const uint32_t pk = lane < reff ? ((uint32_t)band_start(lane, rho16) << 16) | B.dmask[lane] : 0u;
+      int r = 0, done = 0;
+      uint32_t accv = 0;
+#pragma unroll 1
+      for (; done < reff; done++) {
+        const uint32_t x = __builtin_amdgcn_readlane(pk, done);
+        const uint32_t sh = (uint32_t)r - (x >> 16);
+        const uint32_t mk = sh < (uint32_t)kBand ? (x & 0xFFFFu) >> sh : 0u;
+        if (!mk) break;
+        r += __builtin_ctz(mk);
+        accv = lane == done ? (uint32_t)r : accv;
+      }
+      if (lane < done) B.acc[lane] = (uint8_t)accv;
+      int used = done + r;
+      if (done == 0) {
+        // the window's first row rejected kPT times in a row: its attempts one by one, generating as needed
+        const LatentRow r0 = latent_row(RES ? AF[row0] : gld(AF + row0), (int)gld(YB + row0), off);
+        uint32_t k = 0;
+        for (;; k++) {
+          if (S + k >= F) ring_generate(B, h, F, S + k + 1, lane, k11, k21);
+          if (latent_accepts_exact(r0, slot_x1(B, S + k), slot_x2(B, S + k), tb)) break;
+        }
+        const double z = latent_value(r0, slot_x1(B, S + k), slot_x2(B, S + k), tb);
+        if (lane == 0) gst(Y + row0, z);
+        used = (int)k + 1;
+      }
+      // keep the next window's slots generated
+      ring_generate(B, h, F, S + (uint32_t)used + kNeed, lane, k11, k21);
+      if (lane == 0) {
+        B.p_done = done;
+        B.p_used = used;
+      }
+    }
+    __syncthreads();
+    const int done = (int)ufl((uint32_t)B.p_done);
+    const int used = (int)ufl((uint32_t)B.p_used);
+    if (done >= 8) rho16 = min(kRhoMax, (3 * rho16 + (16 * (used - done)) / done + 2) / 4);
+    prow0 = row0;
+    pS = S;
+    pdone = done;
+    row0 += done > 0 ? done : 1;
+    S += (uint32_t)used;
+  }
+  if (wave == 0) {
+    // the state before slot S: its checkpoint, stepped over the slots since
+    const uint32_t q = (S / kCkpt) % (kRing / kCkpt);
+    Rng e{ufl(B.ck[q][0]), ufl(B.ck[q][1]), ufl(B.ck[q][2]), ufl(B.ck[q][3])};
+    for (uint32_t i = 0; i < 2 * (S % kCkpt); i++) rng_step(e, k11, k21);
+    if (lane == 0) {
+      B.fin[0] = e.s0;
+      B.fin[1] = e.s1;
+      B.fin[2] = e.s2;
+      B.fin[3] = e.s3;
+    }
+  }
+  __syncthreads();
+}
+
 // ------------------------------------------------------------------ the sweep kernel
-// Two chains per CU: 2 x 5 waves over 4 SIMDs needs 3 waves per SIMD, i.e. at most 168 VGPRs.
-template <int R, bool RES, bool FX>
+// Two chains per CU: 2 x 5 waves over 4 SIMDs needs 3 waves per SIMD, i.e. at most 168 VGPRs. PB: probit (latent
+// redraw after each sweep, sigma fixed) -- a separate instantiation so that the wbart kernel carries none of its code.
+template <int R, bool RES, bool FX, bool PB>
 __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(3))) void k_sweeps(
     KArgs a) {
   __shared__ Ctl c;
-  extern __shared__ __align__(16) unsigned char dyn[];
+  unsigned char* const dyn = fbh_dyn;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const bool ctl = wave == 0;
   const int dw = wave - 1;
@@ -558,6 +892,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
     uint32_t* dst = reinterpret_cast<uint32_t*>(&c.cd);
     for (int w = lane; w < (int)(sizeof(ChainDesc) / 4); w += 64) dst[w] = src[w];
     c.PG[lane] = a.PG[lane];
+    if (lane < 2) c.rot[lane] = lane ? 21u : 11u;
   }
   __syncthreads();
   const int n = c.cd.n, m = a.m, cap = a.node_cap;
@@ -795,24 +1130,32 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
     if (!ctl) data_pass<R, RES, FX>(c, dyn, P, dw, lane, m - 1, m, cb, true, false, true, keep);
     __syncthreads();
     if (ctl) {
-      if (FX) ctl_combine<true>(c, cb, true, 0, lane);
-      else ctl_combine<false>(c, cb, true, 0, lane);
-      if (a.probit) {
-        // latent redraw, rows in order (:543-547)
-        const double off = (double)c.cd.offset;
-        const uint8_t* YB = a.ybin + c.cd.row_off;
-        double* Y = a.Y + c.cd.row_off;
-        for (int i = 0; i < n; i++) {
-          const double f = RES ? P.AFl[i] : P.AF[i];
-          const double z = YB[i] == 0 ? -rtnorm_dev(g, -f, off, tb) : rtnorm_dev(g, f, -off, tb);
-          if (lane == 0) Y[i] = z;
+      if (PB) {
+        // the latents are redrawn by every wave (:543-547); sigma stays at 1
+        if (lane == 0) {
+          c.pst_[0] = g.s0;
+          c.pst_[1] = g.s1;
+          c.pst_[2] = g.s2;
+          c.pst_[3] = g.s3;
         }
       } else {
+        if (FX) ctl_combine<true>(c, cb, true, 0, lane);
+        else ctl_combine<false>(c, cb, true, 0, lane);
         const float nul = c.cd.nu * c.cd.lambda;
         const float nnu = (float)n + c.cd.nu;
         sigma = __builtin_sqrt(((double)nul + c.rss) / (2.0 * rgamma_dev(g, (double)nnu / 2.0, tb)));
       }
       if (lane == 0) a.sigma_out[c.cd.sig_off + it] = (float)sigma;
+    }
+    if (PB) {
+      __syncthreads();
+      probit_latents<RES>(n, (double)c.cd.offset, P.AF, a.af_off, a.ybin + c.cd.row_off, a.Y + c.cd.row_off, tb,
+                          ufl(c.pst_[0]), ufl(c.pst_[1]), ufl(c.pst_[2]), ufl(c.pst_[3]), ufl(c.rot[0]),
+                          ufl(c.rot[1]));
+      if (ctl) {
+        const ProbitRing& B = *reinterpret_cast<const ProbitRing*>(dyn);
+        g = Rng{ufl(B.fin[0]), ufl(B.fin[1]), ufl(B.fin[2]), ufl(B.fin[3])};
+      }
     }
     __syncthreads();
   }
@@ -881,6 +1224,8 @@ __global__ void k_trmean(const double* TM, float* out, int n, int ndpost) {
 using SweepKernel = void (*)(KArgs);
 constexpr int kRowsRes = 6;    // 1536-row tiles when the running fit is resident in LDS
 constexpr int kRowsStream = 4; // 1024-row tiles otherwise
+static_assert(sizeof(ProbitRing) <= Geo<kRowsStream>::kScratchBytes && sizeof(ProbitRing) <= Geo<kRowsRes>::kScratchBytes,
+              "the probit ring reuses the tile scratch");
 constexpr int kCtlBytes = sizeof(Ctl);
 constexpr size_t kLdsBudgetPerWG = 80 * 1024;
 inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
@@ -891,12 +1236,16 @@ struct Variant {
   bool resident;
 };
 
-inline Variant pick_variant(bool resident, bool fx) {
+template <bool PB>
+inline Variant pick_variant_pb(bool resident, bool fx) {
   if (resident)
-    return fx ? Variant{k_sweeps<kRowsRes, true, true>, Geo<kRowsRes>::kScratchBytes, true}
-              : Variant{k_sweeps<kRowsRes, true, false>, Geo<kRowsRes>::kScratchBytes, true};
-  return fx ? Variant{k_sweeps<kRowsStream, false, true>, Geo<kRowsStream>::kScratchBytes, false}
-            : Variant{k_sweeps<kRowsStream, false, false>, Geo<kRowsStream>::kScratchBytes, false};
+    return fx ? Variant{k_sweeps<kRowsRes, true, true, PB>, Geo<kRowsRes>::kScratchBytes, true}
+              : Variant{k_sweeps<kRowsRes, true, false, PB>, Geo<kRowsRes>::kScratchBytes, true};
+  return fx ? Variant{k_sweeps<kRowsStream, false, true, PB>, Geo<kRowsStream>::kScratchBytes, false}
+            : Variant{k_sweeps<kRowsStream, false, false, PB>, Geo<kRowsStream>::kScratchBytes, false};
+}
+inline Variant pick_variant(bool resident, bool fx, bool probit) {
+  return probit ? pick_variant_pb<true>(resident, fx) : pick_variant_pb<false>(resident, fx);
 }
 
 }  // namespace fbh
@@ -1097,7 +1446,7 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
       free_batch(b);
       return FBH_EINVAL;
     }
-    b->var = pick_variant(res, b->stats_mode == FBH_STATS_FX64);
+    b->var = pick_variant(res, b->stats_mode == FBH_STATS_FX64, b->probit != 0);
     b->af_off = (int)align16(b->var.scratch);
     b->dyn_bytes = (size_t)b->af_off + (res ? 8 * (size_t)b->nmax : 0);
     if (kCtlBytes + b->dyn_bytes > (size_t)lds_max) {

@@ -142,3 +142,44 @@ def test_gpu_maxshape_chain_fx64_matches_oracle(oracle):
     e = oracle.fit(pb, m, nskip, nd, 1, stats_mode=1, node_cap=64)
     for k in ("sigma", "trmean", "trees", "stats"):
         assert same_bits(out[k], e[k]), k
+
+
+@pytest.mark.parametrize("tile_rows", [0, 1024])
+def test_gpu_probit_windows_match_oracle(oracle, tile_rows):
+    """Latent redraw in windows of 64 rows: chains whose n is not a multiple of the window, resident and streamed
+    running fit, latents and generator state after every launch against the oracle."""
+    probs = [W.small_problem(n, 3, 15, 91 + n, probit=True, ntree=10) for n in (64, 130, 2999)]
+    m, nskip, nd = 10, 6, 3
+    chs = [oracle.chain(pb, m, nskip, nd, 21 + c, probit=1) for c, pb in enumerate(probs)]
+    _say("probit windows, tile_rows", tile_rows)
+    with sampler.BatchSampler(probs, m, nskip, nd, [21, 22, 23], probit=1, tile_rows=tile_rows,
+                              sweeps_per_launch=3) as b:
+        for k in range(3):
+            b.run(3)
+            for c, ch in enumerate(chs):
+                ch.sweeps(3)
+                s, e = b.state(c), ch.state()
+                assert np.array_equal(s["rng"], e["rng"]), (c, k)
+                assert same_bits(s["latent"], e["latent"]), (c, k)
+        outs = [b.fetch(c) for c in range(3)]
+    for c, ch in enumerate(chs):
+        assert same_bits(outs[c]["trees"].reshape(-1), ch.trees_out), c
+        assert same_bits(outs[c]["trmean"], ch.trmean()), c
+
+
+def test_gpu_probit_workload_chain_matches_oracle(oracle):
+    """The probit workload's chain (n = 24,576, p = 3, 50 trees), a few sweeps."""
+    pb = W.workload_fpbart_n24576()
+    m, nskip, nd = 50, 3, 2
+    _say("probit workload chain")
+    with sampler.BatchSampler([pb], m, nskip, nd, [1], probit=1) as b:
+        b.run(nskip + nd)
+        s = b.state(0)
+        out = b.fetch(0)
+    ch = oracle.chain(pb, m, nskip, nd, 1, probit=1)
+    ch.sweeps(nskip + nd)
+    e = ch.state()
+    assert np.array_equal(s["rng"], e["rng"])
+    assert same_bits(s["latent"], e["latent"])
+    assert same_bits(out["trees"].reshape(-1), ch.trees_out)
+    assert same_bits(out["trmean"], ch.trmean())

@@ -53,7 +53,9 @@ EXPORTS = ["bart_fit_cpu", "bart_wfit", "bart_set_rng", "fbh_last_status", "fbh_
            "fbh_batch_yhat", "fbh_batch_timing", "fbh_batch_destroy", "fbh_fit_batch", "fbh_tree_text", "fbh_yhat",
            "fbh_compile_trees", "fbh_trees_info", "fbh_trees_arrays", "fbh_trees_free", "fbh_predict_draw",
            "fbh_predict_draws", "fbh_rows_upload", "fbh_rows_free", "fbh_predict_rows", "fbh_predict_timing",
-           "fbh_debug_math", "fbh_debug_libm_host", "fbh_debug_copy", "fbh_batch_trace", "fbh_version"]
+           "fbh_debug_math", "fbh_debug_libm_host", "fbh_debug_copy", "fbh_batch_trace", "fbh_version",
+           "fbh_mlp_create", "fbh_mlp_nparams", "fbh_mlp_params", "fbh_mlp_forward", "fbh_mlp_grad", "fbh_mlp_train",
+           "fbh_mlp_timing", "fbh_mlp_destroy"]
 
 _lock = threading.Lock()
 _lib = None
@@ -100,6 +102,16 @@ def _declare(L):
     L.fbh_debug_math.argtypes = [i, vp, vp, C.c_long, vp, vp, i]
     L.fbh_debug_libm_host.argtypes = [i, C.c_long, C.c_long, vp]
     L.fbh_debug_copy.argtypes = [C.c_long, i, C.POINTER(d)]
+    L.fbh_mlp_create.argtypes = [vp, i, C.c_uint, i, C.POINTER(vp)]
+    L.fbh_mlp_nparams.argtypes = [vp]
+    L.fbh_mlp_nparams.restype = C.c_long
+    L.fbh_mlp_params.argtypes = [vp, vp, vp]
+    L.fbh_mlp_forward.argtypes = [vp, vp, i, vp]
+    L.fbh_mlp_grad.argtypes = [vp, vp, vp, vp, i, vp, vp]
+    L.fbh_mlp_train.argtypes = [vp, vp, vp, vp, i, i, i, C.c_float, C.c_uint, vp]
+    L.fbh_mlp_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i)]
+    L.fbh_mlp_destroy.argtypes = [vp]
+    L.fbh_mlp_destroy.restype = None
     return L
 
 

@@ -153,6 +153,29 @@ int fbh_predict_rows(const fbh_trees* t, const fbh_rows* r, const int* draws, in
 /* average predict-kernel duration (ms) and launch count since the trees were compiled */
 int fbh_predict_timing(const fbh_trees* t, double* avg_ms, int* nlaunch);
 
+/* ---------------------------------------------------------------- learned birth proposal: the network
+ * No reference counterpart (the reference draws the birth rule uniformly, src/bart_kernel.cpp:433-437). A 3-layer MLP
+ * d_in -> h1 -> h2 -> d_out (ReLU), BF16 weights and activations, FP32 accumulation on the matrix cores, FP32 master
+ * weights, Adam. Parameters are packed W1 [d_in][h1], b1, W2 [h1][h2], b2, W3 [h2][d_out], b3 (fp32). Batches are host
+ * arrays x [M][d_in]; target [M] class indices; mask [M][d_out] (1 = the class may be chosen; null = all). */
+typedef struct fbh_mlp fbh_mlp;
+/* He-normal weights from seed, zero biases; max_batch bounds M in the calls below */
+int fbh_mlp_create(const int* dims, int max_batch, unsigned seed, int device, fbh_mlp** out);
+long fbh_mlp_nparams(const fbh_mlp* n);
+/* copy the parameters out (get) and/or in (set; resets the Adam state) */
+int fbh_mlp_params(fbh_mlp* n, float* get, const float* set);
+/* logits [M][d_out] */
+int fbh_mlp_forward(fbh_mlp* n, const float* x, int M, float* logits);
+/* mean masked-softmax cross-entropy over the batch and its gradient (packed like the parameters; either may be null) */
+int fbh_mlp_grad(fbh_mlp* n, const float* x, const int* target, const uint8_t* mask, int M, float* loss, float* grads);
+/* steps Adam steps (learning rate lr) on minibatches of batch rows drawn from the M-row dataset with a stream seeded by
+ * seed; loss_out [steps] (may be null) receives each step's mean loss */
+int fbh_mlp_train(fbh_mlp* n, const float* x, const int* target, const uint8_t* mask, int M, int steps, int batch,
+                  float lr, unsigned seed, float* loss_out);
+/* average duration (ms) of the device forward passes of fbh_mlp_forward so far, and their count */
+int fbh_mlp_timing(const fbh_mlp* n, double* fwd_ms, int* nfwd);
+void fbh_mlp_destroy(fbh_mlp* n);
+
 /* ---------------------------------------------------------------- diagnostics */
 /* Evaluate the device normal draw, logf/cosf tables and double math on host-given inputs (parity tests):
  *   kind 0: out_f[i] = normal from raw generator outputs (in_u[2i], in_u[2i+1])

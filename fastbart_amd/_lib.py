@@ -55,7 +55,8 @@ EXPORTS = ["bart_fit_cpu", "bart_wfit", "bart_set_rng", "fbh_last_status", "fbh_
            "fbh_predict_draws", "fbh_rows_upload", "fbh_rows_free", "fbh_predict_rows", "fbh_predict_timing",
            "fbh_debug_math", "fbh_debug_libm_host", "fbh_debug_copy", "fbh_batch_trace", "fbh_version",
            "fbh_mlp_create", "fbh_mlp_nparams", "fbh_mlp_params", "fbh_mlp_forward", "fbh_mlp_grad", "fbh_mlp_train",
-           "fbh_mlp_timing", "fbh_mlp_destroy"]
+           "fbh_mlp_timing", "fbh_mlp_destroy", "fbh_batch_learned_setup", "fbh_batch_run_learned",
+           "fbh_batch_learned_pairs", "fbh_batch_learned_train", "fbh_batch_learned_timing"]
 
 _lock = threading.Lock()
 _lib = None
@@ -112,6 +113,12 @@ def _declare(L):
     L.fbh_mlp_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i)]
     L.fbh_mlp_destroy.argtypes = [vp]
     L.fbh_mlp_destroy.restype = None
+    L.fbh_batch_learned_setup.argtypes = [vp, vp, C.c_float, C.c_long]
+    L.fbh_batch_run_learned.argtypes = [vp, i]
+    L.fbh_batch_learned_pairs.argtypes = [vp]
+    L.fbh_batch_learned_pairs.restype = C.c_long
+    L.fbh_batch_learned_train.argtypes = [vp, i, i, C.c_float, C.c_uint, vp]
+    L.fbh_batch_learned_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i)]
     return L
 
 

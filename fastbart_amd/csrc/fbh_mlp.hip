@@ -380,10 +380,9 @@ extern "C" int fbh_mlp_params(fbh_mlp* n, float* get, const float* set) {
 
 namespace fbh {
 // forward of rows [0, M) of the device batch xin (fp32 [M][d_in]) into n->Z (fp32 logits [Mp][p3])
-int mlp_forward_dev(fbh_mlp* n, const float* xin_dev, int M) {
+int mlp_forward_dev(fbh_mlp* n, const float* xin_dev, int M, hipStream_t s) {
   const int Mp = up64(M);
   const int* p = n->p;
-  hipStream_t s = n->stream;
   hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)(((long)Mp * p[0] + 255) / 256)), dim3(256), 0, s, xin_dev, M,
                      n->d[0], n->Xb, Mp, p[0]);
   hipLaunchKernelGGL((k_gemm<false, false, EPI_RELU_BF16>), dim3(p[1] / kBN, Mp / kBM), dim3(256), 0, s, n->Xb,
@@ -396,11 +395,11 @@ int mlp_forward_dev(fbh_mlp* n, const float* xin_dev, int M) {
   return FBH_OK;
 }
 
-// masked cross-entropy and the gradients of the mean loss over rows [0, M) (after mlp_forward_dev)
-int mlp_backward_dev(fbh_mlp* n, int M) {
+// masked cross-entropy and the gradients of the mean loss over rows [0, M) (after mlp_forward_dev); targets and masks
+// in n->tgt / n->mask
+int mlp_backward_dev(fbh_mlp* n, int M, hipStream_t s) {
   const int Mp = up64(M);
   const int* p = n->p;
-  hipStream_t s = n->stream;
   hipLaunchKernelGGL(k_softmax_xent, dim3(Mp), dim3(256), 0, s, n->Z, p[3], n->d[3], M, n->tgt, n->mask,
                      1.f / (float)M, n->dZ, n->dZb, n->loss);
   // layer 3
@@ -423,19 +422,35 @@ int mlp_backward_dev(fbh_mlp* n, int M) {
   return FBH_OK;
 }
 
-int mlp_adam_dev(fbh_mlp* n, float lr) {
+int mlp_adam_dev(fbh_mlp* n, float lr, hipStream_t s) {
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   n->adam_t++;
   const float step = lr * (float)(std::sqrt(1.0 - std::pow((double)b2, n->adam_t)) /
                                   (1.0 - std::pow((double)b1, n->adam_t)));
   for (int l = 0; l < 3; l++) {
-    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n->wcount[l] + 255) / 256)), dim3(256), 0, n->stream, n->W[l],
-                       n->mW[l], n->vW[l], n->gW[l], n->Wb[l], n->wcount[l], step, b1, b2, eps);
-    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n->bcount[l] + 255) / 256)), dim3(256), 0, n->stream, n->Bv[l],
-                       n->mB[l], n->vB[l], n->gB[l], (bf16*)nullptr, n->bcount[l], step, b1, b2, eps);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n->wcount[l] + 255) / 256)), dim3(256), 0, s, n->W[l], n->mW[l],
+                       n->vW[l], n->gW[l], n->Wb[l], n->wcount[l], step, b1, b2, eps);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n->bcount[l] + 255) / 256)), dim3(256), 0, s, n->Bv[l], n->mB[l],
+                       n->vB[l], n->gB[l], (bf16*)nullptr, n->bcount[l], step, b1, b2, eps);
   }
   FBH_HIP_CHECK(hipGetLastError());
   return FBH_OK;
+}
+
+const float* mlp_logits(const fbh_mlp* n, int* ld) {
+  *ld = n->p[3];
+  return n->Z;
+}
+void mlp_dims(const fbh_mlp* n, int* d, int* max_batch) {
+  for (int i = 0; i < 4; i++) d[i] = n->d[i];
+  *max_batch = n->maxM;
+}
+// the network's own batch buffers (device): inputs [maxM][d_in], targets [maxM], masks [maxM][d_out], losses
+void mlp_batch_buffers(fbh_mlp* n, float** x, int** tgt, uint8_t** mask, float** loss) {
+  *x = n->xin;
+  *tgt = n->tgt;
+  *mask = n->mask;
+  *loss = n->loss;
 }
 }  // namespace fbh
 
@@ -470,7 +485,7 @@ extern "C" int fbh_mlp_forward(fbh_mlp* n, const float* x, int M, float* logits)
   int rc = mlp_upload_batch(n, x, nullptr, nullptr, M);
   if (rc) return rc;
   FBH_HIP_CHECK(hipEventRecord(n->e0, n->stream));
-  if ((rc = mlp_forward_dev(n, n->xin, M))) return rc;
+  if ((rc = mlp_forward_dev(n, n->xin, M, n->stream))) return rc;
   FBH_HIP_CHECK(hipEventRecord(n->e1, n->stream));
   FBH_HIP_CHECK(hipMemcpy2DAsync(logits, sizeof(float) * n->d[3], n->Z, sizeof(float) * n->p[3],
                                  sizeof(float) * n->d[3], M, hipMemcpyDeviceToHost, n->stream));
@@ -492,7 +507,7 @@ extern "C" int fbh_mlp_grad(fbh_mlp* n, const float* x, const int* target, const
   FBH_HIP_CHECK(hipSetDevice(n->device));
   int rc = mlp_upload_batch(n, x, target, mask, M);
   if (rc) return rc;
-  if ((rc = mlp_forward_dev(n, n->xin, M)) || (rc = mlp_backward_dev(n, M))) return rc;
+  if ((rc = mlp_forward_dev(n, n->xin, M, n->stream)) || (rc = mlp_backward_dev(n, M, n->stream))) return rc;
   FBH_HIP_CHECK(hipStreamSynchronize(n->stream));
   if (loss) {
     std::vector<float> l(M);
@@ -538,7 +553,8 @@ extern "C" int fbh_mlp_train(fbh_mlp* n, const float* x, const int* target, cons
     }
     int rc = mlp_upload_batch(n, xb.data(), tb.data(), mask ? mb.data() : nullptr, batch);
     if (rc) return rc;
-    if ((rc = mlp_forward_dev(n, n->xin, batch)) || (rc = mlp_backward_dev(n, batch)) || (rc = mlp_adam_dev(n, lr)))
+    if ((rc = mlp_forward_dev(n, n->xin, batch, n->stream)) || (rc = mlp_backward_dev(n, batch, n->stream)) ||
+        (rc = mlp_adam_dev(n, lr, n->stream)))
       return rc;
     if (loss_out) {
       std::vector<float> l(batch);

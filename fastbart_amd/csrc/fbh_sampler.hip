@@ -1248,6 +1248,649 @@ inline Variant pick_variant(bool resident, bool fx, bool probit) {
   return probit ? pick_variant_pb<true>(resident, fx) : pick_variant_pb<false>(resident, fx);
 }
 
+constexpr int kRowsStreamL = kRowsStream;  // the learned pipeline streams the running fit
+
+// ==================================================================== learned birth proposal: lock-step pipeline
+// Per tree update j, over all chains of the batch: stage 1 (apply tree j-1's change, leaf sums of tree j, move type
+// and node, the node's features), the network's forward pass on the matrix cores (fbh_mlp.hip), stage 2 (the rule
+// from the mixture (1 - alpha) uniform + alpha network, split sums, Metropolis-Hastings with the proposal correction,
+// leaf values). The proposal correction (no reference counterpart; it plugs into the ratios of
+// src/bart_kernel.cpp:451 and :462): a birth with rule (v, c) multiplies pr by q_unif / q_mix, a death of a node with
+// rule (v, c) by q_mix / q_unif, where q_unif = (1/p) / (U - L + 1) is BART's rule draw and q_mix = (1 - alpha) q_unif +
+// alpha q_net(v, c | features of the node). The features are the node's partial residuals (excluding tree j) binned
+// by each variable's cutpoint rank, computed from an int64 fixed-point running fit so that they are bitwise the same
+// in the state before a birth and in the state after it (where the reverse death evaluates them).
+constexpr int kFeatBins = 16;                          // cutpoint-rank bins per variable (rank >> 4)
+constexpr int kFeat = FBH_PMAX * kFeatBins + 1;        // 513: histograms + depth
+constexpr int kCuts = 255;                             // classes (v, c) -> v * 255 + c
+constexpr int kClasses = FBH_PMAX * kCuts;             // 8160
+constexpr double kFxFit = 4294967296.0;                // running fit at 2^-32
+constexpr double kFxFeat = 1048576.0;                  // feature sums at 2^-20
+constexpr uint16_t kNoRange = 0x00FF;                  // L = 255 > U = 0
+
+struct LpState {
+  uint32_t rng[4];
+  double sigma;
+  int it, mode, nx, n_birth, n_death, n_cap;
+};
+
+struct LpArgs {
+  KArgs a;
+  Ctl* img;                 // [C] control images
+  LpState* st;              // [C]
+  long long* AFX;           // fixed-point running fit (row pools)
+  float* X;                 // [C][kFeat] features of each chain's proposal node
+  uint16_t* rg;             // [C][32] cutpoint ranges at the node, L | U << 8
+  const float* Z;           // network logits [>= C][ldz]
+  int ldz;
+  float alpha;
+  float* px;                // training pairs: features [pcap][kFeat], ranges [pcap][32], class [pcap]
+  uint16_t* prg;
+  int* ptg;
+  unsigned long long* pcount;
+  long pcap;
+  int j;
+};
+
+__device__ __forceinline__ void img_copy(void* dst, const void* src, int tid) {
+  const uint4* s = reinterpret_cast<const uint4*>(src);
+  uint4* d = reinterpret_cast<uint4*>(dst);
+  for (int w = tid; w < (int)(sizeof(Ctl) / 16); w += kBlock) d[w] = s[w];
+}
+static_assert(sizeof(Ctl) % 16 == 0, "control image copied in 16-byte words");
+
+__device__ __forceinline__ long long fit_fx(double th) { return (long long)rint(th * kFxFit); }
+
+__device__ __forceinline__ RowPtrs lp_rows(const KArgs& a, const ChainDesc& cd) {
+  RowPtrs P;
+  P.AF = a.AF + cd.row_off;
+  P.AFl = nullptr;
+  P.TM = a.TM + cd.row_off;
+  P.Y = a.Y + cd.row_off;
+  P.yf = a.yf + cd.row_off;
+  P.W = a.W + cd.row_off;
+  P.Q = a.q + cd.q_off;
+  P.LEAF = a.leaf + cd.leaf_off;
+  P.n = cd.n;
+  P.probit = 0;
+  return P;
+}
+
+// the fixed-point fit follows the change of tree jp (c.tho -> c.thn, leaf ids jp before the data pass rewrites
+// them); each data wave takes the rows of its own summation lanes, as the data pass does
+__device__ void lp_afx_apply(const Ctl& c, const RowPtrs& P, long long* __restrict__ AFX, int jp, int dw, int lane) {
+  const int n = P.n;
+  const uint8_t* lp = P.LEAF + (size_t)jp * n;
+  for (int i = 16 * (lane >> 2) + kLanesPerWave * dw + (lane & 3); i < n; i += 256) {
+    const int lf = lp[i];
+    int lf2 = lf;
+    if (c.pmode == 1 && lf == c.pnx)
+      lf2 = (int)P.Q[(size_t)c.pv * n + i] <= c.pc ? c.pL : c.pR;
+    else if (c.pmode == 2 && (lf == c.pl || lf == c.pr))
+      lf2 = c.pnx;
+    AFX[i] += fit_fx(c.thn[lf2]) - fit_fx(c.tho[lf]);
+  }
+}
+
+// leaf tables of tree buffer 0 (rank / slot of the used leaves, values) and the move type and node (:413-431, :455)
+__device__ void lp_choose(Ctl& c, Rng& g, int lane, int* mode_out, int* nx_out) {
+  const TreeDev& t = c.tree[0];
+  const uint64_t used = t.used, leafm = t.leaf;
+  const int s = lane;
+  const bool isleaf = BITM(leafm, s), isused = BITM(used, s);
+  const bool good = isused && isleaf && t.ngv[s] > 0;
+  const bool nog = isused && !isleaf && BITM(leafm, t.l[s] & 63) && BITM(leafm, t.r[s] & 63);
+  const uint64_t goodm = __ballot(good), nogm = __ballot(nog);
+  const uint64_t lm = used & leafm;
+  c.thc[0][s] = (double)t.th[s];
+  if (BITM(lm, s)) {
+    const int rk = __popcll(lm & ((1ull << s) - 1ull));
+    c.rank_of[0][s] = (uint8_t)rk;
+    c.slot_of[0][rk] = (uint8_t)s;
+  } else {
+    c.rank_of[0][s] = 0;
+  }
+  const int ngood = __popcll(goodm), nnogs = __popcll(nogm);
+  const double PBx = ngood == 0 ? 0 : (used == 1ull ? 1 : 0.5);
+  int mode, nx;
+  if (unif_d(g) < PBx) {
+    mode = 1;
+    nx = select_kth(goodm, (int)(unif_d(g) * ngood));
+  } else if (nnogs > 0) {
+    mode = 2;
+    nx = select_kth(nogm, (int)(unif_d(g) * nnogs));
+  } else {
+    mode = 0;
+    nx = 0;
+  }
+  if (lane == 0) {
+    c.nleaf[0] = __popcll(lm);
+    c.ngood[0] = ngood;
+    c.nnogs[0] = nnogs;
+    c.cmode[0] = 0;
+  }
+  *mode_out = mode;
+  *nx_out = nx;
+}
+
+template <bool FX>
+__global__ __launch_bounds__(kBlock) void k_lp_stage1(LpArgs A) {
+  __shared__ Ctl c;
+  __shared__ unsigned long long H[kFeat];
+  unsigned char* const dyn = fbh_dyn;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, dw = wave - 1;
+  const int chain = blockIdx.x, j = A.j;
+  const KArgs& a = A.a;
+  img_copy(&c, A.img + chain, tid);
+  for (int k = tid; k < kFeat; k += kBlock) H[k] = 0;
+  __syncthreads();
+  const RowPtrs P = lp_rows(a, c.cd);
+  const int n = c.cd.n, p = c.cd.p;
+  TreeDev* TR = a.trees + c.cd.tree_off;
+  LpState* S = A.st + chain;
+  long long* AFX = A.AFX + c.cd.row_off;
+  if (wave == 0) {
+    if (j == 0 && lane == 0) {
+      const double sg = S->sigma;
+      c.inv_s2 = 1.0 / (sg * sg);
+      c.pmode = 0;
+    }
+    copy_tree(&c.tree[0], &TR[j], lane);
+    wave_sync();
+    Rng g{ufl(S->rng[0]), ufl(S->rng[1]), ufl(S->rng[2]), ufl(S->rng[3])};
+    int mode, nx;
+    lp_choose(c, g, lane, &mode, &nx);
+    if (lane == 0) {
+      S->rng[0] = g.s0;
+      S->rng[1] = g.s1;
+      S->rng[2] = g.s2;
+      S->rng[3] = g.s3;
+      S->mode = mode;
+      S->nx = nx;
+      c.cnx[0] = nx;
+      c.cv[0] = mode;  // stage 1 keeps the move type here until stage 2 sets the rule
+    }
+  }
+  __syncthreads();
+  if (wave > 0) {
+    if (j > 0) lp_afx_apply(c, P, AFX, j - 1, dw, lane);
+    data_pass<kRowsStreamL, false, FX>(c, dyn, P, dw, lane, j - 1, j, 0, j > 0, true, false, false);
+  }
+  __syncthreads();
+  if (wave == 0) ctl_combine<FX>(c, 0, false, 0, lane);
+  const int mode = c.cv[0], nx = c.cnx[0];
+  if (mode) {
+    // the node's rows: the leaf nx (birth) or the two children of nx (death)
+    const TreeDev& t = c.tree[0];
+    const int l = mode == 2 ? t.l[nx] : nx, r = mode == 2 ? t.r[nx] : nx;
+    const uint8_t* lc = P.LEAF + (size_t)j * n;
+    for (int i = tid; i < n; i += kBlock) {
+      const int lf = lc[i];
+      if (lf != l && lf != r) continue;
+      const double res = (double)P.Y[i] - (double)(AFX[i] - fit_fx(c.thc[0][lf])) * (1.0 / kFxFit);
+      const double w = (double)P.W[i];
+      const unsigned long long wr = (unsigned long long)(long long)rint(w * res * kFxFeat);
+      for (int v = 0; v < p; v++) atomicAdd(&H[v * kFeatBins + (P.Q[(size_t)v * n + i] >> 4)], wr);
+      atomicAdd(&H[kFeat - 1], (unsigned long long)(long long)rint(w * kFxFeat));
+    }
+  }
+  __syncthreads();
+  float* X = A.X + (size_t)chain * kFeat;
+  if (mode) {
+    const double wt = (double)(long long)H[kFeat - 1];
+    const double inv = wt > 0 ? 1.0 / wt : 0.0;
+    for (int k = tid; k < kFeat - 1; k += kBlock)
+      X[k] = (k / kFeatBins) < p ? (float)((double)(long long)H[k] * inv) : 0.f;
+    if (tid == 0) X[kFeat - 1] = (float)c.tree[0].d[nx] * 0.125f;
+    if (wave == 0 && lane < FBH_PMAX) {
+      uint16_t rv = kNoRange;
+      if (lane < p) {
+        int L, U;
+        node_range(c.tree[0], nx, lane, c.cd.nc[lane], &L, &U);
+        if (U >= L) rv = (uint16_t)(L | (U << 8));
+      }
+      A.rg[(size_t)chain * FBH_PMAX + lane] = rv;
+    }
+  } else {
+    for (int k = tid; k < kFeat; k += kBlock) X[k] = 0.f;
+    if (wave == 0 && lane < FBH_PMAX) A.rg[(size_t)chain * FBH_PMAX + lane] = kNoRange;
+  }
+  __syncthreads();
+  img_copy(A.img + chain, &c, tid);
+}
+
+// class k = v * 255 + c allowed at the node: v < p and L_v <= c <= U_v
+__device__ __forceinline__ bool lp_valid(const uint16_t* rg, int k) {
+  const int v = k / kCuts, cc = k - v * kCuts;
+  const uint16_t r = rg[v];
+  return cc >= (int)(r & 255) && cc <= (int)(r >> 8);
+}
+
+template <bool FX>
+__global__ __launch_bounds__(kBlock) void k_lp_stage2(LpArgs A) {
+  __shared__ Ctl c;
+  __shared__ float red[kBlock];
+  __shared__ float sv[FBH_PMAX];
+  __shared__ uint16_t rg[FBH_PMAX];
+  __shared__ float zmax, zsum;
+  unsigned char* const dyn = fbh_dyn;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, dw = wave - 1;
+  const int chain = blockIdx.x, j = A.j;
+  const KArgs& a = A.a;
+  img_copy(&c, A.img + chain, tid);
+  if (tid < FBH_PMAX) {
+    rg[tid] = A.rg[(size_t)chain * FBH_PMAX + tid];
+    sv[tid] = 0.f;
+  }
+  __syncthreads();
+  const RowPtrs P = lp_rows(a, c.cd);
+  const int m = a.m, cap = a.node_cap, p = c.cd.p;
+  TreeDev* TR = a.trees + c.cd.tree_off;
+  LpState* S = A.st + chain;
+  const LibmTables tb = a.tb;
+  const int mode = c.cv[0], nx = c.cnx[0];
+  const float alpha = A.alpha;
+  // masked softmax of the chain's logits over the classes allowed at the node: max, total, per-variable totals
+  if (mode && alpha > 0.f) {
+    const float* z = A.Z + (size_t)chain * A.ldz;
+    float mx = -INFINITY;
+    for (int k = tid; k < kClasses; k += kBlock)
+      if (lp_valid(rg, k)) mx = fmaxf(mx, z[k]);
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if (lane == 0) red[wave] = mx;
+    __syncthreads();
+    if (tid == 0) {
+      float r = red[0];
+      for (int w = 1; w < kBlock / 64; w++) r = fmaxf(r, red[w]);
+      zmax = r;
+    }
+    __syncthreads();
+    mx = zmax;
+    // per-variable sums: one wave per variable group, lanes over the cuts (fixed order, so reproducible)
+    for (int v = wave; v < p; v += kBlock / 64) {
+      float s = 0.f;
+      for (int cc = lane; cc < kCuts; cc += 64) {
+        const int k = v * kCuts + cc;
+        if (lp_valid(rg, k)) s += expf(z[k] - mx);
+      }
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+      if (lane == 0) sv[v] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      float tot = 0.f;
+      for (int v = 0; v < p; v++) tot += sv[v];
+      zsum = tot;
+    }
+    __syncthreads();
+  }
+  Rng g{0, 0, 0, 0};
+  if (wave == 0) {
+    g = Rng{ufl(S->rng[0]), ufl(S->rng[1]), ufl(S->rng[2]), ufl(S->rng[3])};
+    const TreeDev& t = c.tree[0];
+    const double pb = 0.5;
+    const int ngood = c.ngood[0], nnogs = c.nnogs[0];
+    const uint64_t leafm = t.leaf, used = t.used;
+    const double PBx = ngood == 0 ? 0 : (used == 1ull ? 1 : pb);
+    const float* z = A.Z + (size_t)chain * A.ldz;
+    int cmode = 0, cv = 0, cc = 0;
+    double lpr = 0;
+    uint64_t cand = 0;
+    if (mode == 1) {
+      // the rule: uniform with probability 1 - alpha (:435-437), else from the network
+      int v = -1, cut = -1;
+      const double um = unif_d(g);
+      if (!(alpha > 0.f) || um >= (double)alpha) {
+        v = (int)(unif_d(g) * p);
+        if (v >= p) v = p - 1;
+        const uint16_t r = rg[v];
+        const int L = r & 255, U = r >> 8;
+        if (U >= L) {
+          cut = L + (int)(unif_d(g) * (U - L + 1));
+          if (cut > U) cut = U;
+        }
+      } else {
+        // inverse CDF of the masked softmax with a 48-bit uniform: variable by its total, then the cut
+        const uint32_t h1 = rng_next(g) >> 8, h2 = rng_next(g) >> 8;
+        const double u = ((double)h1 * 16777216.0 + (double)h2 + 0.5) * (1.0 / 281474976710656.0);
+        const double target = u * (double)zsum;
+        double acc = 0;
+        int vlast = -1;
+        for (int q = 0; q < p; q++) {
+          if (!(sv[q] > 0.f)) continue;
+          if (acc + (double)sv[q] > target) {
+            v = q;
+            break;
+          }
+          acc += (double)sv[q];
+          vlast = q;
+        }
+        if (v < 0 && vlast >= 0) {  // the total fell short of the target by rounding: the last variable with mass
+          v = vlast;
+          acc -= (double)sv[vlast];
+        }
+        const uint16_t r = v >= 0 ? rg[v] : kNoRange;  // no variable with mass (non-finite logits): no birth
+        const int L = r & 255, U = r >> 8;
+        cut = U >= L ? U : -1;
+        for (int c0 = L; c0 <= U; c0 += 64) {
+          const int ci = c0 + lane;
+          const float e = ci <= U ? expf(z[v * kCuts + ci] - zmax) : 0.f;
+          // inclusive prefix over the lanes
+          float pre = e;
+          for (int o = 1; o < 64; o <<= 1) {
+            const float y = __shfl_up(pre, o);
+            if (lane >= o) pre += y;
+          }
+          const uint64_t hit = __ballot(ci <= U && acc + (double)pre > target);
+          if (hit) {
+            cut = c0 + __builtin_ctzll(hit);
+            break;
+          }
+          acc += (double)__shfl(pre, 63);
+        }
+      }
+      if (v >= 0 && cut >= 0 && (int)t.d[nx] + 1 < kMaxDepth) {
+        const uint16_t r = rg[v];
+        const int bL = r & 255, bU = r >> 8;
+        const int dnx = t.d[nx];
+        const double PGnx = c.PG[dnx];
+        double PGly, PGry;
+        if (t.ngv[nx] > 1) {
+          PGly = PGry = c.PG[dnx + 1];
+        } else {
+          PGly = (cut - 1 < bL) ? 0 : c.PG[dnx + 1];
+          PGry = (bU < cut + 1) ? 0 : c.PG[dnx + 1];
+        }
+        const double PDy = (ngood > 1) ? 1 - pb : ((PGry == 0 && PGly == 0) ? 1 : 1 - pb);
+        double Pnogy;
+        if (nx == 0) {
+          Pnogy = 1;
+        } else {
+          const int pa = t.par[nx];
+          const bool panog = BITM(leafm, t.l[pa] & 63) && BITM(leafm, t.r[pa] & 63);
+          Pnogy = panog ? 1.0 / nnogs : 1.0 / (nnogs + 1);
+        }
+        const double Pbotx = 1.0 / ngood;
+        const double pr = (PGnx * (1 - PGly) * (1 - PGry) * PDy * Pnogy) / ((1 - PGnx) * Pbotx * PBx);
+        double fac = 1.0;
+        if (alpha > 0.f) {
+          const double qu = (1.0 / p) / (double)(bU - bL + 1);
+          const double qn = (double)expf(z[v * kCuts + cut] - zmax) / (double)zsum;
+          fac = qu / ((1.0 - (double)alpha) * qu + (double)alpha * qn);
+        }
+        lpr = log(pr * fac);
+        cmode = 1;
+        cv = v;
+        cc = cut;
+        const int L = first_free(used, -1, cap), R = L < 0 ? -1 : first_free(used, L, cap);
+        if (L >= 0 && R >= 0) cand = (1ull << L) | (1ull << R);
+      }
+    } else if (mode == 2) {
+      const int dny = t.d[nx], l = t.l[nx], r = t.r[nx];
+      const double PGny = c.PG[dny];
+      const double PGlx = t.ngv[l] > 0 ? c.PG[dny + 1] : 0, PGrx = t.ngv[r] > 0 ? c.PG[dny + 1] : 0;
+      const double PBy = (nx == 0) ? 1 : pb;
+      const int ng2 = ngood - (t.ngv[l] > 0) - (t.ngv[r] > 0) + 1;
+      const double Pboty = 1.0 / ng2, PDx = 1 - PBx, Pnogx = 1.0 / nnogs;
+      const double pr = ((1 - PGny) * PBy * Pboty) / (PGny * (1 - PGlx) * (1 - PGrx) * PDx * Pnogx);
+      double fac = 1.0;
+      if (alpha > 0.f) {
+        const int v = t.v[nx], cut = t.c[nx];
+        const uint16_t rr = rg[v];
+        const int bL = rr & 255, bU = rr >> 8;
+        const double qu = (1.0 / p) / (double)(bU - bL + 1);
+        const double qn = (double)expf(z[v * kCuts + cut] - zmax) / (double)zsum;
+        fac = ((1.0 - (double)alpha) * qu + (double)alpha * qn) / qu;
+      }
+      lpr = log(pr * fac);
+      cmode = 2;
+    }
+    if (lane == 0) {
+      c.cmode[0] = cmode;
+      c.cnx[0] = nx;
+      c.cv[0] = cv;
+      c.cc[0] = cc;
+      c.lpr[0] = lpr;
+      c.cand[0] = cand;
+      c.nL = c.nR = 0;
+    }
+  }
+  __syncthreads();
+  const int cmode = c.cmode[0];
+  if (cmode == 1 && wave > 0) data_pass<kRowsStreamL, false, FX>(c, dyn, P, dw, lane, j - 1, j, 0, false, true, false, false);
+  __syncthreads();
+  if (wave == 0) {
+    if (cmode == 1) ctl_combine<FX>(c, 0, false, 1, lane);
+    TreeDev& t = c.tree[0];
+    const double inv_s2 = c.inv_s2;
+    const double t2 = (double)c.cd.tau * c.cd.tau;
+    const double a_prior = 1 / t2;
+    const double lu = log_unif(g, tb);
+    int n_birth = S->n_birth, n_death = S->n_death, n_cap = S->n_cap;
+    int nmode = 0, nL_ = 0, nR_ = 0, nl_ = 0, nr_ = 0, acc_birth = 0;
+    if (cmode == 1) {
+      if (c.nL >= 5 && c.nR >= 5) {
+        const double bl = c.sL0 * inv_s2, Ml = c.sL1 * inv_s2, br = c.sR0 * inv_s2, Mr = c.sR1 * inv_s2;
+        double lalpha = c.lpr[0] + lh(bl, Ml, t2) + lh(br, Mr, t2) - lh(bl + br, Ml + Mr, t2);
+        if (lalpha > 0) lalpha = 0;
+        if (lu < lalpha) {
+          const int L = first_free(t.used, -1, cap);
+          const int Rr = L < 0 ? -1 : first_free(t.used, L, cap);
+          if (L < 0 || Rr < 0) {
+            n_cap++;
+          } else {
+            if (lane == 0) {
+              t.v[nx] = (uint8_t)c.cv[0];
+              t.c[nx] = (uint8_t)c.cc[0];
+              t.l[nx] = (uint8_t)L;
+              t.r[nx] = (uint8_t)Rr;
+              t.par[L] = t.par[Rr] = (uint8_t)nx;
+              t.l[L] = t.r[L] = t.l[Rr] = t.r[Rr] = 0;
+              t.d[L] = t.d[Rr] = (uint8_t)(t.d[nx] + 1);
+              t.used |= (1ull << L) | (1ull << Rr);
+              t.leaf = (t.leaf & ~(1ull << nx)) | (1ull << L) | (1ull << Rr);
+              c.S0[L] = c.sL0;
+              c.S1[L] = c.sL1;
+              c.S0[Rr] = c.sR0;
+              c.S1[Rr] = c.sR1;
+            }
+            wave_sync();
+            const int gL = count_goodvars_wave(t, L, c.cd.nc, p, lane);
+            const int gR = count_goodvars_wave(t, Rr, c.cd.nc, p, lane);
+            if (lane == 0) {
+              t.ngv[L] = (uint8_t)gL;
+              t.ngv[Rr] = (uint8_t)gR;
+            }
+            nmode = 1;
+            nL_ = L;
+            nR_ = Rr;
+            n_birth++;
+            acc_birth = 1;
+          }
+        }
+      }
+    } else if (cmode == 2) {
+      const int l = t.l[nx], r = t.r[nx];
+      const double bl = c.S0[l] * inv_s2, Ml = c.S1[l] * inv_s2, br = c.S0[r] * inv_s2, Mr = c.S1[r] * inv_s2;
+      double lalpha = c.lpr[0] + lh(bl + br, Ml + Mr, t2) - lh(bl, Ml, t2) - lh(br, Mr, t2);
+      if (lalpha > 0) lalpha = 0;
+      if (lu < lalpha) {
+        const double m0 = c.S0[l] + c.S0[r], m1 = c.S1[l] + c.S1[r];
+        wave_sync();
+        if (lane == 0) {
+          c.S0[nx] = m0;
+          c.S1[nx] = m1;
+          t.used &= ~((1ull << l) | (1ull << r));
+          t.leaf = (t.leaf & ~((1ull << l) | (1ull << r))) | (1ull << nx);
+          t.l[nx] = t.r[nx] = t.v[nx] = t.c[nx] = 0;
+        }
+        nmode = 2;
+        nl_ = l;
+        nr_ = r;
+        n_death++;
+      }
+    }
+    wave_sync();
+    // one normal per slot (:520-528)
+    float zn = 0.f;
+    for (int s = 0; s < cap; s++) {
+      const uint32_t x1 = rng_next(g);
+      const uint32_t x2 = rng_next(g);
+      if (lane == s) zn = normal_f(x1, x2, tb);
+    }
+    const int s = lane;
+    c.tho[s] = c.thc[0][s];
+    if (s < cap && BITM(t.leaf, s) && BITM(t.used, s)) {
+      const double b = c.S0[s] * inv_s2, M = c.S1[s] * inv_s2;
+      t.th[s] = (float)(M / (a_prior + b) + (double)zn / __builtin_sqrt(a_prior + b));
+    }
+    wave_sync();
+    c.thn[s] = (double)t.th[s];
+    const int it = S->it;
+    if (it >= a.nskip && a.keep_trees)
+      a.trees_out[c.cd.tout_off + ((size_t)(it - a.nskip) * m + j) * kNodes + lane] = pack_word(t, lane);
+    copy_tree(&TR[j], &t, lane);
+    // an accepted birth's (features, ranges, class) becomes a training pair
+    if (acc_birth && A.pcap > 0) {
+      unsigned long long k = 0;
+      if (lane == 0) k = atomicAdd(A.pcount, 1ull);
+      k = __shfl(k, 0);
+      if ((long)k < A.pcap) {
+        const float* X = A.X + (size_t)chain * kFeat;
+        for (int q = lane; q < kFeat; q += 64) A.px[(size_t)k * kFeat + q] = X[q];
+        if (lane < FBH_PMAX) A.prg[(size_t)k * FBH_PMAX + lane] = rg[lane];
+        if (lane == 0) A.ptg[k] = c.cv[0] * kCuts + c.cc[0];
+      }
+    }
+    if (lane == 0) {
+      c.pmode = nmode;
+      c.pnx = nx;
+      c.pv = c.cv[0];
+      c.pc = c.cc[0];
+      c.pL = nL_;
+      c.pR = nR_;
+      c.pl = nl_;
+      c.pr = nr_;
+      S->rng[0] = g.s0;
+      S->rng[1] = g.s1;
+      S->rng[2] = g.s2;
+      S->rng[3] = g.s3;
+      S->n_birth = n_birth;
+      S->n_death = n_death;
+      S->n_cap = n_cap;
+    }
+  }
+  __syncthreads();
+  img_copy(A.img + chain, &c, tid);
+}
+
+// closing pass of a sweep (apply tree m-1, residual sum of squares, posterior-mean sum) and the sigma draw (:536-558)
+template <bool FX>
+__global__ __launch_bounds__(kBlock) void k_lp_close(LpArgs A) {
+  __shared__ Ctl c;
+  unsigned char* const dyn = fbh_dyn;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, dw = wave - 1;
+  const int chain = blockIdx.x;
+  const KArgs& a = A.a;
+  img_copy(&c, A.img + chain, tid);
+  __syncthreads();
+  const RowPtrs P = lp_rows(a, c.cd);
+  const int m = a.m, n = c.cd.n;
+  LpState* S = A.st + chain;
+  const int it = S->it;
+  const bool keep = it >= a.nskip;
+  if (wave > 0) {
+    lp_afx_apply(c, P, A.AFX + c.cd.row_off, m - 1, dw, lane);
+    data_pass<kRowsStreamL, false, FX>(c, dyn, P, dw, lane, m - 1, m, 0, true, false, true, keep);
+  }
+  __syncthreads();
+  if (wave == 0) {
+    ctl_combine<FX>(c, 0, true, 0, lane);
+    Rng g{ufl(S->rng[0]), ufl(S->rng[1]), ufl(S->rng[2]), ufl(S->rng[3])};
+    const float nul = c.cd.nu * c.cd.lambda;
+    const float nnu = (float)n + c.cd.nu;
+    const double sigma = __builtin_sqrt(((double)nul + c.rss) / (2.0 * rgamma_dev(g, (double)nnu / 2.0, a.tb)));
+    if (lane == 0) {
+      a.sigma_out[c.cd.sig_off + it] = (float)sigma;
+      S->sigma = sigma;
+      S->it = it + 1;
+      S->rng[0] = g.s0;
+      S->rng[1] = g.s1;
+      S->rng[2] = g.s2;
+      S->rng[3] = g.s3;
+      c.pmode = 0;
+    }
+  }
+  __syncthreads();
+  img_copy(A.img + chain, &c, tid);
+}
+
+// control images and fixed-point fits from the batch state (all trees applied, between sweeps)
+__global__ __launch_bounds__(kBlock) void k_lp_begin(LpArgs A) {
+  __shared__ Ctl c;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int chain = blockIdx.x;
+  const KArgs& a = A.a;
+  if (tid < 64) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(a.desc + chain);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&c.cd);
+    for (int w = lane; w < (int)(sizeof(ChainDesc) / 4); w += 64) dst[w] = src[w];
+    c.PG[lane] = a.PG[lane];
+    if (lane < 2) c.rot[lane] = lane ? 21u : 11u;
+    if (lane == 0) c.pmode = 0;
+  }
+  const ChainState* cs = a.state + chain;
+  LpState* S = A.st + chain;
+  if (tid == 0) {
+    for (int w = 0; w < 4; w++) S->rng[w] = cs->rng[w];
+    S->sigma = cs->sigma;
+    S->it = cs->it;
+    S->mode = S->nx = 0;
+    S->n_birth = cs->n_birth;
+    S->n_death = cs->n_death;
+    S->n_cap = cs->n_cap;
+  }
+  __syncthreads();
+  const int n = c.cd.n, m = a.m;
+  const TreeDev* TR = a.trees + c.cd.tree_off;
+  const uint8_t* LEAF = a.leaf + c.cd.leaf_off;
+  long long* AFX = A.AFX + c.cd.row_off;
+  for (int i = tid; i < n; i += kBlock) {
+    long long s = 0;
+    for (int jj = 0; jj < m; jj++) s += fit_fx((double)TR[jj].th[LEAF[(size_t)jj * n + i]]);
+    AFX[i] = s;
+  }
+  __syncthreads();
+  img_copy(A.img + chain, &c, tid);
+}
+
+// a minibatch of training pairs into the network's batch buffers: features, class, allowed classes from the ranges
+__global__ void k_lp_gather(const float* __restrict__ px, const uint16_t* __restrict__ prg, const int* __restrict__ ptg,
+                            const int* __restrict__ idx, int M, float* __restrict__ x, int* __restrict__ tgt,
+                            uint8_t* __restrict__ mask) {
+  const int b = blockIdx.x, tid = threadIdx.x;
+  if (b >= M) return;
+  const long r = idx[b];
+  for (int k = tid; k < kFeat; k += blockDim.x) x[(size_t)b * kFeat + k] = px[(size_t)r * kFeat + k];
+  for (int k = tid; k < kClasses; k += blockDim.x) mask[(size_t)b * kClasses + k] = lp_valid(prg + r * FBH_PMAX, k);
+  if (tid == 0) tgt[b] = ptg[r];
+}
+
+// the learned runs' scalars back into the batch state
+__global__ void k_lp_end(LpArgs A, int C) {
+  const int chain = blockIdx.x * blockDim.x + threadIdx.x;
+  if (chain >= C) return;
+  const LpState* S = A.st + chain;
+  ChainState* cs = A.a.state + chain;
+  for (int w = 0; w < 4; w++) cs->rng[w] = S->rng[w];
+  cs->sigma = S->sigma;
+  cs->it = S->it;
+  cs->n_birth = S->n_birth;
+  cs->n_death = S->n_death;
+  cs->n_cap = S->n_cap;
+}
+
+
 }  // namespace fbh
 // ==================================================================== host side
 using namespace fbh;
@@ -1284,16 +1927,34 @@ struct fbh_batch {
   size_t rows_total = 0, q_total = 0, leaf_total = 0;
   std::vector<hipEvent_t> ev;  // start/stop pairs per launch
   double bytes_per_sweep = 0;
+  // learned birth proposal
+  fbh_mlp* net = nullptr;
+  float alpha = 0.f;
+  Ctl* d_img = nullptr;
+  LpState* d_lps = nullptr;
+  long long* d_afx = nullptr;
+  float* d_X = nullptr;
+  uint16_t* d_rg = nullptr;
+  float* d_px = nullptr;
+  uint16_t* d_prg = nullptr;
+  int* d_ptg = nullptr;
+  unsigned long long* d_pcount = nullptr;
+  int* d_idx = nullptr;
+  long pcap = 0;
+  std::vector<hipEvent_t> lev;  // start/stop pairs per learned sweep
 };
 
 static void free_batch(fbh_batch* b) {
   if (!b) return;
   if (b->device >= 0) (void)hipSetDevice(b->device);
-  void* ptrs[] = {b->d_desc, b->d_state, b->d_trees, b->d_q, b->d_Y, b->d_W, b->d_yin, b->d_AF, b->d_TM,
-                  b->d_ybin, b->d_leaf, b->d_sigma, b->d_tout, b->d_PG, b->d_trm, b->d_trace};
+  void* ptrs[] = {b->d_desc, b->d_state, b->d_trees, b->d_q,   b->d_Y,   b->d_W,   b->d_yin,    b->d_AF,
+                  b->d_TM,   b->d_ybin,  b->d_leaf,  b->d_sigma, b->d_tout, b->d_PG, b->d_trm, b->d_trace,
+                  b->d_img,  b->d_lps,   b->d_afx,   b->d_X,     b->d_rg,   b->d_px, b->d_prg, b->d_ptg,
+                  b->d_pcount, b->d_idx};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto e : b->ev) (void)hipEventDestroy(e);
+  for (auto e : b->lev) (void)hipEventDestroy(e);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -1749,4 +2410,205 @@ extern "C" void bart_wfit(const uint8_t* qin, const float* yin, const float* wti
                           int keep_trees, int* stats_out) {
   bart_fit_cpu(qin, yin, wtin, numcut, pgtab, sigma_out, trmean_out, trees_out, n, p, m, nskip, ndpost, tau, lambda,
                nu, sigest, seed, keep_trees, stats_out, 0, nullptr, 0.f);
+}
+
+// ==================================================================== learned birth proposal (host)
+static LpArgs lp_args(const fbh_batch* b, int j) {
+  LpArgs A;
+  A.a = make_args(b);
+  A.img = b->d_img;
+  A.st = b->d_lps;
+  A.AFX = b->d_afx;
+  A.X = b->d_X;
+  A.rg = b->d_rg;
+  int ld = 0;
+  A.Z = b->net ? mlp_logits(b->net, &ld) : nullptr;
+  A.ldz = ld;
+  A.alpha = b->alpha;
+  A.px = b->d_px;
+  A.prg = b->d_prg;
+  A.ptg = b->d_ptg;
+  A.pcount = b->d_pcount;
+  A.pcap = b->pcap;
+  A.j = j;
+  return A;
+}
+
+extern "C" int fbh_batch_learned_setup(fbh_batch* b, fbh_mlp* net, float alpha, long pair_capacity) {
+  if (!b || !(alpha >= 0.f && alpha <= 1.f) || (alpha > 0.f && !net) || pair_capacity < 0 ||
+      (pair_capacity > 0 && !net)) {
+    set_error(FBH_EINVAL, "fbh_batch_learned_setup: bad argument (alpha in [0, 1], a network when alpha > 0 or when "
+                          "collecting pairs)");
+    return FBH_EINVAL;
+  }
+  if (b->probit) {
+    set_error(FBH_EINVAL, "fbh_batch_learned_setup: the learned proposal runs wbart chains only");
+    return FBH_EINVAL;
+  }
+  if (net) {
+    int d[4], mb = 0;
+    mlp_dims(net, d, &mb);
+    if (d[0] != kFeat || d[3] != kClasses || mb < b->C) {
+      set_error(FBH_EINVAL, "fbh_batch_learned_setup: the network must map 513 features to 8160 classes with a batch "
+                            "of at least the chain count");
+      return FBH_EINVAL;
+    }
+  }
+  FBH_HIP_CHECK(hipSetDevice(b->device));
+  FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
+  int rc;
+  if (!b->d_img) {
+    if ((rc = dalloc(&b->d_img, b->C)) || (rc = dalloc(&b->d_lps, b->C)) || (rc = dalloc(&b->d_afx, b->rows_total)) ||
+        (rc = dalloc(&b->d_X, (size_t)b->C * kFeat)) || (rc = dalloc(&b->d_rg, (size_t)b->C * FBH_PMAX)) ||
+        (rc = dalloc(&b->d_pcount, 1)))
+      return rc;
+    FBH_HIP_CHECK(hipMemset(b->d_pcount, 0, sizeof(unsigned long long)));
+    FBH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_lp_stage1<false>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, Geo<kRowsStreamL>::kScratchBytes));
+  }
+  if (pair_capacity != b->pcap) {
+    for (void* q : {(void*)b->d_px, (void*)b->d_prg, (void*)b->d_ptg, (void*)b->d_idx})
+      if (q) (void)hipFree(q);
+    b->d_px = nullptr;
+    b->d_prg = nullptr;
+    b->d_ptg = nullptr;
+    b->d_idx = nullptr;
+    b->pcap = 0;
+    if (pair_capacity > 0) {
+      int d[4], mb = 0;
+      mlp_dims(net, d, &mb);
+      if ((rc = dalloc(&b->d_px, (size_t)pair_capacity * kFeat)) ||
+          (rc = dalloc(&b->d_prg, (size_t)pair_capacity * FBH_PMAX)) || (rc = dalloc(&b->d_ptg, pair_capacity)) ||
+          (rc = dalloc(&b->d_idx, mb)))
+        return rc;
+      b->pcap = pair_capacity;
+    }
+    FBH_HIP_CHECK(hipMemset(b->d_pcount, 0, sizeof(unsigned long long)));
+  }
+  b->net = net;
+  b->alpha = alpha;
+  return FBH_OK;
+}
+
+template <bool FX>
+static int lp_run(fbh_batch* b, int nsweeps) {
+  const size_t dyn = Geo<kRowsStreamL>::kScratchBytes;
+  for (auto fn : {reinterpret_cast<const void*>(k_lp_stage1<FX>), reinterpret_cast<const void*>(k_lp_stage2<FX>),
+                  reinterpret_cast<const void*>(k_lp_close<FX>)})
+    FBH_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+  hipLaunchKernelGGL(k_lp_begin, dim3(b->C), dim3(kBlock), 0, b->stream, lp_args(b, 0));
+  FBH_HIP_CHECK(hipGetLastError());
+  for (int sw = 0; sw < nsweeps; sw++) {
+    hipEvent_t e0, e1;
+    FBH_HIP_CHECK(hipEventCreate(&e0));
+    FBH_HIP_CHECK(hipEventCreate(&e1));
+    b->lev.push_back(e0);
+    b->lev.push_back(e1);
+    FBH_HIP_CHECK(hipEventRecord(e0, b->stream));
+    for (int j = 0; j < b->m; j++) {
+      const LpArgs A = lp_args(b, j);
+      hipLaunchKernelGGL(k_lp_stage1<FX>, dim3(b->C), dim3(kBlock), dyn, b->stream, A);
+      if (b->alpha > 0.f) {
+        const int rc = mlp_forward_dev(b->net, b->d_X, b->C, b->stream);
+        if (rc) return rc;
+      }
+      hipLaunchKernelGGL(k_lp_stage2<FX>, dim3(b->C), dim3(kBlock), dyn, b->stream, A);
+    }
+    hipLaunchKernelGGL(k_lp_close<FX>, dim3(b->C), dim3(kBlock), dyn, b->stream, lp_args(b, b->m - 1));
+    FBH_HIP_CHECK(hipGetLastError());
+    FBH_HIP_CHECK(hipEventRecord(e1, b->stream));
+  }
+  hipLaunchKernelGGL(k_lp_end, dim3((b->C + 63) / 64), dim3(64), 0, b->stream, lp_args(b, 0), b->C);
+  FBH_HIP_CHECK(hipGetLastError());
+  b->it += nsweeps;
+  return FBH_OK;
+}
+
+extern "C" int fbh_batch_run_learned(fbh_batch* b, int nsweeps) {
+  if (!b || nsweeps < 0) {
+    set_error(FBH_EINVAL, "fbh_batch_run_learned: bad argument");
+    return FBH_EINVAL;
+  }
+  if (!b->d_img) {
+    set_error(FBH_EINVAL, "fbh_batch_run_learned: call fbh_batch_learned_setup first");
+    return FBH_EINVAL;
+  }
+  FBH_HIP_CHECK(hipSetDevice(b->device));
+  nsweeps = std::min(nsweeps, b->nskip + b->ndpost - b->it);
+  if (nsweeps <= 0) return FBH_OK;
+  return b->stats_mode == FBH_STATS_FX64 ? lp_run<true>(b, nsweeps) : lp_run<false>(b, nsweeps);
+}
+
+extern "C" long fbh_batch_learned_pairs(fbh_batch* b) {
+  if (!b || !b->d_pcount) return 0;
+  if (hipSetDevice(b->device) != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) return -1;
+  unsigned long long k = 0;
+  if (hipMemcpy(&k, b->d_pcount, sizeof k, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  return (long)std::min<unsigned long long>(k, (unsigned long long)b->pcap);
+}
+
+extern "C" int fbh_batch_learned_train(fbh_batch* b, int steps, int batch, float lr, unsigned seed, float* loss_out) {
+  if (!b || !b->net || steps < 0 || batch < 1 || !(lr > 0.f)) {
+    set_error(FBH_EINVAL, "fbh_batch_learned_train: bad argument or no network");
+    return FBH_EINVAL;
+  }
+  int d[4], mb = 0;
+  mlp_dims(b->net, d, &mb);
+  if (batch > mb) {
+    set_error(FBH_EINVAL, "fbh_batch_learned_train: batch above the network's max_batch");
+    return FBH_EINVAL;
+  }
+  const long npairs = fbh_batch_learned_pairs(b);
+  if (npairs < 1) {
+    set_error(FBH_EINVAL, "fbh_batch_learned_train: no training pairs collected");
+    return FBH_EINVAL;
+  }
+  float *x, *loss;
+  int* tgt;
+  uint8_t* mask;
+  mlp_batch_buffers(b->net, &x, &tgt, &mask, &loss);
+  uint64_t st = (uint64_t)seed * 0x9E3779B97F4A7C15ull + 3;
+  std::vector<int> idx(batch);
+  std::vector<float> l(batch);
+  for (int s = 0; s < steps; s++) {
+    for (int k = 0; k < batch; k++) {
+      uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      idx[k] = (int)((z ^ (z >> 31)) % (uint64_t)npairs);
+    }
+    FBH_HIP_CHECK(hipMemcpyAsync(b->d_idx, idx.data(), sizeof(int) * batch, hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(k_lp_gather, dim3(batch), dim3(256), 0, b->stream, b->d_px, b->d_prg, b->d_ptg, b->d_idx,
+                       batch, x, tgt, mask);
+    FBH_HIP_CHECK(hipGetLastError());
+    int rc;
+    if ((rc = mlp_forward_dev(b->net, x, batch, b->stream)) || (rc = mlp_backward_dev(b->net, batch, b->stream)) ||
+        (rc = mlp_adam_dev(b->net, lr, b->stream)))
+      return rc;
+    if (loss_out) {
+      FBH_HIP_CHECK(hipMemcpyAsync(l.data(), loss, sizeof(float) * batch, hipMemcpyDeviceToHost, b->stream));
+      FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
+      double t = 0;
+      for (float v : l) t += v;
+      loss_out[s] = (float)(t / batch);
+    }
+  }
+  FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
+  return FBH_OK;
+}
+
+extern "C" int fbh_batch_learned_timing(const fbh_batch* b, double* avg_ms, int* nsweeps) {
+  if (!b) return FBH_EINVAL;
+  double tot = 0;
+  int k = 0;
+  for (size_t i = 0; i + 1 < b->lev.size(); i += 2) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, b->lev[i], b->lev[i + 1]) == hipSuccess) {
+      tot += ms;
+      k++;
+    }
+  }
+  if (avg_ms) *avg_ms = k ? tot / k : 0;
+  if (nsweeps) *nsweeps = k;
+  return FBH_OK;
 }

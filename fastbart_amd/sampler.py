@@ -104,6 +104,31 @@ class BatchSampler:
         check(lib().fbh_batch_timing(self.h, C.byref(avg), C.byref(k), C.byref(by)), "fbh_batch_timing")
         return avg.value, k.value, by.value
 
+    # -- learned birth proposal
+    def learned_setup(self, net=None, alpha=0.0, pair_capacity=0):
+        """Mixture weight alpha of the network (fastbart_amd.mlp.MLP, 513 -> ... -> 8160) in the birth rule draw;
+        pair_capacity > 0 records accepted births as training pairs."""
+        self._net = net
+        check(lib().fbh_batch_learned_setup(self.h, net.h if net is not None else None, float(alpha),
+                                            int(pair_capacity)), "fbh_batch_learned_setup")
+
+    def run_learned(self, nsweeps):
+        check(lib().fbh_batch_run_learned(self.h, int(nsweeps)), "fbh_batch_run_learned")
+
+    def learned_pairs(self):
+        return int(lib().fbh_batch_learned_pairs(self.h))
+
+    def learned_train(self, steps, batch=512, lr=1e-3, seed=1):
+        losses = np.zeros(max(int(steps), 1), np.float32)
+        check(lib().fbh_batch_learned_train(self.h, int(steps), int(batch), float(lr), int(seed) & 0xFFFFFFFF,
+                                            ptr(losses)), "fbh_batch_learned_train")
+        return losses[:steps]
+
+    def learned_timing(self):
+        ms, k = C.c_double(), C.c_int()
+        check(lib().fbh_batch_learned_timing(self.h, C.byref(ms), C.byref(k)), "fbh_batch_learned_timing")
+        return ms.value, k.value
+
     def enable_trace(self, cap):
         check(lib().fbh_batch_trace(self.h, int(cap), None, 0), "fbh_batch_trace")
         self._trace_cap = cap

@@ -176,6 +176,24 @@ int fbh_mlp_train(fbh_mlp* n, const float* x, const int* target, const uint8_t* 
 int fbh_mlp_timing(const fbh_mlp* n, double* fwd_ms, int* nfwd);
 void fbh_mlp_destroy(fbh_mlp* n);
 
+/* ---------------------------------------------------------------- learned birth proposal: sampling with it
+ * No reference counterpart. The batch's next sweeps run tree update by tree update over all chains in lock-step:
+ * the birth rule comes from the mixture (1 - alpha) x BART's uniform rule + alpha x the network's masked softmax given
+ * the node's 32 x 16 partial-residual histogram and depth, and the Metropolis-Hastings ratios carry the exact proposal
+ * correction (births x q_unif / q_mix, deaths x q_mix / q_unif), so the posterior is unchanged. alpha = 0 is BART's
+ * proposal. wbart batches only; the network maps 513 features to 8160 = 32 x 255 classes (v * 255 + c). */
+/* attach the network (may be null when alpha = 0 and nothing is collected) and the mixture weight; pair_capacity > 0
+ * records each accepted birth's (features, allowed rules, rule) for training (the count restarts when it changes) */
+int fbh_batch_learned_setup(fbh_batch* b, fbh_mlp* net, float alpha, long pair_capacity);
+/* the next nsweeps sweeps with the learned proposal (asynchronous; outputs as for fbh_batch_run) */
+int fbh_batch_run_learned(fbh_batch* b, int nsweeps);
+/* training pairs collected so far (at most the capacity) */
+long fbh_batch_learned_pairs(fbh_batch* b);
+/* steps Adam steps on minibatches of the collected pairs (device-resident), loss_out [steps] may be null */
+int fbh_batch_learned_train(fbh_batch* b, int steps, int batch, float lr, unsigned seed, float* loss_out);
+/* average duration (ms) of the learned sweeps so far, and their count */
+int fbh_batch_learned_timing(const fbh_batch* b, double* avg_ms, int* nsweeps);
+
 /* ---------------------------------------------------------------- diagnostics */
 /* Evaluate the device normal draw, logf/cosf tables and double math on host-given inputs (parity tests):
  *   kind 0: out_f[i] = normal from raw generator outputs (in_u[2i], in_u[2i+1])

@@ -14,6 +14,10 @@ Other workloads (extra lines, not the headline):
   fpbart_n24576  one probit chain, n = 24,576, p = 3, ntree 50, nskip 1000, ndpost 100
   maxshape64     64 chains, n = 65,536, p = 32 (24 continuous + 8 integer columns), node cap 64, FX64 statistics,
                  ntree 200, nskip 500, ndpost 100 (default here: 5 warmup + 40 timed sweeps)
+  learned512     the 512-chain batch twice: the plain fit, then the learned schedule -- plain sweeps, then sweeps with
+                 BART's proposal in the lock-step pipeline collecting accepted births, training of the network on the
+                 device, then sweeps with the mixture 0.2 uniform / 0.8 network up to the end (the kept draws); reports
+                 min-ESS/s of both (training time apart) and the posterior-mean agreement
 
 One JSON line on stdout. Statistics mode REF16 (double sums in 16 row lanes, bit-identical to the reference CPU
 sampler) unless the workload says FX64. The CPU baseline is the reference's own core (oracle/_ref, built from the
@@ -42,6 +46,7 @@ WORKLOADS = {
     "fwbart_n4096": (1, 200, 1000, 100, 0, 0, 0, 20, None),
     "fpbart_n24576": (1, 50, 1000, 100, 1, 0, 0, 20, None),
     "maxshape64": (64, 200, 500, 100, 0, 1, 64, 5, 40),
+    "learned512": (512, 200, 1000, 100, 0, 0, 0, 20, None),
 }
 
 
@@ -51,7 +56,7 @@ def log(*a):
 
 def make_chain(workload: str, c: int) -> dict:
     from fastbart_amd import workloads as Wk
-    if workload == "batch512":
+    if workload in ("batch512", "learned512"):
         return Wk.batch512_chain(c)
     if workload == "fwbart_n4096":
         return Wk.workload_fwbart_n4096()
@@ -72,6 +77,8 @@ def describe(workload: str, chains: int, nskip: int, ndpost: int) -> str:
                          f"REF16 statistics",
         "maxshape64": f"max-shape batched fwbart, {chains} chains, n 65536, p 32, 255 cutpoints, node cap 64, ntree 200, "
                       f"nskip {nskip}, ndpost {ndpost}, FX64 statistics",
+        "learned512": f"batched fwbart with the learned birth proposal, {chains} chains, n_i~U{{3091..4573}}, "
+                      f"p_i~U{{15..19}}, 255 cutpoints, ntree 200, nskip {nskip}, ndpost {ndpost}, REF16 statistics",
     }[workload]
 
 
@@ -165,6 +172,92 @@ def measured_traffic(workload: str, sweeps_per_launch: float):
         return None
 
 
+# ------------------------------------------------------------------ learned vs plain
+def learned_vs_plain(args, chains, nskip, ndpost, cpu):
+    """Plain fit and learned schedule on the same chains and seeds; min-ESS/s of both over the kept draws."""
+    from fastbart_amd import sampler
+    from fastbart_amd.mlp import MLP
+    total = nskip + ndpost
+    n_plain0 = max(0, nskip - args.collect - args.learned_burn)
+    probs = [make_chain("learned512", c) for c in range(chains)]
+    seeds = [1 + c for c in range(chains)]
+    rows_total = sum(p["q"].shape[1] for p in probs)
+    M = 200
+
+    def ess_of(b):
+        tot = 0.0
+        means = []
+        for c in range(chains):
+            out = b.fetch(c)
+            yh = b.yhat(c)
+            tot += min_ess_chain(out["sigma"][nskip:], yh, args.ess_rows)
+            means.append(out["trmean"].astype(np.float64))
+        return tot, means
+
+    # plain
+    b = sampler.BatchSampler(probs, M, nskip, ndpost, seeds, keep_trees=1, sweeps_per_launch=20)
+    b.sync()
+    t0 = time.perf_counter()
+    b.run(total)
+    b.sync()
+    t_plain = time.perf_counter() - t0
+    ess_plain, mean_plain = ess_of(b)
+    b.close()
+    log(f"plain: {total} sweeps {t_plain:.2f} s, sum min-ESS {ess_plain:.1f}")
+    # learned schedule
+    net = MLP((513, 256, 256, 8160), max_batch=max(chains, 512), seed=5)
+    b = sampler.BatchSampler(probs, M, nskip, ndpost, seeds, keep_trees=1, sweeps_per_launch=20)
+    b.sync()
+    t0 = time.perf_counter()
+    b.run(n_plain0)
+    b.learned_setup(net, 0.0, pair_capacity=args.pairs)
+    b.run_learned(args.collect)
+    b.sync()
+    t_a = time.perf_counter() - t0
+    npairs = b.learned_pairs()
+    t1 = time.perf_counter()
+    losses = b.learned_train(args.train_steps, batch=512, lr=args.lr, seed=3)
+    t_train = time.perf_counter() - t1
+    b.learned_setup(net, args.alpha, pair_capacity=0)
+    t2 = time.perf_counter()
+    b.run_learned(total - n_plain0 - args.collect)
+    b.sync()
+    t_b = time.perf_counter() - t2
+    t_learned = t_a + t_b
+    ms_sweep, nls = b.learned_timing()
+    ess_learned, mean_learned = ess_of(b)
+    st = np.array([b.fetch(c)["stats"] for c in range(chains)])
+    b.close()
+    fwd_ms, _ = net.timing() if hasattr(net, "timing") else (0.0, 0)
+    corr = float(np.mean([np.corrcoef(a, bb)[0, 1] for a, bb in zip(mean_plain, mean_learned)]))
+    log(f"learned: {t_learned:.2f} s sampling + {t_train:.2f} s training on {npairs} pairs "
+        f"(loss {losses[:20].mean():.3f} -> {losses[-20:].mean():.3f}); sum min-ESS {ess_learned:.1f}; "
+        f"posterior-mean correlation with plain {corr:.4f}")
+    value = chains * M * total / t_learned
+    ess_ps, ess_pp = ess_learned / t_learned, ess_plain / t_plain
+    line = {
+        "metric": METRIC, "value": value, "unit": "tree-updates/s", "n_gpus": 1, "steps": total, "warmup": 0,
+        "ms_per_step": t_learned / total * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": describe("learned512", chains, nskip, ndpost), "chains": chains, "ntree": M,
+                   "rows_total": rows_total, "parallelism": "chains",
+                   "schedule": f"{n_plain0} plain sweeps, {args.collect} sweeps collecting births (alpha 0), "
+                               f"{args.train_steps} Adam steps, {total - n_plain0 - args.collect} sweeps at alpha "
+                               f"{args.alpha}"},
+        "min_ess_per_sec": ess_ps,
+        "plain": {"min_ess_per_sec": ess_pp, "seconds": t_plain, "sum_min_ess": ess_plain},
+        "learned": {"min_ess_per_sec": ess_ps, "seconds_sampling": t_learned, "seconds_training": t_train,
+                    "sum_min_ess": ess_learned, "pairs": npairs, "ms_per_learned_sweep": ms_sweep,
+                    "learned_sweeps": nls, "births": int(st[:, 0].sum()), "deaths": int(st[:, 1].sum())},
+        "ess_per_sec_ratio_learned_over_plain": ess_ps / ess_pp if ess_pp else None,
+        "posterior_mean_corr": corr,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 # ------------------------------------------------------------------ main
 def main() -> int:
     ap = argparse.ArgumentParser()
@@ -179,6 +272,13 @@ def main() -> int:
     ap.add_argument("--ess-rows", type=int, default=0, help="rows per chain used for ESS (0 = all)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration under rocprofv3)")
+    ap.add_argument("--ndpost", type=int, default=None, help="kept sweeps (learned512; longer runs steady ESS)")
+    ap.add_argument("--collect", type=int, default=100, help="learned512: sweeps collecting births")
+    ap.add_argument("--learned-burn", type=int, default=100, help="learned512: burn-in sweeps at the mixture")
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--pairs", type=int, default=2000000, help="learned512: training-pair capacity")
+    ap.add_argument("--alpha", type=float, default=0.8)
+    ap.add_argument("--lr", type=float, default=1e-3)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs every chain on a single MI355X "
@@ -187,6 +287,12 @@ def main() -> int:
     wl = args.workload
     nch, M, NSKIP, NDPOST, probit, stats_mode, node_cap, w_def, k_def = WORKLOADS[wl]
     chains = min(nch, args.chains) if args.chains else nch
+    if wl == "learned512":
+        cpu = None
+        if not args.no_cpu:
+            cores = min(16, len(os.sched_getaffinity(0)))
+            cpu = cpu_baseline("batch512", cores)
+        return learned_vs_plain(args, chains, NSKIP, args.ndpost or NDPOST, cpu)
     W_ = max(0, args.warmup if args.warmup is not None else w_def)
     K = args.steps if args.steps is not None else (k_def if k_def is not None else NSKIP + NDPOST - W_)
     total = W_ + K

@@ -80,3 +80,19 @@ class MLP:
             self.close()
         except Exception:
             pass
+
+
+def _time_forward(batch=512, reps=20):
+    """Average device time of the proposal network's forward pass (513 -> 256 -> 256 -> 8160) on a fixed batch."""
+    net = MLP((513, 256, 256, 8160), max_batch=batch, seed=1)
+    x = np.random.default_rng(0).standard_normal((batch, 513)).astype(np.float32)
+    for _ in range(3 + reps):
+        net.forward(x)
+    ms, k = net.timing()
+    flop = 2 * batch * (513 * 256 + 256 * 256 + 256 * 8160)
+    print(f"proposal network forward, batch {batch}: {ms:.4f} ms average over {k} calls, "
+          f"{flop / ms / 1e9:.1f} TFLOP/s ({flop / 1e9:.3f} GFLOP per call)", flush=True)
+
+
+if __name__ == "__main__":
+    _time_forward()

@@ -57,12 +57,12 @@ def quantize(x: np.ndarray, xi: np.ndarray) -> np.ndarray:
 
 
 def ols_sigma(X: np.ndarray, y: np.ndarray) -> float:
-    """Residual standard error of y ~ 1 + X (the lm() estimate R/fwbart.R:80 uses)."""
+    """Residual standard error of y ~ 1 + X (the lm() estimate R/fwbart.R:24 uses)."""
     n, p = X.shape
     A = np.column_stack([np.ones(n), X])
-    coef, *_ = np.linalg.lstsq(A, y, rcond=None)
+    coef, _, rank, _ = np.linalg.lstsq(A, y, rcond=None)
     r = y - A @ coef
-    return float(np.sqrt(r @ r / (n - p - 1)))
+    return float(np.sqrt(r @ r / (n - rank)))  # lm() drops aliased columns: residual df = n - rank
 
 
 def wbart_problem(X: np.ndarray, y: np.ndarray, w: np.ndarray | None = None, ntree: int = 200, numcut: int = 100,

@@ -97,6 +97,24 @@ __device__ __forceinline__ void gst(T* p, T v) {
   typedef __attribute__((address_space(1))) T* gptr;
   *(gptr)p = v;
 }
+// element i of a global array whose base is wave-uniform: a 32-bit byte offset from the scalar base, so that no 64-bit
+// per-row address is formed (nor kept live across loops)
+template <class T>
+__device__ __forceinline__ T ldo(const T* base, uint32_t i) {
+  typedef const __attribute__((address_space(1))) T* gptr;
+  return *(gptr)(reinterpret_cast<const char*>(base) + (size_t)(i * (uint32_t)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ void sto(T* base, uint32_t i, T v) {
+  typedef __attribute__((address_space(1))) T* gptr;
+  *(gptr)(reinterpret_cast<char*>(base) + (size_t)(i * (uint32_t)sizeof(T))) = v;
+}
+// a per-thread row index that the compiler may not hoist out of the tile loop (rows of the first tile are the same in
+// every pass; kept live across the sweep they would only be spilled)
+__device__ __forceinline__ uint32_t opaque(uint32_t i) {
+  asm volatile("" : "+v"(i));
+  return i;
+}
 
 // xoshiro128** (Blackman & Vigna), state order of src/bart_kernel.cpp:54-58
 __device__ __forceinline__ uint32_t rng_next(Rng& g) {
@@ -554,19 +572,47 @@ __device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, cons
 
 __device__ __forceinline__ long long fx_round(double x, double scale) { return (long long)rint(x * scale); }
 
+// One phase-A thread's loads of a tile, kept raw (one register per byte) so that nothing consumes them before phase A:
+// the loads of tile t+1 are issued before phase B of tile t and arrive while it runs.
+template <int R>
+struct TileIn {
+  uint32_t la[R], lb[R], qa[R], qb[R];  // leaf id of tree j-1, of tree j; rank on v_prev, on v_cur
+  float w[R];
+  double y[R], af[R];
+};
+
+template <int R, bool RES>
+__device__ __forceinline__ void tile_load(TileIn<R>& T, const RowPtrs& P, const uint8_t* __restrict__ la,
+                                          const uint8_t* __restrict__ lb, const uint8_t* __restrict__ qa,
+                                          const uint8_t* __restrict__ qb, int ii0, int iis, int ka) {
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const uint32_t i = opaque((uint32_t)min((ii0 + iis + 16 * r) * kLanes + ka, P.n - 1));  // rows past n re-read n-1
+    T.la[r] = ldo(la, i);
+    T.lb[r] = ldo(lb, i);
+    T.qa[r] = ldo(qa, i);
+    T.qb[r] = ldo(qb, i);
+    T.y[r] = ldo(P.Y, i);
+    T.w[r] = ldo(P.W, i);
+    T.af[r] = RES ? P.AFl[i] : ldo((const double*)P.AF, i);
+  }
+}
+
 template <int R, bool RES, bool FX>
 __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int dw, int lane, int jp, int j, int cb,
-                          bool apply, bool stats, bool rss, bool keep) {
+                          bool apply, bool stats, bool rss, bool keep, TileIn<R>& T, bool first_loaded) {
   using G = Geo<R>;
   const int n = P.n;
-  const int pmode = apply ? c.pmode : 0, pnx = c.pnx, pc = c.pc, pL = c.pL, pR = c.pR, pl = c.pl, pr = c.pr;
-  const int cmode = stats ? c.cmode[cb] : 0, cnx = c.cnx[cb], cc = c.cc[cb];
-  const int nleaf = c.nleaf[cb];
+  // every control field read here is the same for the whole workgroup: read once into scalar registers
+  const int pmode = apply ? (int)ufl(c.pmode) : 0, pnx = ufl(c.pnx), pc = ufl(c.pc), pL = ufl(c.pL), pR = ufl(c.pR),
+            pl = ufl(c.pl), pr = ufl(c.pr), pv = ufl(c.pv);
+  const int cmode = stats ? (int)ufl(c.cmode[cb]) : 0, cnx = ufl(c.cnx[cb]), cc = ufl(c.cc[cb]), cv = ufl(c.cv[cb]);
+  const int nleaf = ufl(c.nleaf[cb]);
   const int ntask = rss ? 2 : 2 * (nleaf + (cmode == 1 ? 2 : 0));
   uint8_t* __restrict__ lprev = apply ? P.LEAF + (size_t)jp * n : nullptr;
   const uint8_t* __restrict__ lcur = stats ? P.LEAF + (size_t)j * n : nullptr;
-  const uint8_t* __restrict__ qp = (apply && pmode == 1) ? P.Q + (size_t)c.pv * n : nullptr;
-  const uint8_t* __restrict__ qc = (stats && cmode == 1) ? P.Q + (size_t)c.cv[cb] * n : nullptr;
+  const uint8_t* __restrict__ qp = (apply && pmode == 1) ? P.Q + (size_t)pv * n : nullptr;
+  const uint8_t* __restrict__ qc = (stats && cmode == 1) ? P.Q + (size_t)cv * n : nullptr;
   // unused streams read a valid dummy column so that every load is unconditional
   const uint8_t* __restrict__ la = lprev ? lprev : P.LEAF;
   const uint8_t* __restrict__ lb = lcur ? lcur : P.LEAF;
@@ -594,55 +640,45 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
   const int nper_kb = kb < n ? (n - kb + kLanes - 1) / kLanes : 0;
   int my_nl = 0, my_nr = 0;
 
+  if (!first_loaded) tile_load<R, RES>(T, P, la, lb, qa, qb, 0, iis, ka);
   for (int tb = 0; tb < n; tb += G::kTile) {
     const int ii0 = tb / kLanes;
-    // ---- phase A: all of the tile's loads first (rows past n re-read row n-1), then the arithmetic
-    uint32_t bytes4[R];  // leaf id of tree j-1 | leaf id of tree j << 8 | rank on v_prev << 16 | rank on v_cur << 24
-    float wv[R];
-    double yd[R], afv[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-      const int i = min((ii0 + iis + 16 * r) * kLanes + ka, n - 1);
-      bytes4[r] = (uint32_t)la[i] | ((uint32_t)lb[i] << 8) | ((uint32_t)qa[i] << 16) | ((uint32_t)qb[i] << 24);
-      yd[r] = P.Y[i];
-      wv[r] = P.W[i];
-      afv[r] = RES ? P.AFl[i] : P.AF[i];
-    }
+    // ---- phase A: the arithmetic on the tile's loads (issued before the previous tile's phase B)
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int idx = iis + 16 * r;
-      const int i = (ii0 + idx) * kLanes + ka;
-      if (i >= n) continue;
-      double af = afv[r];
-      const uint32_t b4 = bytes4[r];
+      const int i0 = (ii0 + idx) * kLanes + ka;
+      if (i0 >= n) continue;
+      const uint32_t i = opaque((uint32_t)i0);
+      double af = T.af[r];
       if (apply) {
-        const int lf = (int)(b4 & 255);
+        const int lf = (int)T.la[r];
         int lf2 = lf;
         if (pmode == 1 && lf == pnx)
-          lf2 = (int)((b4 >> 16) & 255) <= pc ? pL : pR;
+          lf2 = (int)T.qa[r] <= pc ? pL : pR;
         else if (pmode == 2 && (lf == pl || lf == pr))
           lf2 = pnx;
         af = af - c.tho[lf] + c.thn[lf2];
-        lprev[i] = (uint8_t)lf2;
+        sto(lprev, i, (uint8_t)lf2);
         if (RES) P.AFl[i] = af;
-        else P.AF[i] = af;
+        else sto(P.AF, i, af);
       }
-      const double y = yd[r];
-      const double w = (double)wv[r];
+      const double y = T.y[r];
+      const double w = (double)T.w[r];
       if (rss) {
         const double e = y - af;
         const double c2 = (w * e) * e;
         v0a[idx] = 0.0;
         v1a[idx] = FX ? __longlong_as_double(fx_round(c2, 16777216.0)) : c2;
         tga[idx] = 0;
-        if (keep) P.TM[i] += af;
+        if (keep) sto(P.TM, i, ldo((const double*)P.TM, i) + af);
       } else {
-        const int lf = (int)((b4 >> 8) & 255);
+        const int lf = (int)T.lb[r];
         const double rr = y - af + thc[lf];
         const double wr = w * rr;
         int side = 0;
         if (cmode == 1 && lf == cnx) {
-          side = (int)(b4 >> 24) <= cc ? 1 : 2;
+          side = (int)T.qb[r] <= cc ? 1 : 2;
           my_nl += side == 1;
           my_nr += side == 2;
         }
@@ -652,6 +688,8 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
       }
     }
     wave_sync();
+    // the next tile's loads go out now and arrive during phase B
+    if (tb + G::kTile < n) tile_load<R, RES>(T, P, la, lb, qa, qb, ii0 + G::kTW4, iis, ka);
     // ---- phase B: this thread's tasks walk lane kb's rows of the tile in row order
     const int cnt = min(G::kTW4, max(0, nper_kb - ii0));
 #pragma unroll
@@ -674,6 +712,23 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
     const int task = slot + kTaskSlots * g;
     if (task < ntask) c.part[task][kb] = acc[g];
   }
+}
+
+// The next pass's first tile, loaded between the end of pass j and the end of tree j's Metropolis-Hastings step: the
+// columns it reads are fixed by then (tree j's leaf ids, tree j+1's, and the ranks on both proposal variables); whether
+// tree j's birth was accepted only decides whether phase A uses the ranks on its variable. After the last tree it is
+// the closing pass's tile.
+template <int R, bool RES>
+__device__ __forceinline__ void load_next_first_tile(TileIn<R>& T, const Ctl& c, const RowPtrs& P, int dw, int lane,
+                                                     int j, int m, int cb) {
+  const int n = P.n;
+  const int iis = lane >> 2, ka = kLanesPerWave * dw + (lane & 3);
+  const bool last = j + 1 == m;
+  const uint8_t* la = P.LEAF + (size_t)j * n;
+  const uint8_t* lb = last ? P.LEAF : P.LEAF + (size_t)(j + 1) * n;
+  const uint8_t* qa = ufl(c.cmode[cb]) == 1 ? P.Q + (size_t)ufl(c.cv[cb]) * n : P.Q;
+  const uint8_t* qb = (!last && ufl(c.cmode[cb ^ 1]) == 1) ? P.Q + (size_t)ufl(c.cv[cb ^ 1]) * n : P.Q;
+  tile_load<R, RES>(T, P, la, lb, qa, qb, 0, iis, ka);
 }
 
 // ------------------------------------------------------------------ probit latent redraw (all waves)
@@ -883,7 +938,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
     KArgs a) {
   __shared__ Ctl c;
   unsigned char* const dyn = fbh_dyn;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: the role branches below are scalar
   const bool ctl = wave == 0;
   const int dw = wave - 1;
   const int chain = blockIdx.x;
@@ -895,16 +951,17 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
     if (lane < 2) c.rot[lane] = lane ? 21u : 11u;
   }
   __syncthreads();
-  const int n = c.cd.n, m = a.m, cap = a.node_cap;
+  const int n = (int)ufl((uint32_t)c.cd.n), m = a.m, cap = a.node_cap;
+  const uint64_t row_off = ufl_t(c.cd.row_off);
   RowPtrs P;
-  P.AF = a.AF + c.cd.row_off;
+  P.AF = a.AF + row_off;
   P.AFl = reinterpret_cast<double*>(dyn + a.af_off);
-  P.TM = a.TM + c.cd.row_off;
-  P.Y = a.Y + c.cd.row_off;
-  P.yf = a.yf + c.cd.row_off;
-  P.W = a.W + c.cd.row_off;
-  P.Q = a.q + c.cd.q_off;
-  P.LEAF = a.leaf + c.cd.leaf_off;
+  P.TM = a.TM + row_off;
+  P.Y = a.Y + row_off;
+  P.yf = a.yf + row_off;
+  P.W = a.W + row_off;
+  P.Q = a.q + ufl_t(c.cd.q_off);
+  P.LEAF = a.leaf + ufl_t(c.cd.leaf_off);
   P.n = n;
   P.probit = a.probit;
   TreeDev* TR = a.trees + c.cd.tree_off;
@@ -915,20 +972,14 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
   if (RES && !ctl)
     for (int i = tid - 64; i < n; i += kBlock - 64) P.AFl[i] = P.AF[i];
 
-  Rng g{0, 0, 0, 0};
-  double sigma = 0;
-  int n_birth = 0, n_death = 0, n_cap = 0;
+  // The two roles run separate loops with the same sequence of workgroup barriers per sweep (1 + 2 m + 1, and 2 more
+  // for the probit redraw), so that neither role's live registers are held across the other's code.
   if (ctl) {
-    g = Rng{ufl(cs->rng[0]), ufl(cs->rng[1]), ufl(cs->rng[2]), ufl(cs->rng[3])};
-    sigma = cs->sigma;
-    n_birth = cs->n_birth;
-    n_death = cs->n_death;
-    n_cap = cs->n_cap;
-  }
-
-  for (int it = a.it0; it < a.it1; it++) {
-    const bool keep = it >= a.nskip;
-    if (ctl) {
+    Rng g{ufl(cs->rng[0]), ufl(cs->rng[1]), ufl(cs->rng[2]), ufl(cs->rng[3])};
+    double sigma = cs->sigma;
+    int n_birth = cs->n_birth, n_death = cs->n_death, n_cap = cs->n_cap;
+    for (int it = a.it0; it < a.it1; it++) {
+      const bool keep = it >= a.nskip;
       if (lane == 0) {
         c.inv_s2 = 1.0 / (sigma * sigma);
         c.pmode = 0;
@@ -937,13 +988,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       copy_tree(&c.tree[0], &TR[0], lane);
       wave_sync();
       ctl_propose(c, 0, g, cap, lane);
-    }
-    __syncthreads();
-    int cb = 0;
-    for (int j = 0; j < m; j++) {
-      if (!ctl) {
-        data_pass<R, RES, FX>(c, dyn, P, dw, lane, j - 1, j, cb, j > 0, true, false, false);
-      } else {
+      __syncthreads();
+      int cb = 0;
+      for (int j = 0; j < m; j++) {
         // the previous update's tree (buffer cb ^ 1) is stored before that buffer receives tree j+1
         if (j > 0) {
           const TreeDev& tp = c.tree[cb ^ 1];
@@ -986,9 +1033,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         }
         if (want) c.z[lane] = __builtin_sqrtf(-2.f * lf) * cf;
         if (lane == 0) c.lu = luv;
-      }
-      __syncthreads();
-      if (ctl) {
+        __syncthreads();
         // ---- lane sums, Metropolis-Hastings (:469-518), leaf values (:520-528)
         const int cmode = c.cmode[cb];
         if (FX) ctl_combine<true>(c, cb, false, cmode, lane);
@@ -1116,20 +1161,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
             }
           }
         }
+        __syncthreads();
+        cb ^= 1;
       }
-      __syncthreads();
-      cb ^= 1;
-    }
-    // ---- closing pass (:536-541); meanwhile the control wave stores the last tree (buffer cb ^ 1)
-    if (ctl) {
+      // closing pass (:536-541) on the data waves; meanwhile the last tree (buffer cb ^ 1) is stored
       const TreeDev& tp = c.tree[cb ^ 1];
       if (keep && a.keep_trees)
         a.trees_out[c.cd.tout_off + ((size_t)(it - a.nskip) * m + (m - 1)) * kNodes + lane] = pack_word(tp, lane);
       copy_tree(&TR[m - 1], &tp, lane);
-    }
-    if (!ctl) data_pass<R, RES, FX>(c, dyn, P, dw, lane, m - 1, m, cb, true, false, true, keep);
-    __syncthreads();
-    if (ctl) {
+      __syncthreads();
       if (PB) {
         // the latents are redrawn by every wave (:543-547); sigma stays at 1
         if (lane == 0) {
@@ -1146,31 +1186,55 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         sigma = __builtin_sqrt(((double)nul + c.rss) / (2.0 * rgamma_dev(g, (double)nnu / 2.0, tb)));
       }
       if (lane == 0) a.sigma_out[c.cd.sig_off + it] = (float)sigma;
-    }
-    if (PB) {
-      __syncthreads();
-      probit_latents<RES>(n, (double)c.cd.offset, P.AF, a.af_off, a.ybin + c.cd.row_off, a.Y + c.cd.row_off, tb,
-                          ufl(c.pst_[0]), ufl(c.pst_[1]), ufl(c.pst_[2]), ufl(c.pst_[3]), ufl(c.rot[0]),
-                          ufl(c.rot[1]));
-      if (ctl) {
+      if (PB) {
+        __syncthreads();
+        probit_latents<RES>(n, (double)c.cd.offset, P.AF, a.af_off, a.ybin + c.cd.row_off, a.Y + c.cd.row_off, tb,
+                            ufl(c.pst_[0]), ufl(c.pst_[1]), ufl(c.pst_[2]), ufl(c.pst_[3]), ufl(c.rot[0]),
+                            ufl(c.rot[1]));
         const ProbitRing& B = *reinterpret_cast<const ProbitRing*>(dyn);
         g = Rng{ufl(B.fin[0]), ufl(B.fin[1]), ufl(B.fin[2]), ufl(B.fin[3])};
       }
+      __syncthreads();
     }
-    __syncthreads();
-  }
-  if (RES && !ctl)
-    for (int i = tid - 64; i < n; i += kBlock - 64) P.AF[i] = P.AFl[i];
-  if (ctl && lane == 0) {
-    cs->rng[0] = g.s0;
-    cs->rng[1] = g.s1;
-    cs->rng[2] = g.s2;
-    cs->rng[3] = g.s3;
-    cs->sigma = sigma;
-    cs->it = a.it1;
-    cs->n_birth = n_birth;
-    cs->n_death = n_death;
-    cs->n_cap = n_cap;
+    if (lane == 0) {
+      cs->rng[0] = g.s0;
+      cs->rng[1] = g.s1;
+      cs->rng[2] = g.s2;
+      cs->rng[3] = g.s3;
+      cs->sigma = sigma;
+      cs->it = a.it1;
+      cs->n_birth = n_birth;
+      cs->n_death = n_death;
+      cs->n_cap = n_cap;
+    }
+  } else {
+    TileIn<R> T;
+    for (int it = a.it0; it < a.it1; it++) {
+      const bool keep = it >= a.nskip;
+      __syncthreads();
+      int cb = 0;
+      bool pre = false;
+      for (int j = 0; j < m; j++) {
+        data_pass<R, RES, FX>(c, dyn, P, dw, lane, j - 1, j, cb, j > 0, true, false, false, T, pre);
+        __syncthreads();
+        // while the control wave decides tree j, the next pass's first tile is loaded
+        load_next_first_tile<R, RES>(T, c, P, dw, lane, j, m, cb);
+        pre = true;
+        __syncthreads();
+        cb ^= 1;
+      }
+      data_pass<R, RES, FX>(c, dyn, P, dw, lane, m - 1, m, cb, true, false, true, keep, T, pre);
+      __syncthreads();
+      if (PB) {
+        __syncthreads();
+        probit_latents<RES>(n, (double)c.cd.offset, P.AF, a.af_off, a.ybin + c.cd.row_off, a.Y + c.cd.row_off, tb,
+                            ufl(c.pst_[0]), ufl(c.pst_[1]), ufl(c.pst_[2]), ufl(c.pst_[3]), ufl(c.rot[0]),
+                            ufl(c.rot[1]));
+      }
+      __syncthreads();
+    }
+    if (RES)
+      for (int i = tid - 64; i < n; i += kBlock - 64) P.AF[i] = P.AFl[i];
   }
 }
 
@@ -1414,7 +1478,8 @@ __global__ __launch_bounds__(kBlock) void k_lp_stage1(LpArgs A) {
   __syncthreads();
   if (wave > 0) {
     if (j > 0) lp_afx_apply(c, P, AFX, j - 1, dw, lane);
-    data_pass<kRowsStreamL, false, FX>(c, dyn, P, dw, lane, j - 1, j, 0, j > 0, true, false, false);
+    TileIn<kRowsStreamL> T;
+    data_pass<kRowsStreamL, false, FX>(c, dyn, P, dw, lane, j - 1, j, 0, j > 0, true, false, false, T, false);
   }
   __syncthreads();
   if (wave == 0) ctl_combine<FX>(c, 0, false, 0, lane);
@@ -1657,7 +1722,10 @@ __global__ __launch_bounds__(kBlock) void k_lp_stage2(LpArgs A) {
   }
   __syncthreads();
   const int cmode = c.cmode[0];
-  if (cmode == 1 && wave > 0) data_pass<kRowsStreamL, false, FX>(c, dyn, P, dw, lane, j - 1, j, 0, false, true, false, false);
+  if (cmode == 1 && wave > 0) {
+    TileIn<kRowsStreamL> T;
+    data_pass<kRowsStreamL, false, FX>(c, dyn, P, dw, lane, j - 1, j, 0, false, true, false, false, T, false);
+  }
   __syncthreads();
   if (wave == 0) {
     if (cmode == 1) ctl_combine<FX>(c, 0, false, 1, lane);
@@ -1801,7 +1869,8 @@ __global__ __launch_bounds__(kBlock) void k_lp_close(LpArgs A) {
   const bool keep = it >= a.nskip;
   if (wave > 0) {
     lp_afx_apply(c, P, A.AFX + c.cd.row_off, m - 1, dw, lane);
-    data_pass<kRowsStreamL, false, FX>(c, dyn, P, dw, lane, m - 1, m, 0, true, false, true, keep);
+    TileIn<kRowsStreamL> T;
+    data_pass<kRowsStreamL, false, FX>(c, dyn, P, dw, lane, m - 1, m, 0, true, false, true, keep, T, false);
   }
   __syncthreads();
   if (wave == 0) {

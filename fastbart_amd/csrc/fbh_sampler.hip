@@ -109,8 +109,9 @@ __device__ __forceinline__ void sto(T* base, uint32_t i, T v) {
   typedef __attribute__((address_space(1))) T* gptr;
   *(gptr)(reinterpret_cast<char*>(base) + (size_t)(i * (uint32_t)sizeof(T))) = v;
 }
-// a per-thread row index that the compiler may not hoist out of the tile loop (rows of the first tile are the same in
-// every pass; kept live across the sweep they would only be spilled)
+// a per-thread value that the compiler may not hoist out of a loop: the row indices derived from it are recomputed
+// where they are used (those of the first tile are the same in every pass; kept live across the sweep they would be
+// spilled, and a reload from scratch waits for every load issued before it)
 __device__ __forceinline__ uint32_t opaque(uint32_t i) {
   asm volatile("" : "+v"(i));
   return i;
@@ -527,11 +528,16 @@ struct RowPtrs {
   int n, probit;
 };
 
-// Walk one lane's rows in row order, adding src[i] to the task's sum where (tag[i] & mk) == vl. Rows go in blocks of 16
-// whose LDS reads are all issued before the (strictly sequential) adds.
+// Walk one lane's rows in row order, adding src[i] to the task's sum where the task matches the row's tag: bit f of the
+// task's pattern, f being the tag's field (sh, wd) -- the leaf rank (bits 0-4) or the birth-split side (bits 6-7). Two
+// bit-field extracts and a conversion make the 0.0 / 1.0 factor: no compare, so no lane mask and no wait states on
+// the sequential adds. Rows go in blocks of 16 whose LDS reads are all issued before the adds.
+__device__ __forceinline__ uint32_t tag_match(uint32_t word, uint32_t off, uint32_t wd, uint32_t pat) {
+  return __builtin_amdgcn_ubfe(pat, __builtin_amdgcn_ubfe(word, off, wd), 1u);
+}
 template <bool FX>
 __device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, const double* __restrict__ src, int cnt,
-                                           int mk, int vl, double acc) {
+                                           uint32_t sh, uint32_t wd, uint32_t pat, double acc) {
   int i = 0;
   if (FX) {
     long long s = __double_as_longlong(acc);
@@ -544,9 +550,9 @@ __device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, cons
 #pragma unroll
       for (int u = 0; u < 16; u++) x[u] = v[i + u];
 #pragma unroll
-      for (int u = 0; u < 16; u++) s += ((int)((t4[u >> 2] >> (8 * (u & 3))) & mk) == vl) ? x[u] : 0ll;
+      for (int u = 0; u < 16; u++) s += tag_match(t4[u >> 2], 8 * (u & 3) + sh, wd, pat) ? x[u] : 0ll;
     }
-    for (; i < cnt; i++) s += ((int)(tag[i] & mk) == vl) ? v[i] : 0ll;
+    for (; i < cnt; i++) s += tag_match(tag[i], sh, wd, pat) ? v[i] : 0ll;
     return __longlong_as_double(s);
   }
   double s = acc;
@@ -558,15 +564,9 @@ __device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, cons
 #pragma unroll
     for (int u = 0; u < 16; u++) x[u] = src[i + u];
 #pragma unroll
-    for (int u = 0; u < 16; u++) {
-      const double mm = ((int)((t4[u >> 2] >> (8 * (u & 3))) & mk) == vl) ? 1.0 : 0.0;
-      s = fma(mm, x[u], s);
-    }
+    for (int u = 0; u < 16; u++) s = fma((double)tag_match(t4[u >> 2], 8 * (u & 3) + sh, wd, pat), x[u], s);
   }
-  for (; i < cnt; i++) {
-    const double mm = ((int)(tag[i] & mk) == vl) ? 1.0 : 0.0;
-    s = fma(mm, src[i], s);
-  }
+  for (; i < cnt; i++) s = fma((double)tag_match(tag[i], sh, wd, pat), src[i], s);
   return s;
 }
 
@@ -585,9 +585,10 @@ template <int R, bool RES>
 __device__ __forceinline__ void tile_load(TileIn<R>& T, const RowPtrs& P, const uint8_t* __restrict__ la,
                                           const uint8_t* __restrict__ lb, const uint8_t* __restrict__ qa,
                                           const uint8_t* __restrict__ qb, int ii0, int iis, int ka) {
+  const uint32_t base = opaque((uint32_t)(iis * kLanes + ka));  // row (ii0 + iis + 16 r) * 16 + ka
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    const uint32_t i = opaque((uint32_t)min((ii0 + iis + 16 * r) * kLanes + ka, P.n - 1));  // rows past n re-read n-1
+    const uint32_t i = min(base + (uint32_t)((ii0 + 16 * r) * kLanes), (uint32_t)(P.n - 1));  // past n: row n-1
     T.la[r] = ldo(la, i);
     T.lb[r] = ldo(lb, i);
     T.qa[r] = ldo(qa, i);
@@ -644,12 +645,12 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
   for (int tb = 0; tb < n; tb += G::kTile) {
     const int ii0 = tb / kLanes;
     // ---- phase A: the arithmetic on the tile's loads (issued before the previous tile's phase B)
+    const uint32_t abase = opaque((uint32_t)(iis * kLanes + ka));
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int idx = iis + 16 * r;
-      const int i0 = (ii0 + idx) * kLanes + ka;
-      if (i0 >= n) continue;
-      const uint32_t i = opaque((uint32_t)i0);
+      const uint32_t i = abase + (uint32_t)((ii0 + 16 * r) * kLanes);
+      if (i >= (uint32_t)n) continue;
       double af = T.af[r];
       if (apply) {
         const int lf = (int)T.la[r];
@@ -696,11 +697,13 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
     for (int g = 0; g < kGroups; g++) {
       const int task = slot + kTaskSlots * g;
       if (task < ntask) {
-        // task = 2 rank + kind: kind 0 sums w, kind 1 sums w*r; ranks below nleaf are leaves, then the split halves
+        // task = 2 rank + kind: kind 0 sums w, kind 1 sums w*r; ranks below nleaf are leaves (rank field), then the
+        // split halves (side field 1 and 2); the closing pass sums every row (an empty field matches bit 0)
         const int q = task >> 1;
-        const int mk = rss ? 0 : (q < nleaf ? 63 : 0xC0);
-        const int vl = rss ? 0 : (q < nleaf ? q : (q == nleaf ? 0x40 : 0x80));
-        acc[g] = lane_sum<FX>(tgb, (task & 1) ? v1b : v0b, cnt, mk, vl, acc[g]);
+        const uint32_t sh = (!rss && q >= nleaf) ? 6u : 0u;
+        const uint32_t wd = rss ? 0u : (q < nleaf ? 5u : 2u);
+        const uint32_t pat = rss ? 1u : (q < nleaf ? (1u << q) : (q == nleaf ? 2u : 4u));
+        acc[g] = lane_sum<FX>(tgb, (task & 1) ? v1b : v0b, cnt, sh, wd, pat, acc[g]);
       }
     }
     wave_sync();
@@ -1005,13 +1008,19 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         const uint64_t tw1 = tsrc[lane < kTreeWords - 64 ? 64 + lane : lane];
         // this update's generator outputs: the MH uniform (:482), then two per slot for the leaf normals (:521-523)
         const uint32_t x0 = rng_next(g);
-        uint32_t my1 = 0, my2 = 0;
+        // the 2 cap steps keep only each output's raw s1 word (lane s gets slot s's pair); the output function is then
+        // applied on all lanes at once
+        const uint32_t k11 = ufl(c.rot[0]), k21 = ufl(c.rot[1]);
+        uint32_t r1 = 0, r2 = 0;
         for (int s = 0; s < cap; s++) {
-          const uint32_t a1 = rng_next(g);
-          const uint32_t a2 = rng_next(g);
-          my1 = lane == s ? a1 : my1;
-          my2 = lane == s ? a2 : my2;
+          const uint32_t a1 = g.s1;
+          rng_step(g, k11, k21);
+          const uint32_t a2 = g.s1;
+          rng_step(g, k11, k21);
+          r1 = lane == s ? a1 : r1;
+          r2 = lane == s ? a2 : r2;
         }
+        const uint32_t my1 = starstar(r1), my2 = starstar(r2);
         if (lane == 0) {
           c.rng_mark[0] = g.s0;
           c.rng_mark[1] = g.s1;

@@ -1,6 +1,6 @@
 """Benchmark: tree-updates/sec/GPU of the batched plain sampler, plus min-ESS/sec.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch512|fwbart_n4096|fpbart_n24576|maxshape64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch512|fwbart_n4096|fpbart_n24576|maxshape64|learned512|predict600|predict_probit100]
 
 Default workload (the headline): batched fwbart -- 512 independent chains in one persistent launch, chain c with
 n_c ~ U{3091..4573} rows, p_c ~ U{15..19} predictors, x ~ U(0,1) with 255 cutpoints, y = 10 sin(pi x1 x2) +
@@ -18,6 +18,9 @@ Other workloads (extra lines, not the headline):
                  BART's proposal in the lock-step pipeline collecting accepted births, training of the network on the
                  device, then sweeps with the mixture 0.2 uniform / 0.8 network up to the end (the kept draws); reports
                  min-ESS/s of both (training time apart) and the posterior-mean agreement
+  predict600     predict_draw: 6 fits of the batched workload (200 trees, 100 kept draws each), 600 draws on 5,237
+                 new rows; value in row-trees/s (a step is one pass over all 600 draws)
+  predict_probit100  predict_draw for the 100 draws of the fpbart fit on its 24,576 rows (50 trees)
 
 One JSON line on stdout. Statistics mode REF16 (double sums in 16 row lanes, bit-identical to the reference CPU
 sampler) unless the workload says FX64. The CPU baseline is the reference's own core (oracle/_ref, built from the
@@ -47,6 +50,8 @@ WORKLOADS = {
     "fpbart_n24576": (1, 50, 1000, 100, 1, 0, 0, 20, None),
     "maxshape64": (64, 200, 500, 100, 0, 1, 64, 5, 40),
     "learned512": (512, 200, 1000, 100, 0, 0, 0, 20, None),
+    "predict600": (6, 200, 1000, 100, 0, 0, 0, 3, 20),
+    "predict_probit100": (1, 50, 1000, 100, 1, 0, 0, 3, 20),
 }
 
 
@@ -258,6 +263,149 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
     return 0
 
 
+# ------------------------------------------------------------------ prediction (compile_trees / predict_draw)
+def _predict_cpu(jobs, calls):
+    """The oracle's restatement of predict_draw_cpp (src/predict.cpp:49-73) on one core: `calls` single-draw calls
+    spread over the fits; returns (row-trees per second, CPU seconds)."""
+    from oracle import pyoracle
+    orc = pyoracle.Oracle()
+    prepared = []
+    for text, cuts, x, mu in jobs:
+        oc = orc.compile_trees(text, cuts)
+        prepared.append((oc, [np.ascontiguousarray(x[:, v]) for v in range(x.shape[1])], mu))
+    work, t0 = 0, time.perf_counter()
+    for k in range(calls):
+        oc, cols, mu = prepared[k % len(prepared)]
+        orc.predict_draw(oc, cols, 1 + k % oc["ndpost"], mu)
+        work += len(cols[0]) * oc["ntree"]
+    dt = time.perf_counter() - t0
+    return work / dt, dt
+
+
+def predict_bench(args, wl):
+    """Fits of the workload's chains (full nskip + ndpost on the device), their treedraws text compiled once, then
+    predict_draw for every kept draw on new rows resident on the device. value = rows x trees per second of the
+    batched path (one launch per fit over its draws, device-side output); the line also carries single-draw calls
+    (one launch per draw) and the host-output rate (PCIe included)."""
+    from fastbart_amd import sampler
+    from fastbart_amd.predict import CompiledTrees, DeviceBuffer, DeviceRows, device_sync
+    from fastbart_amd import workloads as Wk
+    if wl == "predict600":
+        fits = [Wk.batch512_chain(c) for c in range(6)]
+        m, nskip, ndpost, probit, n_new = 200, 1000, 100, 0, 5237
+        desc = ("predict_draw on 6 fits of the batched fwbart workload (chains 0-5, ntree 200, 100 kept draws each): "
+                "600 draws x 5237 new rows, x ~ U(0,1)")
+    else:
+        fits = [Wk.workload_fpbart_n24576()]
+        m, nskip, ndpost, probit, n_new = 50, 1000, 100, 1, 24576
+        desc = "predict_draw for all 100 draws of the fpbart probit fit (ntree 50) on its 24576 training rows"
+    t0 = time.perf_counter()
+    with sampler.BatchSampler(fits, m, nskip, ndpost, [1 + c for c in range(len(fits))], probit=probit,
+                              keep_trees=1) as b:
+        b.run(nskip + ndpost)
+        outs = [b.fetch(c) for c in range(len(fits))]
+    log(f"{wl}: fitted {len(fits)} chain(s) in {time.perf_counter() - t0:.1f} s")
+    cts, rows, hostx, jobs = [], [], [], []
+    for c, pb in enumerate(fits):
+        p = pb["q"].shape[0]
+        text, _ = sampler.tree_text(outs[c]["trees"], p)
+        mu = float(pb["offset"]) if probit else float(pb["fmean"])
+        ct = CompiledTrees(text, pb["cutpoints"], mu=mu)
+        if wl == "predict600":
+            x = np.random.Generator(np.random.PCG64(4000 + c)).uniform(0.0, 1.0, size=(n_new, p))
+        else:
+            x = pb["x"]
+        cts.append(ct)
+        rows.append(DeviceRows(x))
+        hostx.append(x)
+        jobs.append((text, pb["cutpoints"], x, mu))
+    nodes = [ct.nnodes / ct.ndpost for ct in cts]
+    rank = all(ct.rank_form for ct in cts)
+    d_draws = DeviceBuffer.from_array(np.arange(1, ndpost + 1, dtype=np.int32))
+    d_outs = [DeviceBuffer(8 * ndpost * r.n) for r in rows]
+    draws_total = ndpost * len(fits)
+    rowtrees = sum(ndpost * r.n * m for r in rows)
+
+    def batched():
+        for ct, r, o in zip(cts, rows, d_outs):
+            r.predict_device(ct, d_draws, ndpost, o)
+        device_sync()
+
+    singles = [DeviceBuffer.from_array(np.array([k], np.int32)) for k in range(1, ndpost + 1)]
+
+    def single_calls():
+        for ct, r, o in zip(cts, rows, d_outs):
+            for k in range(ndpost):
+                r.predict_device(ct, singles[k], 1, o)
+        device_sync()
+
+    for _ in range(max(1, args.warmup if args.warmup is not None else 3)):
+        batched()
+        single_calls()
+    reps = max(1, args.steps if args.steps is not None else 20)
+    k0 = [ct.timing() for ct in cts]
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        batched()
+    T = time.perf_counter() - t1
+    k1 = [ct.timing() for ct in cts]
+    # average kernel duration of the timed batched launches (HIP events on the launch stream)
+    ms_sum = sum(a1 * n1 - a0 * n0 for (a0, n0), (a1, n1) in zip(k0, k1))
+    nl = sum(n1 - n0 for (a0, n0), (a1, n1) in zip(k0, k1))
+    ms_launch = ms_sum / nl if nl else 0.0
+    t2 = time.perf_counter()
+    for _ in range(reps):
+        single_calls()
+    T1 = time.perf_counter() - t2
+    t3 = time.perf_counter()
+    for _ in range(max(1, reps // 4)):
+        for ct, r in zip(cts, rows):
+            r.predict(ct, np.arange(1, ndpost + 1))
+    T2 = (time.perf_counter() - t3) / max(1, reps // 4)
+    value = rowtrees * reps / T
+    # algorithmic bytes per launch (one fit, ndpost draws): x once and f once per (draw, row), 24 B per node of the
+    # draw's trees
+    p_avg = float(np.mean([r.p for r in rows]))
+    n_avg = float(np.mean([r.n for r in rows]))
+    bytes_launch = ndpost * n_avg * (8 * p_avg + 8) + ndpost * 24 * float(np.mean(nodes))
+    achieved = bytes_launch / (ms_launch / 1e3) / 1e9 if ms_launch else 0.0
+    log(f"{wl}: batched {value / 1e9:.2f} G row-trees/s ({T / reps * 1e3:.3f} ms per pass of {draws_total} draws, "
+        f"kernel {ms_launch:.3f} ms per launch of {ndpost} draws); single-draw calls "
+        f"{rowtrees * reps / T1 / 1e9:.2f} G/s; host output {rowtrees / T2 / 1e9:.2f} G/s; rank form {rank}")
+    cpu = None
+    if not args.no_cpu:
+        cpu_v, cpu_s = _predict_cpu(jobs, 12 if wl == "predict600" else 6)
+        cpu = dict(value=cpu_v, unit="row-trees/s", cores=1, kind="port",
+                   sample=f"{12 if wl == 'predict600' else 6} single-draw predict_draw calls of the oracle's "
+                          f"restatement of src/predict.cpp (same fits and rows), one core, {cpu_s:.1f} s")
+    for x in d_outs + singles + [d_draws]:
+        x.close()
+    line = {
+        "metric": "predict_draw row-trees/sec (compile_trees / predict_draw)", "value": value, "unit": "row-trees/s",
+        "n_gpus": 1, "steps": reps, "warmup": args.warmup if args.warmup is not None else 3,
+        "ms_per_step": T / reps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": desc, "fits": len(fits), "draws": draws_total, "rows": [r.n for r in rows],
+                   "ntree": m, "nodes_per_draw": float(np.mean(nodes)), "kernel": "k_predict_rank" if rank else
+                   "k_predict", "parallelism": "rows x draws"},
+        "single_draw_calls": {"value": rowtrees * reps / T1, "ms_per_call": T1 / (reps * draws_total) * 1e3},
+        "host_output": {"value": rowtrees / T2, "note": "fbh_predict_rows: device result copied to host memory"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_predict_rank" if rank else "k_predict", "kernel_avg_ms": ms_launch, "launches": nl,
+                     "algorithmic_bytes_per_launch": bytes_launch},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(line), flush=True)
+    for r in rows:
+        r.close()
+    for ct in cts:
+        ct.close()
+    return 0
+
+
 # ------------------------------------------------------------------ main
 def main() -> int:
     ap = argparse.ArgumentParser()
@@ -285,6 +433,8 @@ def main() -> int:
               "(chains shard across GPUs with no exchange; see the design notes). Use --gpus 1.", file=sys.stderr)
         return 2
     wl = args.workload
+    if wl.startswith("predict"):
+        return predict_bench(args, wl)
     nch, M, NSKIP, NDPOST, probit, stats_mode, node_cap, w_def, k_def = WORKLOADS[wl]
     chains = min(nch, args.chains) if args.chains else nch
     if wl == "learned512":

@@ -1,11 +1,13 @@
 // Output formats and prediction: treedraws text and varcount (src/glue.cpp:29-51), per-draw training fits
 // (src/glue.cpp:52-56) on the device, and compile_trees / predict_draw (src/predict.cpp) with a tree-outer device
 // kernel over flat child-pair arrays.
+#include <atomic>
 #include <charconv>
 #include <cmath>
 #include <cstring>
 #include <string>
 #include <thread>
+#include <mutex>
 #include <unordered_map>
 #include <vector>
 
@@ -37,9 +39,107 @@ __global__ __launch_bounds__(256) void k_yhat(const uint64_t* __restrict__ trees
   out[(size_t)dr * n + i] = f;
 }
 
-// ------------------------------------------------------------------ predictor kernel
-// Tree-outer over a block of rows: the workgroup stages the flat node arrays of one draw in LDS (when they fit) and
-// every thread adds the draw's trees in order for its row; x >= cut goes to the right child (src/predict.cpp:125-133).
+// ------------------------------------------------------------------ predictor kernels
+// Rank form (the fast path). With the cutpoints xi[v][0..K-1] of a variable nondecreasing and free of NaN, the
+// predicate x >= xi[v][c] of src/predict.cpp:68 is monotone in c, so it holds exactly when c < rank(x), where
+// rank(x) = #{k : x >= xi[v][k]} (0 for NaN, which compares false with everything). A row is then described by p
+// 16-bit ranks instead of p doubles, and a node by 8 bytes: the index of its left child (its own index for a leaf),
+// the variable and the cut index (0xFFFF for a leaf, which no rank exceeds, so a leaf maps to itself). A tree is
+// walked a fixed number of steps, its depth, with no branch: k = next[k] + (rank[var[k]] > c[k]).
+constexpr int kRankThreads = 256;
+constexpr int kRankRowsPerThread = 2;
+constexpr int kRankRows = kRankThreads * kRankRowsPerThread;  // rows per workgroup
+constexpr uint32_t kLeafCut = 0xFFFFu;
+
+// ranks [p][n] u16 of the rows x [p][n] against the cutpoints xi (flat, variable v < pcut at xi[off[v] ..
+// off[v+1]-1]; a variable without cutpoints has rank 0): a binary search with the reference's own comparison x >= xi[k]
+__global__ __launch_bounds__(256) void k_rank(const double* __restrict__ x, const double* __restrict__ xi,
+                                              const int* __restrict__ off, int n, int p, int pcut,
+                                              uint16_t* __restrict__ rank) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (i >= n || v >= p) return;
+  const double xv = x[(size_t)v * n + i];
+  const double* c = v < pcut ? xi + off[v] : xi;
+  int lo = 0, hi = v < pcut ? off[v + 1] - off[v] : 0;  // invariant: x >= c[k] for k < lo, not for k >= hi
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (xv >= c[mid]) lo = mid + 1;
+    else hi = mid;
+  }
+  rank[(size_t)v * n + i] = (uint16_t)lo;
+}
+
+// One workgroup = one draw x kRankRows rows. LDS holds the draw's node words and leaf values, its per-tree
+// (root, depth) words and the rows' ranks [p][kRankRows]; every thread adds the trees in order for its rows, from
+// 0.0, then mu (src/predict.cpp:64-72).
+__global__ __launch_bounds__(kRankThreads) void k_predict_rank(const uint64_t* __restrict__ node,
+                                                               const double* __restrict__ theta,
+                                                               const uint32_t* __restrict__ tinfo,
+                                                               const int* __restrict__ draw_lo,
+                                                               const int* __restrict__ draws,
+                                                               const uint16_t* __restrict__ rank, int n, int p,
+                                                               int ntree, int ndpost, double mu,
+                                                               double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char pred_lds[];
+  const int k = blockIdx.y;
+  const int dr = draws[k] - 1;
+  const int row0 = blockIdx.x * kRankRows;
+  if (dr < 0 || dr >= ndpost) {  // a draw outside 1..ndpost (device draw lists are not checked on the host): NaN
+    for (int r = threadIdx.x; r < kRankRows; r += kRankThreads)
+      if (row0 + r < n) out[(size_t)k * n + row0 + r] = __builtin_nan("");
+    return;
+  }
+  const int lo = draw_lo[dr], cnt = draw_lo[dr + 1] - lo;
+  uint64_t* s_node = reinterpret_cast<uint64_t*>(pred_lds);
+  double* s_theta = reinterpret_cast<double*>(s_node + cnt);
+  uint32_t* s_tinfo = reinterpret_cast<uint32_t*>(s_theta + cnt);
+  uint16_t* s_rank = reinterpret_cast<uint16_t*>(s_tinfo + ((ntree + 3) & ~3));
+  for (int t = threadIdx.x; t < cnt; t += kRankThreads) {
+    s_node[t] = node[lo + t];
+    s_theta[t] = theta[lo + t];
+  }
+  for (int t = threadIdx.x; t < ntree; t += kRankThreads) s_tinfo[t] = tinfo[(size_t)dr * ntree + t];
+  for (int v = 0; v < p; v++)
+    for (int r = threadIdx.x; r < kRankRows; r += kRankThreads) {
+      const int i = row0 + r;
+      s_rank[v * kRankRows + r] = i < n ? rank[(size_t)v * n + i] : (uint16_t)0;
+    }
+  __syncthreads();
+  double f[kRankRowsPerThread];
+  const uint16_t* rk[kRankRowsPerThread];
+#pragma unroll
+  for (int j = 0; j < kRankRowsPerThread; j++) {
+    f[j] = 0.0;
+    rk[j] = s_rank + threadIdx.x + j * kRankThreads;
+  }
+  for (int t = 0; t < ntree; t++) {
+    const uint32_t ti = s_tinfo[t];
+    const int depth = (int)(ti >> 24);
+    int kk[kRankRowsPerThread];
+#pragma unroll
+    for (int j = 0; j < kRankRowsPerThread; j++) kk[j] = (int)(ti & 0xFFFFFFu);
+    for (int d = 0; d < depth; d++) {
+#pragma unroll
+      for (int j = 0; j < kRankRowsPerThread; j++) {
+        const uint64_t w = s_node[kk[j]];
+        const uint32_t var = (uint32_t)(w >> 32) & 0xFFFFu, c = (uint32_t)(w >> 48);
+        kk[j] = (int)(uint32_t)w + (int)((uint32_t)rk[j][var * kRankRows] > c);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kRankRowsPerThread; j++) f[j] += s_theta[kk[j]];
+  }
+#pragma unroll
+  for (int j = 0; j < kRankRowsPerThread; j++) {
+    const int i = row0 + threadIdx.x + j * kRankThreads;
+    if (i < n) out[(size_t)k * n + i] = f[j] + mu;
+  }
+}
+
+// General form: any cutpoint table, any draw size. Tree-outer over a block of rows: the workgroup stages the flat node
+// arrays of one draw in LDS (when they fit) and every thread adds the draw's trees in order for its row; x >= cut goes
+// to the right child (src/predict.cpp:64-72).
 constexpr int kPredThreads = 256;
 constexpr int kPredLdsNodes = 2048;  // 2048 nodes x (int var, int child, double cut, double theta) = 48 KiB
 
@@ -48,13 +148,19 @@ __global__ __launch_bounds__(kPredThreads) void k_predict(const int* __restrict_
                                                           const double* __restrict__ theta,
                                                           const int* __restrict__ root, const int* __restrict__ draws,
                                                           const int* __restrict__ draw_lo, const double* __restrict__ x,
-                                                          int n, int ntree, double mu, double* __restrict__ out) {
+                                                          int n, int ntree, int ndpost, double mu,
+                                                          double* __restrict__ out) {
   __shared__ int s_var[kPredLdsNodes];
   __shared__ int s_child[kPredLdsNodes];
   __shared__ double s_cut[kPredLdsNodes];
   __shared__ double s_theta[kPredLdsNodes];
   const int k = blockIdx.y;
   const int dr = draws[k] - 1;
+  if (dr < 0 || dr >= ndpost) {  // a draw outside 1..ndpost (device draw lists are not checked on the host): NaN
+    const int i = blockIdx.x * kPredThreads + threadIdx.x;
+    if (i < n) out[(size_t)k * n + i] = __builtin_nan("");
+    return;
+  }
   const int* r0 = root + (size_t)dr * ntree;
   const int lo = draw_lo[dr], hi = draw_lo[dr + 1];
   const int cnt = hi - lo;
@@ -244,25 +350,62 @@ extern "C" size_t fbh_tree_text(const uint64_t* trees, int ndpost, int m, int p,
 struct fbh_trees {
   int device = 0;
   int ndpost = 0, ntree = 0, p = 0;
+  // flat breadth-first arrays of src/predict.cpp:12-45 (absolute node indexes)
   std::vector<int> var, child, root, draw_lo;
   std::vector<double> cut, theta;
+  // rank form: node words (next | var << 32 | c << 48, indexes relative to the draw), per-tree (root | depth << 24),
+  // the cutpoint table the ranks are taken against; rank_ok when the table and the trees allow the rank form
+  std::vector<uint64_t> node_w;
+  std::vector<uint32_t> tinfo;
+  std::vector<double> xi;
+  std::vector<int> xoff;
+  int pcut = 0, max_draw_nodes = 0;
+  int max_var = -1;  // largest split variable (-1: no internal node)
+  bool rank_ok = false;
   int *d_var = nullptr, *d_child = nullptr, *d_root = nullptr, *d_lo = nullptr;
   double *d_cut = nullptr, *d_theta = nullptr;
-  std::vector<hipEvent_t> ev;
+  uint64_t* d_node = nullptr;
+  uint32_t* d_tinfo = nullptr;
+  double* d_xi = nullptr;
+  int* d_xoff = nullptr;
+  // per-handle workspace; calls on one handle are serialised by mu and run in order on the caller's stream
+  std::mutex mu;
+  int* d_draws = nullptr;
+  size_t draws_cap = 0;
+  double* d_out = nullptr;
+  size_t out_cap = 0;
+  uint16_t* d_rank = nullptr;
+  size_t rank_cap = 0;
+  uint64_t rank_rows_uid = 0;  // the rows the ranks in d_rank belong to (0 = none)
+  hipStream_t rank_stream = nullptr;
+  // launch timing: a ring of event pairs, folded into the totals when a pair is reused
+  static constexpr int kEv = 128;
+  hipEvent_t ev0[kEv] = {}, ev1[kEv] = {};
+  bool ev_live[kEv] = {};
+  int ev_next = 0;
+  double ev_ms = 0.0;
+  long ev_count = 0;
 };
 
 struct fbh_rows {
   int device = 0, n = 0, p = 0;
+  uint64_t uid = 0;
   double* d_x = nullptr;
 };
+
+static std::atomic<uint64_t> g_rows_uid{0};
 
 static void free_trees(fbh_trees* t) {
   if (!t) return;
   (void)hipSetDevice(t->device);
-  void* ptrs[] = {t->d_var, t->d_child, t->d_root, t->d_lo, t->d_cut, t->d_theta};
+  void* ptrs[] = {t->d_var, t->d_child, t->d_root, t->d_lo,  t->d_cut,   t->d_theta, t->d_node,
+                  t->d_tinfo, t->d_xi,  t->d_xoff, t->d_draws, t->d_out, t->d_rank};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  for (auto e : t->ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < fbh_trees::kEv; k++) {
+    if (t->ev0[k]) (void)hipEventDestroy(t->ev0[k]);
+    if (t->ev1[k]) (void)hipEventDestroy(t->ev1[k]);
+  }
   delete t;
 }
 
@@ -299,6 +442,48 @@ struct Parser {
 };
 }  // namespace
 
+// The rank form of compiled trees (see k_predict_rank): possible when every variable's cutpoints are nondecreasing
+// and free of NaN with at most 65,535 of them, every tree is at most 255 levels deep and every draw holds fewer than
+// 2^24 nodes. Otherwise predictions take the general kernel.
+static void build_rank_form(fbh_trees* t, const double* cutpoints, const int* cut_off, int pcut,
+                            const std::vector<int>& cidx, const std::vector<int>& tdepth) {
+  t->rank_ok = false;
+  t->pcut = pcut;
+  t->xoff.assign(cut_off, cut_off + pcut + 1);
+  t->xi.assign(cutpoints, cutpoints + (pcut > 0 ? cut_off[pcut] : 0));
+  if (pcut > 65535) return;
+  for (int v = 0; v < pcut; v++) {
+    const int k0 = cut_off[v], k1 = cut_off[v + 1];
+    if (k1 - k0 > 65535) return;
+    for (int k = k0; k < k1; k++) {
+      if (!(cutpoints[k] == cutpoints[k])) return;                 // NaN
+      if (k > k0 && !(cutpoints[k - 1] <= cutpoints[k])) return;   // decreasing
+    }
+  }
+  for (int d : tdepth)
+    if (d > 255) return;
+  t->max_draw_nodes = 0;
+  for (int dr = 0; dr < t->ndpost; dr++) {
+    const int cnt = t->draw_lo[dr + 1] - t->draw_lo[dr];
+    if (cnt >= (1 << 24)) return;
+    t->max_draw_nodes = std::max(t->max_draw_nodes, cnt);
+  }
+  const size_t nn = t->var.size();
+  t->node_w.resize(nn);
+  for (int dr = 0; dr < t->ndpost; dr++)
+    for (int k = t->draw_lo[dr]; k < t->draw_lo[dr + 1]; k++) {
+      const bool leaf = t->child[k] < 0;
+      const uint64_t next = (uint64_t)(uint32_t)((leaf ? k : t->child[k]) - t->draw_lo[dr]);
+      const uint64_t var = leaf ? 0u : (uint64_t)t->var[k], c = leaf ? kLeafCut : (uint64_t)cidx[k];
+      t->node_w[k] = next | (var << 32) | (c << 48);
+    }
+  const long ntr = (long)t->ndpost * t->ntree;
+  t->tinfo.resize(ntr);
+  for (long tr = 0; tr < ntr; tr++)
+    t->tinfo[tr] = (uint32_t)(t->root[tr] - t->draw_lo[tr / t->ntree]) | ((uint32_t)tdepth[tr] << 24);
+  t->rank_ok = true;
+}
+
 extern "C" int fbh_compile_trees(const char* text, size_t len, const double* cutpoints, const int* cut_off, int pcut,
                                  int device, fbh_trees** out) {
   if (!text || !out || (pcut > 0 && (!cutpoints || !cut_off))) {
@@ -327,7 +512,8 @@ extern "C" int fbh_compile_trees(const char* text, size_t len, const double* cut
   };
   std::vector<Nd> nd;
   std::unordered_map<unsigned long long, int> pos;
-  std::vector<int> qk, slot;
+  std::vector<int> qk, slot, cidx, depth;
+  std::vector<int> tdepth(ntr, 0);
   for (long tr = 0; tr < ntr; tr++) {
     if (tr % t->ntree == 0) t->draw_lo[tr / t->ntree] = (int)t->var.size();
     const long long nn = ps.integer();
@@ -358,6 +544,8 @@ extern "C" int fbh_compile_trees(const char* text, size_t len, const double* cut
     t->child.push_back(-1);
     t->cut.push_back(0);
     t->theta.push_back(0);
+    cidx.push_back(0);
+    depth.push_back(0);
     for (size_t h = 0; h < qk.size(); h++) {
       const int k = qk[h], me = slot[h];
       auto L = pos.find(2 * nd[k].id);
@@ -375,12 +563,17 @@ extern "C" int fbh_compile_trees(const char* text, size_t len, const double* cut
         t->var[me] = v;
         t->cut[me] = cutpoints[cut_off[v] + c];
         t->child[me] = ch;
+        cidx[me] = c;
+        t->max_var = std::max(t->max_var, v);
         for (int z = 0; z < 2; z++) {
           t->var.push_back(0);
           t->child.push_back(-1);
           t->cut.push_back(0);
           t->theta.push_back(0);
+          cidx.push_back(0);
+          depth.push_back(depth[me] + 1);
         }
+        tdepth[tr] = std::max(tdepth[tr], depth[me] + 1);
         qk.push_back(L->second);
         slot.push_back(ch);
         qk.push_back(R->second);
@@ -389,11 +582,12 @@ extern "C" int fbh_compile_trees(const char* text, size_t len, const double* cut
     }
   }
   t->draw_lo[t->ndpost] = (int)t->var.size();
+  build_rank_form(t, cutpoints, cut_off, pcut, cidx, tdepth);
   *out = t;
   return FBH_OK;
 }
 
-// device copies of the flat arrays, made on the first prediction (parsing needs no device)
+// device copies of the flat arrays (and of the rank form), made on the first prediction (parsing needs no device)
 static int upload_trees(fbh_trees* t) {
   if (t->d_var) return FBH_OK;
   int rc = ensure_device(t->device);
@@ -408,6 +602,12 @@ static int upload_trees(fbh_trees* t) {
   up((void**)&t->d_theta, t->theta.data(), sizeof(double) * t->theta.size());
   up((void**)&t->d_root, t->root.data(), sizeof(int) * t->root.size());
   up((void**)&t->d_lo, t->draw_lo.data(), sizeof(int) * t->draw_lo.size());
+  if (t->rank_ok) {
+    up((void**)&t->d_node, t->node_w.data(), sizeof(uint64_t) * t->node_w.size());
+    up((void**)&t->d_tinfo, t->tinfo.data(), sizeof(uint32_t) * t->tinfo.size());
+    up((void**)&t->d_xi, t->xi.data(), sizeof(double) * t->xi.size());
+    up((void**)&t->d_xoff, t->xoff.data(), sizeof(int) * t->xoff.size());
+  }
   up((void**)&t->d_var, t->var.data(), sizeof(int) * t->var.size());
   if (e != hipSuccess) {
     set_error(FBH_EHIP, std::string("fbh_predict: upload: ") + hipGetErrorString(e));
@@ -445,6 +645,7 @@ extern "C" int fbh_rows_upload(const double* const* cols, int n, int p, int devi
   int rc = ensure_device(device);
   if (rc) return rc;
   fbh_rows* r = new fbh_rows();
+  r->uid = ++g_rows_uid;
   r->device = device;
   r->n = n;
   r->p = p;
@@ -471,10 +672,9 @@ extern "C" void fbh_rows_free(fbh_rows* r) {
   delete r;
 }
 
-extern "C" int fbh_predict_rows(const fbh_trees* tc, const fbh_rows* r, const int* draws, int ndraws, double mu,
-                                double* out_host) {
-  fbh_trees* t = const_cast<fbh_trees*>(tc);
-  if (!t || !r || !draws || ndraws < 0 || !out_host) {
+// argument checks shared by the predict entry points (draws on the host)
+static int check_predict(const fbh_trees* t, const fbh_rows* r, int ndraws) {
+  if (!t || !r || ndraws < 0) {
     set_error(FBH_EINVAL, "fbh_predict: bad argument");
     return FBH_EINVAL;
   }
@@ -483,57 +683,168 @@ extern "C" int fbh_predict_rows(const fbh_trees* tc, const fbh_rows* r, const in
                               std::to_string(t->p));
     return FBH_EINVAL;
   }
+  if (r->device != t->device) {
+    set_error(FBH_EINVAL, "fbh_predict: rows and trees live on different devices");
+    return FBH_EINVAL;
+  }
+  if (t->max_var >= r->p) {
+    set_error(FBH_EINVAL, "fbh_predict: a split variable is outside the newdata columns");
+    return FBH_EINVAL;
+  }
+  return FBH_OK;
+}
+
+static bool grow(void** buf, size_t* cap, size_t bytes) {
+  if (*cap >= bytes) return true;
+  if (*buf) (void)hipFree(*buf);
+  *buf = nullptr;
+  *cap = 0;
+  if (hipMalloc(buf, bytes) != hipSuccess) return false;
+  *cap = bytes;
+  return true;
+}
+
+// LDS bytes of one k_predict_rank workgroup for draws of up to `nodes` nodes
+static size_t rank_lds_bytes(int nodes, int ntree, int p) {
+  return (size_t)nodes * 16 + (size_t)((ntree + 3) & ~3) * 4 + (size_t)p * kRankRows * 2;
+}
+// Dynamic LDS the rank kernel may use on a device: the device's per-workgroup limit, raised for the kernel once per
+// device (0 if that fails, which sends every prediction to the general kernel).
+static size_t rank_lds_limit(int device) {
+  static std::mutex m;
+  static size_t lim[64];
+  static bool done[64];
+  if (device < 0 || device >= 64) return 0;
+  std::lock_guard<std::mutex> lk(m);
+  if (!done[device]) {
+    done[device] = true;
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(k_predict_rank), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            v) == hipSuccess)
+      lim[device] = (size_t)v;
+  }
+  return lim[device];
+}
+
+// Launch the prediction of ndraws draws (device array d_draws, 1-based, checked by the caller) for the rows r into
+// d_out [ndraws][n] on stream s. Caller holds t->mu.
+static int predict_launch(fbh_trees* t, const fbh_rows* r, const int* d_draws, int ndraws, double mu, double* d_out,
+                          hipStream_t s) {
+  if (ndraws == 0 || r->n == 0) return FBH_OK;
+  int rc = upload_trees(t);
+  if (rc) return rc;
+  FBH_HIP_CHECK(hipSetDevice(t->device));
+  // fold the oldest event pair of the ring into the totals before reusing it
+  const int slot = t->ev_next;
+  t->ev_next = (t->ev_next + 1) % fbh_trees::kEv;
+  if (!t->ev0[slot]) {
+    FBH_HIP_CHECK(hipEventCreate(&t->ev0[slot]));
+    FBH_HIP_CHECK(hipEventCreate(&t->ev1[slot]));
+  }
+  if (t->ev_live[slot]) {
+    float ms = 0;
+    FBH_HIP_CHECK(hipEventSynchronize(t->ev1[slot]));
+    if (hipEventElapsedTime(&ms, t->ev0[slot], t->ev1[slot]) == hipSuccess) {
+      t->ev_ms += ms;
+      t->ev_count++;
+    }
+    t->ev_live[slot] = false;
+  }
+  const size_t lds = rank_lds_bytes(t->max_draw_nodes, t->ntree, r->p);
+  const size_t lds_max = rank_lds_limit(t->device);
+  const bool rank_form = t->rank_ok && lds <= lds_max;
+  if (rank_form && t->rank_rows_uid != r->uid) {
+    // the rows' ranks against this table; rows are immutable, so the ranks stay valid while the same rows are used
+    if (!grow((void**)&t->d_rank, &t->rank_cap, sizeof(uint16_t) * (size_t)r->n * (r->p ? r->p : 1))) {
+      set_error(FBH_ENOMEM, "fbh_predict: hipMalloc of the rank buffer failed");
+      return FBH_ENOMEM;
+    }
+    if (r->p > 0) {
+      hipLaunchKernelGGL(k_rank, dim3((r->n + 255) / 256, r->p), dim3(256), 0, s, r->d_x, t->d_xi, t->d_xoff, r->n,
+                         r->p, t->pcut, t->d_rank);
+      FBH_HIP_CHECK(hipGetLastError());
+    }
+    t->rank_rows_uid = r->uid;
+  }
+  FBH_HIP_CHECK(hipEventRecord(t->ev0[slot], s));
+  if (rank_form) {
+    dim3 grid((r->n + kRankRows - 1) / kRankRows, ndraws);
+    hipLaunchKernelGGL(k_predict_rank, grid, dim3(kRankThreads), lds, s, t->d_node, t->d_theta, t->d_tinfo, t->d_lo,
+                       d_draws, t->d_rank, r->n, r->p, t->ntree, t->ndpost, mu, d_out);
+  } else {
+    dim3 grid((r->n + kPredThreads - 1) / kPredThreads, ndraws);
+    hipLaunchKernelGGL(k_predict, grid, dim3(kPredThreads), 0, s, t->d_var, t->d_child, t->d_cut, t->d_theta,
+                       t->d_root, d_draws, t->d_lo, r->d_x, r->n, t->ntree, t->ndpost, mu, d_out);
+  }
+  FBH_HIP_CHECK(hipGetLastError());
+  FBH_HIP_CHECK(hipEventRecord(t->ev1[slot], s));
+  t->ev_live[slot] = true;
+  return FBH_OK;
+}
+
+extern "C" int fbh_predict_rows(const fbh_trees* tc, const fbh_rows* r, const int* draws, int ndraws, double mu,
+                                double* out_host) {
+  fbh_trees* t = const_cast<fbh_trees*>(tc);
+  int rc = check_predict(t, r, ndraws);
+  if (rc) return rc;
+  if (!draws || !out_host) {
+    set_error(FBH_EINVAL, "fbh_predict: null draws or output");
+    return FBH_EINVAL;
+  }
   for (int k = 0; k < ndraws; k++)
     if (draws[k] < 1 || draws[k] > t->ndpost) {
       set_error(FBH_EINVAL, "fbh_predict: draw " + std::to_string(draws[k]) + " out of range 1.." +
                                 std::to_string(t->ndpost));
       return FBH_EINVAL;
     }
-  for (size_t k = 0; k < t->var.size(); k++)
-    if (t->child[k] >= 0 && (t->var[k] < 0 || t->var[k] >= r->p)) {
-      set_error(FBH_EINVAL, "fbh_predict: a split variable is outside the newdata columns");
-      return FBH_EINVAL;
-    }
   if (ndraws == 0 || r->n == 0) return FBH_OK;
-  int rc = upload_trees(t);
+  std::lock_guard<std::mutex> lk(t->mu);
+  rc = ensure_device(t->device);
   if (rc) return rc;
   FBH_HIP_CHECK(hipSetDevice(t->device));
-  int* d_draws = nullptr;
-  double* d_out = nullptr;
-  FBH_HIP_CHECK(hipMalloc(&d_draws, sizeof(int) * ndraws));
-  FBH_HIP_CHECK(hipMalloc(&d_out, sizeof(double) * (size_t)ndraws * r->n));
-  FBH_HIP_CHECK(hipMemcpy(d_draws, draws, sizeof(int) * ndraws, hipMemcpyHostToDevice));
-  hipEvent_t e0, e1;
-  FBH_HIP_CHECK(hipEventCreate(&e0));
-  FBH_HIP_CHECK(hipEventCreate(&e1));
-  t->ev.push_back(e0);
-  t->ev.push_back(e1);
-  dim3 grid((r->n + kPredThreads - 1) / kPredThreads, ndraws);
-  FBH_HIP_CHECK(hipEventRecord(e0, nullptr));
-  hipLaunchKernelGGL(k_predict, grid, dim3(kPredThreads), 0, nullptr, t->d_var, t->d_child, t->d_cut, t->d_theta,
-                     t->d_root, d_draws, t->d_lo, r->d_x, r->n, t->ntree, mu, d_out);
-  FBH_HIP_CHECK(hipGetLastError());
-  FBH_HIP_CHECK(hipEventRecord(e1, nullptr));
-  FBH_HIP_CHECK(hipDeviceSynchronize());
-  FBH_HIP_CHECK(hipMemcpy(out_host, d_out, sizeof(double) * (size_t)ndraws * r->n, hipMemcpyDeviceToHost));
-  (void)hipFree(d_draws);
-  (void)hipFree(d_out);
+  if (!grow((void**)&t->d_draws, &t->draws_cap, sizeof(int) * ndraws) ||
+      !grow((void**)&t->d_out, &t->out_cap, sizeof(double) * (size_t)ndraws * r->n)) {
+    set_error(FBH_ENOMEM, "fbh_predict: hipMalloc of the workspace failed");
+    return FBH_ENOMEM;
+  }
+  FBH_HIP_CHECK(hipMemcpy(t->d_draws, draws, sizeof(int) * ndraws, hipMemcpyHostToDevice));
+  rc = predict_launch(t, r, t->d_draws, ndraws, mu, t->d_out, nullptr);
+  if (rc) return rc;
+  FBH_HIP_CHECK(hipStreamSynchronize(nullptr));
+  FBH_HIP_CHECK(hipMemcpy(out_host, t->d_out, sizeof(double) * (size_t)ndraws * r->n, hipMemcpyDeviceToHost));
   return FBH_OK;
 }
 
-extern "C" int fbh_predict_timing(const fbh_trees* t, double* avg_ms, int* nlaunch) {
+extern "C" int fbh_predict_rows_device(const fbh_trees* tc, const fbh_rows* r, const int* draws_dev, int ndraws,
+                                       double mu, double* out_dev, void* stream) {
+  fbh_trees* t = const_cast<fbh_trees*>(tc);
+  int rc = check_predict(t, r, ndraws);
+  if (rc) return rc;
+  if (ndraws > 0 && (!draws_dev || !out_dev)) {
+    set_error(FBH_EINVAL, "fbh_predict_rows_device: null draws or output");
+    return FBH_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(t->mu);
+  return predict_launch(t, r, draws_dev, ndraws, mu, out_dev, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int fbh_predict_timing(const fbh_trees* tc, double* avg_ms, int* nlaunch) {
+  fbh_trees* t = const_cast<fbh_trees*>(tc);
   if (!t) return FBH_EINVAL;
-  double tot = 0;
-  int k = 0;
-  for (size_t i = 0; i + 1 < t->ev.size(); i += 2) {
+  std::lock_guard<std::mutex> lk(t->mu);
+  double tot = t->ev_ms;
+  long k = t->ev_count;
+  for (int s = 0; s < fbh_trees::kEv; s++) {
+    if (!t->ev_live[s]) continue;
     float ms = 0;
-    if (hipEventElapsedTime(&ms, t->ev[i], t->ev[i + 1]) == hipSuccess) {
+    if (hipEventSynchronize(t->ev1[s]) == hipSuccess && hipEventElapsedTime(&ms, t->ev0[s], t->ev1[s]) == hipSuccess) {
       tot += ms;
       k++;
     }
   }
   if (avg_ms) *avg_ms = k ? tot / k : 0;
-  if (nlaunch) *nlaunch = k;
+  if (nlaunch) *nlaunch = (int)k;
   return FBH_OK;
 }
 
@@ -554,4 +865,40 @@ extern "C" int fbh_predict_draws(const fbh_trees* t, const double* const* cols, 
 extern "C" int fbh_predict_draw(const fbh_trees* t, const double* const* cols, int n, int p, int m_draw, double mu,
                                 double* out) {
   return fbh_predict_draws(t, cols, n, p, &m_draw, 1, mu, out);
+}
+
+extern "C" int fbh_trees_rank_form(const fbh_trees* t) { return t && t->rank_ok ? 1 : 0; }
+
+extern "C" int fbh_device_alloc(size_t bytes, int device, void** out) {
+  if (!out) return FBH_EINVAL;
+  *out = nullptr;
+  int rc = ensure_device(device);
+  if (rc) return rc;
+  FBH_HIP_CHECK(hipSetDevice(device));
+  if (hipMalloc(out, bytes ? bytes : 8) != hipSuccess) {
+    set_error(FBH_ENOMEM, "fbh_device_alloc: hipMalloc failed");
+    return FBH_ENOMEM;
+  }
+  return FBH_OK;
+}
+
+extern "C" void fbh_device_free(void* p, int device) {
+  if (!p) return;
+  (void)hipSetDevice(device);
+  (void)hipFree(p);
+}
+
+extern "C" int fbh_device_copy(void* dst, const void* src, size_t bytes, int kind, int device) {
+  if ((!dst || !src) && bytes) return FBH_EINVAL;
+  FBH_HIP_CHECK(hipSetDevice(device));
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                       : hipMemcpyDeviceToDevice;
+  FBH_HIP_CHECK(hipMemcpy(dst, src, bytes, k));
+  return FBH_OK;
+}
+
+extern "C" int fbh_device_sync(int device) {
+  FBH_HIP_CHECK(hipSetDevice(device));
+  FBH_HIP_CHECK(hipDeviceSynchronize());
+  return FBH_OK;
 }

@@ -34,6 +34,17 @@ class CompiledTrees:
         self.probit = probit
         self.device = device
 
+    @property
+    def rank_form(self) -> bool:
+        """True when predictions take the rank-form kernel (sorted, NaN-free cutpoints)."""
+        return bool(lib().fbh_trees_rank_form(self.h))
+
+    def timing(self):
+        """(average predict-kernel duration in ms, launches) since compilation."""
+        ms, k = C.c_double(), C.c_int()
+        check(lib().fbh_predict_timing(self.h, C.byref(ms), C.byref(k)), "fbh_predict_timing")
+        return ms.value, k.value
+
     def arrays(self):
         k, r = self.nnodes, self.ndpost * self.ntree
         var, child, root = np.zeros(k, np.int32), np.zeros(k, np.int32), np.zeros(r, np.int32)
@@ -99,6 +110,13 @@ class DeviceRows:
         check(lib().fbh_predict_rows(ct.h, self.h, ptr(d), len(d), ct.mu, ptr(out)), "fbh_predict_rows")
         return out
 
+    def predict_device(self, ct: CompiledTrees, draws_dev: "DeviceBuffer", ndraws: int, out_dev: "DeviceBuffer"):
+        """Enqueue draws_dev[0..ndraws) (device int32, 1-based) into out_dev [ndraws][n] (device float64); no wait."""
+        if draws_dev.nbytes < 4 * ndraws or out_dev.nbytes < 8 * ndraws * self.n:
+            raise ValueError("predict_device: device buffers too small")
+        check(lib().fbh_predict_rows_device(ct.h, self.h, draws_dev.ptr, int(ndraws), ct.mu, out_dev.ptr, None),
+              "fbh_predict_rows_device")
+
     def close(self):
         if getattr(self, "h", None):
             lib().fbh_rows_free(self.h)
@@ -109,6 +127,44 @@ class DeviceRows:
             self.close()
         except Exception:
             pass
+
+
+class DeviceBuffer:
+    """Device memory owned by the library (fbh_device_alloc), for the device-side entry points."""
+
+    def __init__(self, nbytes: int, device: int = 0):
+        p = C.c_void_p()
+        check(lib().fbh_device_alloc(int(nbytes), device, C.byref(p)), "fbh_device_alloc")
+        self.ptr, self.nbytes, self.device = p, int(nbytes), device
+
+    @classmethod
+    def from_array(cls, a: np.ndarray, device: int = 0) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes, device)
+        check(lib().fbh_device_copy(b.ptr, ptr(a), a.nbytes, 0, device), "fbh_device_copy")
+        return b
+
+    def to_array(self, dtype, shape) -> np.ndarray:
+        out = np.zeros(shape, dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("to_array: larger than the buffer")
+        check(lib().fbh_device_copy(ptr(out), self.ptr, out.nbytes, 1, self.device), "fbh_device_copy")
+        return out
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().fbh_device_free(self.ptr, self.device)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_sync(device: int = 0):
+    check(lib().fbh_device_sync(device), "fbh_device_sync")
 
 
 def compile_trees(fit, device: int = 0) -> CompiledTrees:

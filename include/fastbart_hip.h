@@ -148,10 +148,26 @@ int fbh_predict_draws(const fbh_trees* t, const double* const* cols, int n, int 
 typedef struct fbh_rows fbh_rows;
 int fbh_rows_upload(const double* const* cols, int n, int p, int device, fbh_rows** out);
 void fbh_rows_free(fbh_rows* r);
-/* out_dev may be null (results copied to out_host [ndraws][n]) */
+/* draws (host, 1-based) for the uploaded rows; results copied to out_host [ndraws][n] (predict_draw_cpp,
+ * src/predict.cpp:49-73, over many draws) */
 int fbh_predict_rows(const fbh_trees* t, const fbh_rows* r, const int* draws, int ndraws, double mu, double* out_host);
+/* The same with the draw list and the output in device memory (draws_dev [ndraws], out_dev [ndraws][n]), enqueued on
+ * stream (a hipStream_t; null = the default stream) without waiting. A draw outside 1..ndpost yields NaN for its row of
+ * out_dev. Calls on one trees handle must use one stream (or be ordered by the caller): the handle keeps the ranks of
+ * the last rows it predicted in a buffer of its own. */
+int fbh_predict_rows_device(const fbh_trees* t, const fbh_rows* r, const int* draws_dev, int ndraws, double mu,
+                            double* out_dev, void* stream);
 /* average predict-kernel duration (ms) and launch count since the trees were compiled */
 int fbh_predict_timing(const fbh_trees* t, double* avg_ms, int* nlaunch);
+/* 1 when the trees and their cutpoint table take the rank-form kernel (cutpoints of every variable nondecreasing and
+ * free of NaN, at most 65,535 per variable, trees at most 255 levels deep), 0 when the general kernel is used */
+int fbh_trees_rank_form(const fbh_trees* t);
+/* device memory for the device-side entry points: allocate, free, copy (kind 0 host->device, 1 device->host,
+ * 2 device->device), wait for the device */
+int fbh_device_alloc(size_t bytes, int device, void** out);
+void fbh_device_free(void* p, int device);
+int fbh_device_copy(void* dst, const void* src, size_t bytes, int kind, int device);
+int fbh_device_sync(int device);
 
 /* ---------------------------------------------------------------- learned birth proposal: the network
  * No reference counterpart (the reference draws the birth rule uniformly, src/bart_kernel.cpp:433-437). A 3-layer MLP

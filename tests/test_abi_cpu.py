@@ -124,6 +124,25 @@ def test_compile_trees_parser_matches_oracle(oracle, golden_cases):
     assert ct.ndpost == b["ndpost"] and ct.ntree == b["ntree"]
 
 
+def test_compile_trees_rank_form_eligibility():
+    """The rank-form predictor needs every variable's cutpoints nondecreasing and free of NaN: x >= xi[c] is then
+    monotone in c and equals c < #{k : x >= xi[k]}. Brute-force check of that equivalence on ties, infinities and NaN
+    rows, and of which tables the library accepts for the rank form."""
+    from fastbart_amd.predict import CompiledTrees
+    text = "1 1 2\n3\n1 1 2 0\n2 0 0 -1\n3 0 0 1\n"
+    xi_ok = [np.array([0.5]), np.array([0.1, 0.2, 0.2, 0.7, 0.7, 0.9])]
+    assert CompiledTrees(text, xi_ok).rank_form
+    assert not CompiledTrees(text, [xi_ok[0], xi_ok[1][::-1].copy()]).rank_form
+    assert not CompiledTrees(text, [xi_ok[0], np.array([0.1, np.nan, 0.3])]).rank_form
+    assert not CompiledTrees(text, [np.array([np.nan]), xi_ok[1]]).rank_form
+    xi = xi_ok[1]
+    xs = np.concatenate([xi, np.nextafter(xi, -np.inf), np.nextafter(xi, np.inf), [-np.inf, np.inf, np.nan, -0.0]])
+    for x in xs:
+        rank = sum(1 for k in range(len(xi)) if x >= xi[k])
+        for c in range(len(xi)):
+            assert (x >= xi[c]) == (c < rank), (x, c)
+
+
 def test_compile_trees_rejects_malformed_text():
     from fastbart_amd.predict import CompiledTrees
     with pytest.raises(_lib.FbhError):

@@ -264,22 +264,23 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
 
 
 # ------------------------------------------------------------------ prediction (compile_trees / predict_draw)
-def _predict_cpu(jobs, calls):
-    """The oracle's restatement of predict_draw_cpp (src/predict.cpp:49-73) on one core: `calls` single-draw calls
-    spread over the fits; returns (row-trees per second, CPU seconds)."""
+def _predict_cpu(jobs, seconds):
+    """The oracle's restatement of predict_draw_cpp (src/predict.cpp:49-73) on one core: single-draw calls spread
+    over the fits for about `seconds`; returns (row-trees per second, CPU seconds, calls)."""
     from oracle import pyoracle
     orc = pyoracle.Oracle()
     prepared = []
     for text, cuts, x, mu in jobs:
         oc = orc.compile_trees(text, cuts)
         prepared.append((oc, [np.ascontiguousarray(x[:, v]) for v in range(x.shape[1])], mu))
-    work, t0 = 0, time.perf_counter()
-    for k in range(calls):
+    work, k, t0 = 0, 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
         oc, cols, mu = prepared[k % len(prepared)]
-        orc.predict_draw(oc, cols, 1 + k % oc["ndpost"], mu)
+        orc.predict_draw(oc, cols, 1 + (k // len(prepared)) % oc["ndpost"], mu)
         work += len(cols[0]) * oc["ntree"]
+        k += 1
     dt = time.perf_counter() - t0
-    return work / dt, dt
+    return work / dt, dt, k
 
 
 def predict_bench(args, wl):
@@ -374,12 +375,15 @@ def predict_bench(args, wl):
         f"{rowtrees * reps / T1 / 1e9:.2f} G/s; host output {rowtrees / T2 / 1e9:.2f} G/s; rank form {rank}")
     cpu = None
     if not args.no_cpu:
-        cpu_v, cpu_s = _predict_cpu(jobs, 12 if wl == "predict600" else 6)
+        cpu_v, cpu_s, calls = _predict_cpu(jobs, 5.0)
         cpu = dict(value=cpu_v, unit="row-trees/s", cores=1, kind="port",
-                   sample=f"{12 if wl == 'predict600' else 6} single-draw predict_draw calls of the oracle's "
-                          f"restatement of src/predict.cpp (same fits and rows), one core, {cpu_s:.1f} s")
+                   sample=f"{calls} single-draw predict_draw calls of the oracle's restatement of src/predict.cpp "
+                          f"(same fits and rows), one core, {cpu_s:.1f} s")
     for x in d_outs + singles + [d_draws]:
         x.close()
+    if args.calib_copy:
+        ms = sampler.debug_copy(args.calib_copy)
+        log(f"calibration copy of {args.calib_copy} MiB: {ms:.3f} ms")
     line = {
         "metric": "predict_draw row-trees/sec (compile_trees / predict_draw)", "value": value, "unit": "row-trees/s",
         "n_gpus": 1, "steps": reps, "warmup": args.warmup if args.warmup is not None else 3,

@@ -47,8 +47,8 @@ __global__ __launch_bounds__(256) void k_yhat(const uint64_t* __restrict__ trees
 // the variable and the cut index (0xFFFF for a leaf, which no rank exceeds, so a leaf maps to itself). A tree is
 // walked a fixed number of steps, its depth, with no branch: k = next[k] + (rank[var[k]] > c[k]).
 constexpr int kRankThreads = 256;
-constexpr int kRankRowsPerThread = 2;
-constexpr int kRankRows = kRankThreads * kRankRowsPerThread;  // rows per workgroup
+constexpr int kRankTrees = 4;             // trees walked at once per row (independent walks, added in tree order)
+constexpr int kRankRowsMax = 2 * kRankThreads;  // rows per workgroup at most (2 per thread)
 constexpr uint32_t kLeafCut = 0xFFFFu;
 
 // ranks [p][n] u16 of the rows x [p][n] against the cutpoints xi (flat, variable v < pcut at xi[off[v] ..
@@ -70,9 +70,12 @@ __global__ __launch_bounds__(256) void k_rank(const double* __restrict__ x, cons
   rank[(size_t)v * n + i] = (uint16_t)lo;
 }
 
-// One workgroup = one draw x kRankRows rows. LDS holds the draw's node words and leaf values, its per-tree
-// (root, depth) words and the rows' ranks [p][kRankRows]; every thread adds the trees in order for its rows, from
-// 0.0, then mu (src/predict.cpp:64-72).
+// One workgroup = one draw x (256 R) rows. LDS holds the draw's node words and leaf values, its per-tree
+// (root, depth) words and the rows' ranks [p][256 R]; every thread adds the trees in order for its R rows, from 0.0,
+// then mu (src/predict.cpp:64-72). The trees are walked kRankTrees at a time (for the deepest of them; a leaf maps to
+// itself), so that a thread has R x kRankTrees independent LDS chains in flight; their leaf values are then added in
+// tree order.
+template <int R>
 __global__ __launch_bounds__(kRankThreads) void k_predict_rank(const uint64_t* __restrict__ node,
                                                                const double* __restrict__ theta,
                                                                const uint32_t* __restrict__ tinfo,
@@ -81,12 +84,13 @@ __global__ __launch_bounds__(kRankThreads) void k_predict_rank(const uint64_t* _
                                                                const uint16_t* __restrict__ rank, int n, int p,
                                                                int ntree, int ndpost, double mu,
                                                                double* __restrict__ out) {
+  constexpr int kRows = kRankThreads * R;
   extern __shared__ __align__(16) unsigned char pred_lds[];
   const int k = blockIdx.y;
   const int dr = draws[k] - 1;
-  const int row0 = blockIdx.x * kRankRows;
+  const int row0 = blockIdx.x * kRows;
   if (dr < 0 || dr >= ndpost) {  // a draw outside 1..ndpost (device draw lists are not checked on the host): NaN
-    for (int r = threadIdx.x; r < kRankRows; r += kRankThreads)
+    for (int r = threadIdx.x; r < kRows; r += kRankThreads)
       if (row0 + r < n) out[(size_t)k * n + row0 + r] = __builtin_nan("");
     return;
   }
@@ -101,37 +105,52 @@ __global__ __launch_bounds__(kRankThreads) void k_predict_rank(const uint64_t* _
   }
   for (int t = threadIdx.x; t < ntree; t += kRankThreads) s_tinfo[t] = tinfo[(size_t)dr * ntree + t];
   for (int v = 0; v < p; v++)
-    for (int r = threadIdx.x; r < kRankRows; r += kRankThreads) {
+    for (int r = threadIdx.x; r < kRows; r += kRankThreads) {
       const int i = row0 + r;
-      s_rank[v * kRankRows + r] = i < n ? rank[(size_t)v * n + i] : (uint16_t)0;
+      s_rank[v * kRows + r] = i < n ? rank[(size_t)v * n + i] : (uint16_t)0;
     }
   __syncthreads();
-  double f[kRankRowsPerThread];
-  const uint16_t* rk[kRankRowsPerThread];
+  double f[R];
+  const uint16_t* rk[R];
 #pragma unroll
-  for (int j = 0; j < kRankRowsPerThread; j++) {
+  for (int j = 0; j < R; j++) {
     f[j] = 0.0;
     rk[j] = s_rank + threadIdx.x + j * kRankThreads;
   }
-  for (int t = 0; t < ntree; t++) {
-    const uint32_t ti = s_tinfo[t];
-    const int depth = (int)(ti >> 24);
-    int kk[kRankRowsPerThread];
+  for (int t0 = 0; t0 < ntree; t0 += kRankTrees) {
+    // roots and the deepest depth of the group; trees past the end repeat tree t0 and are not added
+    int root[kRankTrees];
+    int depth = 0;
 #pragma unroll
-    for (int j = 0; j < kRankRowsPerThread; j++) kk[j] = (int)(ti & 0xFFFFFFu);
+    for (int u = 0; u < kRankTrees; u++) {
+      const uint32_t ti = s_tinfo[t0 + u < ntree ? t0 + u : t0];
+      root[u] = (int)(ti & 0xFFFFFFu);
+      depth = max(depth, (int)(ti >> 24));
+    }
+    int kk[R][kRankTrees];
+#pragma unroll
+    for (int j = 0; j < R; j++)
+#pragma unroll
+      for (int u = 0; u < kRankTrees; u++) kk[j][u] = root[u];
     for (int d = 0; d < depth; d++) {
 #pragma unroll
-      for (int j = 0; j < kRankRowsPerThread; j++) {
-        const uint64_t w = s_node[kk[j]];
-        const uint32_t var = (uint32_t)(w >> 32) & 0xFFFFu, c = (uint32_t)(w >> 48);
-        kk[j] = (int)(uint32_t)w + (int)((uint32_t)rk[j][var * kRankRows] > c);
-      }
+      for (int j = 0; j < R; j++)
+#pragma unroll
+        for (int u = 0; u < kRankTrees; u++) {
+          const uint64_t w = s_node[kk[j][u]];
+          const uint32_t var = (uint32_t)(w >> 32) & 0xFFFFu, c = (uint32_t)(w >> 48);
+          kk[j][u] = (int)(uint32_t)w + (int)((uint32_t)rk[j][var * kRows] > c);
+        }
     }
 #pragma unroll
-    for (int j = 0; j < kRankRowsPerThread; j++) f[j] += s_theta[kk[j]];
+    for (int u = 0; u < kRankTrees; u++)
+      if (t0 + u < ntree) {
+#pragma unroll
+        for (int j = 0; j < R; j++) f[j] += s_theta[kk[j][u]];
+      }
   }
 #pragma unroll
-  for (int j = 0; j < kRankRowsPerThread; j++) {
+  for (int j = 0; j < R; j++) {
     const int i = row0 + threadIdx.x + j * kRankThreads;
     if (i < n) out[(size_t)k * n + i] = f[j] + mu;
   }
@@ -705,8 +724,8 @@ static bool grow(void** buf, size_t* cap, size_t bytes) {
 }
 
 // LDS bytes of one k_predict_rank workgroup for draws of up to `nodes` nodes
-static size_t rank_lds_bytes(int nodes, int ntree, int p) {
-  return (size_t)nodes * 16 + (size_t)((ntree + 3) & ~3) * 4 + (size_t)p * kRankRows * 2;
+static size_t rank_lds_bytes(int nodes, int ntree, int p, int rows) {
+  return (size_t)nodes * 16 + (size_t)((ntree + 3) & ~3) * 4 + (size_t)p * rows * 2;
 }
 // Dynamic LDS the rank kernel may use on a device: the device's per-workgroup limit, raised for the kernel once per
 // device (0 if that fails, which sends every prediction to the general kernel).
@@ -720,8 +739,10 @@ static size_t rank_lds_limit(int device) {
     done[device] = true;
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(k_predict_rank), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            v) == hipSuccess)
+        hipFuncSetAttribute(reinterpret_cast<const void*>(k_predict_rank<1>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, v) == hipSuccess &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(k_predict_rank<2>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, v) == hipSuccess)
       lim[device] = (size_t)v;
   }
   return lim[device];
@@ -751,7 +772,10 @@ static int predict_launch(fbh_trees* t, const fbh_rows* r, const int* d_draws, i
     }
     t->ev_live[slot] = false;
   }
-  const size_t lds = rank_lds_bytes(t->max_draw_nodes, t->ntree, r->p);
+  // two rows per thread when that still gives every CU two workgroups, else one (few draws: more, shorter blocks)
+  const long blocks2 = (long)((r->n + kRankRowsMax - 1) / kRankRowsMax) * ndraws;
+  const int rows_per_thread = blocks2 >= 512 ? 2 : 1;
+  const size_t lds = rank_lds_bytes(t->max_draw_nodes, t->ntree, r->p, kRankThreads * rows_per_thread);
   const size_t lds_max = rank_lds_limit(t->device);
   const bool rank_form = t->rank_ok && lds <= lds_max;
   if (rank_form && t->rank_rows_uid != r->uid) {
@@ -769,9 +793,14 @@ static int predict_launch(fbh_trees* t, const fbh_rows* r, const int* d_draws, i
   }
   FBH_HIP_CHECK(hipEventRecord(t->ev0[slot], s));
   if (rank_form) {
-    dim3 grid((r->n + kRankRows - 1) / kRankRows, ndraws);
-    hipLaunchKernelGGL(k_predict_rank, grid, dim3(kRankThreads), lds, s, t->d_node, t->d_theta, t->d_tinfo, t->d_lo,
-                       d_draws, t->d_rank, r->n, r->p, t->ntree, t->ndpost, mu, d_out);
+    const int rows = kRankThreads * rows_per_thread;
+    dim3 grid((r->n + rows - 1) / rows, ndraws);
+    if (rows_per_thread == 2)
+      hipLaunchKernelGGL(k_predict_rank<2>, grid, dim3(kRankThreads), lds, s, t->d_node, t->d_theta, t->d_tinfo,
+                         t->d_lo, d_draws, t->d_rank, r->n, r->p, t->ntree, t->ndpost, mu, d_out);
+    else
+      hipLaunchKernelGGL(k_predict_rank<1>, grid, dim3(kRankThreads), lds, s, t->d_node, t->d_theta, t->d_tinfo,
+                         t->d_lo, d_draws, t->d_rank, r->n, r->p, t->ntree, t->ndpost, mu, d_out);
   } else {
     dim3 grid((r->n + kPredThreads - 1) / kPredThreads, ndraws);
     hipLaunchKernelGGL(k_predict, grid, dim3(kPredThreads), 0, s, t->d_var, t->d_child, t->d_cut, t->d_theta,

@@ -41,7 +41,10 @@ struct Geo {
   static constexpr int kTile = kLanes * kTW4;                 // rows in a tile
   static constexpr int kStride = kTW4 + 1;                    // doubles per lane row block (bank offset)
   static constexpr int kTagStride = (kTW4 + 4) & ~3;          // bytes per lane tag block
-  static constexpr int kScratchBytes = 2 * kLanes * kStride * 8 + kLanes * kTagStride;
+  // the w*r blocks start 8 doubles (16 banks) after the w blocks end: a phase-B wave reads w and w*r of the same row of
+  // the same lane in one instruction (even and odd task slots), which then fall in different banks
+  static constexpr int kV1Pad = 8;
+  static constexpr int kScratchBytes = (2 * kLanes * kStride + kV1Pad) * 8 + kLanes * kTagStride;
 };
 
 struct KArgs {
@@ -116,6 +119,12 @@ __device__ __forceinline__ uint32_t opaque(uint32_t i) {
   asm volatile("" : "+v"(i));
   return i;
 }
+
+// Index of row i in the LDS-resident running fit: one pad slot after every 16 rows. A phase-A wave touches rows
+// 16 s + k for 16 consecutive s and its 4 lanes k; unpadded, those doubles fall in 4 bank pairs (16-way conflicts),
+// padded in all 16.
+__device__ __forceinline__ uint32_t afl(uint32_t i) { return i + (i >> 4); }
+inline size_t afl_doubles(size_t n) { return n + (n + 15) / 16; }
 
 // xoshiro128** (Blackman & Vigna), state order of src/bart_kernel.cpp:54-58
 __device__ __forceinline__ uint32_t rng_next(Rng& g) {
@@ -595,7 +604,7 @@ __device__ __forceinline__ void tile_load(TileIn<R>& T, const RowPtrs& P, const 
     T.qb[r] = ldo(qb, i);
     T.y[r] = ldo(P.Y, i);
     T.w[r] = ldo(P.W, i);
-    T.af[r] = RES ? P.AFl[i] : ldo((const double*)P.AF, i);
+    T.af[r] = RES ? P.AFl[afl(i)] : ldo((const double*)P.AF, i);
   }
 }
 
@@ -626,7 +635,7 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
   const int al = lane & 3, iis = lane >> 2, ka = kLanesPerWave * dw + al;
   const int bl = lane >> 4, slot = lane & 15, kb = kLanesPerWave * dw + bl;
   double* v0 = reinterpret_cast<double*>(scratch);
-  double* v1 = v0 + kLanes * G::kStride;
+  double* v1 = v0 + kLanes * G::kStride + G::kV1Pad;
   uint8_t* tg = reinterpret_cast<uint8_t*>(v1 + kLanes * G::kStride);
   double* v0a = v0 + (kLanesPerWave * dw + al) * G::kStride;
   double* v1a = v1 + (kLanesPerWave * dw + al) * G::kStride;
@@ -661,7 +670,7 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
           lf2 = pnx;
         af = af - c.tho[lf] + c.thn[lf2];
         sto(lprev, i, (uint8_t)lf2);
-        if (RES) P.AFl[i] = af;
+        if (RES) P.AFl[afl(i)] = af;
         else sto(P.AF, i, af);
       }
       const double y = T.y[r];
@@ -817,7 +826,7 @@ __device__ __attribute__((noinline)) void probit_latents(int n_, double off_, co
       double fv = 0.;
       int ybv = 0;
       if (reff > 0 && lr < reff) {
-        fv = RES ? AF[row0 + lr] : gld(AF + row0 + lr);
+        fv = RES ? AF[afl(row0 + lr)] : gld(AF + row0 + lr);
         ybv = gld(YB + row0 + lr);
       }
       const int j = tid - 64;
@@ -828,7 +837,7 @@ __device__ __attribute__((noinline)) void probit_latents(int n_, double off_, co
       float lfj = 0.f, cfj = 0.f;
       double ldj = 0.;
       if (fin) {
-        fj = RES ? AF[prow0 + j] : gld(AF + prow0 + j);
+        fj = RES ? AF[afl(prow0 + j)] : gld(AF + prow0 + j);
         ybj = gld(YB + prow0 + j);
         const uint32_t k = pS + (uint32_t)j + B.acc[j];
         x1 = slot_x1(B, k);
@@ -891,7 +900,7 @@ __device__ __attribute__((noinline)) void probit_latents(int n_, double off_, co
       int used = done + r;
       if (done == 0) {
         // the window's first row rejected kPT times in a row: its attempts one by one, generating as needed
-        const LatentRow r0 = latent_row(RES ? AF[row0] : gld(AF + row0), (int)gld(YB + row0), off);
+        const LatentRow r0 = latent_row(RES ? AF[afl(row0)] : gld(AF + row0), (int)gld(YB + row0), off);
         uint32_t k = 0;
         for (;; k++) {
           if (S + k >= F) ring_generate(B, h, F, S + k + 1, lane, k11, k21);
@@ -973,7 +982,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
   const double t2 = (double)c.cd.tau * c.cd.tau;
   const double a_prior = 1 / t2;
   if (RES && !ctl)
-    for (int i = tid - 64; i < n; i += kBlock - 64) P.AFl[i] = P.AF[i];
+    for (int i = tid - 64; i < n; i += kBlock - 64) P.AFl[afl(i)] = P.AF[i];
 
   // The two roles run separate loops with the same sequence of workgroup barriers per sweep (1 + 2 m + 1, and 2 more
   // for the probit redraw), so that neither role's live registers are held across the other's code.
@@ -1243,7 +1252,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       __syncthreads();
     }
     if (RES)
-      for (int i = tid - 64; i < n; i += kBlock - 64) P.AF[i] = P.AFl[i];
+      for (int i = tid - 64; i < n; i += kBlock - 64) P.AF[i] = P.AFl[afl(i)];
   }
 }
 
@@ -2178,7 +2187,7 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device) != hipSuccess)
       lds_max = 65536;
     const size_t budget = std::min<size_t>((size_t)lds_max, kLdsBudgetPerWG);
-    const size_t need_res = kCtlBytes + align16(Geo<kRowsRes>::kScratchBytes) + 8 * (size_t)b->nmax;
+    const size_t need_res = kCtlBytes + align16(Geo<kRowsRes>::kScratchBytes) + 8 * afl_doubles((size_t)b->nmax);
     const bool res = need_res <= budget && o->tile_rows != Geo<kRowsStream>::kTile;
     if (o->tile_rows == Geo<kRowsRes>::kTile && !res) {
       set_error(FBH_EINVAL, "fbh_batch_create: 1536-row tiles need the running fit resident in LDS (n too large)");
@@ -2187,7 +2196,7 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     }
     b->var = pick_variant(res, b->stats_mode == FBH_STATS_FX64, b->probit != 0);
     b->af_off = (int)align16(b->var.scratch);
-    b->dyn_bytes = (size_t)b->af_off + (res ? 8 * (size_t)b->nmax : 0);
+    b->dyn_bytes = (size_t)b->af_off + (res ? 8 * afl_doubles((size_t)b->nmax) : 0);
     if (kCtlBytes + b->dyn_bytes > (size_t)lds_max) {
       set_error(FBH_EINVAL, "fbh_batch_create: LDS need exceeds the device limit");
       free_batch(b);

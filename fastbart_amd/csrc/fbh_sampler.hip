@@ -653,48 +653,73 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
   if (!first_loaded) tile_load<R, RES>(T, P, la, lb, qa, qb, 0, iis, ka);
   for (int tb = 0; tb < n; tb += G::kTile) {
     const int ii0 = tb / kLanes;
-    // ---- phase A: the arithmetic on the tile's loads (issued before the previous tile's phase B)
+    // ---- phase A: the arithmetic on the tile's loads (issued before the previous tile's phase B). First the leaf ids
+    // after tree j-1's change and every LDS gather of the tile's rows (leaf values, ranks), all in flight together;
+    // then the arithmetic and the stores. Rows past n compute on row n-1's loads and store nothing global.
     const uint32_t abase = opaque((uint32_t)(iis * kLanes + ka));
+    constexpr int RC = 2;  // rows per gather batch (small batches keep the registers in budget)
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-      const int idx = iis + 16 * r;
-      const uint32_t i = abase + (uint32_t)((ii0 + 16 * r) * kLanes);
-      if (i >= (uint32_t)n) continue;
-      double af = T.af[r];
-      if (apply) {
+    for (int r0 = 0; r0 < R; r0 += RC) {
+      int lfn[RC];
+      double tho_r[RC], thn_r[RC], thc_r[RC];
+      uint32_t rank_r[RC];
+#pragma unroll
+      for (int u = 0; u < RC; u++) {
+        const int r = r0 + u;
         const int lf = (int)T.la[r];
         int lf2 = lf;
-        if (pmode == 1 && lf == pnx)
-          lf2 = (int)T.qa[r] <= pc ? pL : pR;
-        else if (pmode == 2 && (lf == pl || lf == pr))
-          lf2 = pnx;
-        af = af - c.tho[lf] + c.thn[lf2];
-        sto(lprev, i, (uint8_t)lf2);
-        if (RES) P.AFl[afl(i)] = af;
-        else sto(P.AF, i, af);
-      }
-      const double y = T.y[r];
-      const double w = (double)T.w[r];
-      if (rss) {
-        const double e = y - af;
-        const double c2 = (w * e) * e;
-        v0a[idx] = 0.0;
-        v1a[idx] = FX ? __longlong_as_double(fx_round(c2, 16777216.0)) : c2;
-        tga[idx] = 0;
-        if (keep) sto(P.TM, i, ldo((const double*)P.TM, i) + af);
-      } else {
-        const int lf = (int)T.lb[r];
-        const double rr = y - af + thc[lf];
-        const double wr = w * rr;
-        int side = 0;
-        if (cmode == 1 && lf == cnx) {
-          side = (int)T.qb[r] <= cc ? 1 : 2;
-          my_nl += side == 1;
-          my_nr += side == 2;
+        if (pmode == 1)
+          lf2 = lf == pnx ? ((int)T.qa[r] <= pc ? pL : pR) : lf;
+        else if (pmode == 2)
+          lf2 = (lf == pl || lf == pr) ? pnx : lf;
+        lfn[u] = lf2;
+        if (apply) {
+          tho_r[u] = c.tho[lf];
+          thn_r[u] = c.thn[lf2];
         }
-        v0a[idx] = FX ? __longlong_as_double(fx_round(w, 4294967296.0)) : w;
-        v1a[idx] = FX ? __longlong_as_double(fx_round(wr, 4294967296.0)) : wr;
-        tga[idx] = (uint8_t)(rank_of[lf] | (side << 6));
+        if (!rss) {
+          thc_r[u] = thc[T.lb[r]];
+          rank_r[u] = rank_of[T.lb[r]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < RC; u++) {
+        const int r = r0 + u;
+        const int idx = iis + 16 * r;
+        const uint32_t i = abase + (uint32_t)((ii0 + 16 * r) * kLanes);
+        const bool valid = i < (uint32_t)n;
+        double af = T.af[r];
+        if (apply) {
+          af = af - tho_r[u] + thn_r[u];
+          if (valid) {
+            sto(lprev, i, (uint8_t)lfn[u]);
+            if (RES) P.AFl[afl(i)] = af;
+            else sto(P.AF, i, af);
+          }
+        }
+        const double y = T.y[r];
+        const double w = (double)T.w[r];
+        if (rss) {
+          const double e = y - af;
+          const double c2 = (w * e) * e;
+          v0a[idx] = 0.0;
+          v1a[idx] = FX ? __longlong_as_double(fx_round(c2, 16777216.0)) : c2;
+          tga[idx] = 0;
+          if (keep && valid) sto(P.TM, i, ldo((const double*)P.TM, i) + af);
+        } else {
+          const int lf = (int)T.lb[r];
+          const double rr = y - af + thc_r[u];
+          const double wr = w * rr;
+          int side = 0;
+          if (cmode == 1 && lf == cnx && valid) {
+            side = (int)T.qb[r] <= cc ? 1 : 2;
+            my_nl += side == 1;
+            my_nr += side == 2;
+          }
+          v0a[idx] = FX ? __longlong_as_double(fx_round(w, 4294967296.0)) : w;
+          v1a[idx] = FX ? __longlong_as_double(fx_round(wr, 4294967296.0)) : wr;
+          tga[idx] = (uint8_t)(rank_r[u] | (side << 6));
+        }
       }
     }
     wave_sync();

@@ -660,6 +660,8 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
     constexpr int RC = 2;  // rows per gather batch (small batches keep the registers in budget)
 #pragma unroll
     for (int r0 = 0; r0 < R; r0 += RC) {
+      // a batch whose first row for this wave is past n (the last tile) is skipped, and so are the later ones
+      if ((uint32_t)((ii0 + 16 * r0) * kLanes + kLanesPerWave * dw) >= (uint32_t)n) break;
       int lfn[RC];
       double tho_r[RC], thn_r[RC], thc_r[RC];
       uint32_t rank_r[RC];

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "fbh_internal.h"
@@ -582,16 +583,19 @@ __device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, cons
 __device__ __forceinline__ long long fx_round(double x, double scale) { return (long long)rint(x * scale); }
 
 // One phase-A thread's loads of a tile, kept raw (one register per byte) so that nothing consumes them before phase A:
-// the loads of tile t+1 are issued before phase B of tile t and arrive while it runs.
-template <int R>
+// the loads of tile t+1 are issued before phase B of tile t and arrive while it runs. YF: the outcome is read as the
+// float the caller passed (wbart; (double) of it is the reference's y), else the double latent (probit). AFT: the
+// running fit is part of the tile (streamed from HBM); when it is resident in LDS, phase A reads it with its gathers.
+template <int R, bool YF = false, bool AFT = true>
 struct TileIn {
   uint32_t la[R], lb[R], qa[R], qb[R];  // leaf id of tree j-1, of tree j; rank on v_prev, on v_cur
   float w[R];
-  double y[R], af[R];
+  typename std::conditional<YF, float, double>::type y[R];
+  double af[AFT ? R : 1];
 };
 
-template <int R, bool RES>
-__device__ __forceinline__ void tile_load(TileIn<R>& T, const RowPtrs& P, const uint8_t* __restrict__ la,
+template <int R, bool RES, bool YF, bool AFT>
+__device__ __forceinline__ void tile_load(TileIn<R, YF, AFT>& T, const RowPtrs& P, const uint8_t* __restrict__ la,
                                           const uint8_t* __restrict__ lb, const uint8_t* __restrict__ qa,
                                           const uint8_t* __restrict__ qb, int ii0, int iis, int ka) {
   const uint32_t base = opaque((uint32_t)(iis * kLanes + ka));  // row (ii0 + iis + 16 r) * 16 + ka
@@ -602,15 +606,17 @@ __device__ __forceinline__ void tile_load(TileIn<R>& T, const RowPtrs& P, const 
     T.lb[r] = ldo(lb, i);
     T.qa[r] = ldo(qa, i);
     T.qb[r] = ldo(qb, i);
-    T.y[r] = ldo(P.Y, i);
+    if (YF) T.y[r] = ldo(P.yf, i);
+    else T.y[r] = ldo(P.Y, i);
     T.w[r] = ldo(P.W, i);
-    T.af[r] = RES ? P.AFl[afl(i)] : ldo((const double*)P.AF, i);
+    if (AFT) T.af[r] = RES ? P.AFl[afl(i)] : ldo((const double*)P.AF, i);
   }
 }
 
-template <int R, bool RES, bool FX>
+template <int R, bool RES, bool FX, bool YF, bool AFT>
 __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int dw, int lane, int jp, int j, int cb,
-                          bool apply, bool stats, bool rss, bool keep, TileIn<R>& T, bool first_loaded) {
+                          bool apply, bool stats, bool rss, bool keep, TileIn<R, YF, AFT>& T, bool first_loaded) {
+  static_assert(AFT || RES, "a running fit outside LDS is part of the tile");
   using G = Geo<R>;
   const int n = P.n;
   // every control field read here is the same for the whole workgroup: read once into scalar registers
@@ -657,13 +663,13 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
     // after tree j-1's change and every LDS gather of the tile's rows (leaf values, ranks), all in flight together;
     // then the arithmetic and the stores. Rows past n compute on row n-1's loads and store nothing global.
     const uint32_t abase = opaque((uint32_t)(iis * kLanes + ka));
-    constexpr int RC = 2;  // rows per gather batch (small batches keep the registers in budget)
+    constexpr int RC = (R % 3 == 0) ? 3 : 2;  // rows per gather batch (batches keep the registers in budget)
 #pragma unroll
     for (int r0 = 0; r0 < R; r0 += RC) {
       // a batch whose first row for this wave is past n (the last tile) is skipped, and so are the later ones
       if ((uint32_t)((ii0 + 16 * r0) * kLanes + kLanesPerWave * dw) >= (uint32_t)n) break;
       int lfn[RC];
-      double tho_r[RC], thn_r[RC], thc_r[RC];
+      double tho_r[RC], thn_r[RC], thc_r[RC], af_r[RC];
       uint32_t rank_r[RC];
 #pragma unroll
       for (int u = 0; u < RC; u++) {
@@ -683,6 +689,12 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
           thc_r[u] = thc[T.lb[r]];
           rank_r[u] = rank_of[T.lb[r]];
         }
+        if (AFT) {
+          af_r[u] = T.af[r];
+        } else {
+          const uint32_t i = min(abase + (uint32_t)((ii0 + 16 * r) * kLanes), (uint32_t)(n - 1));
+          af_r[u] = P.AFl[afl(i)];
+        }
       }
 #pragma unroll
       for (int u = 0; u < RC; u++) {
@@ -690,7 +702,7 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
         const int idx = iis + 16 * r;
         const uint32_t i = abase + (uint32_t)((ii0 + 16 * r) * kLanes);
         const bool valid = i < (uint32_t)n;
-        double af = T.af[r];
+        double af = af_r[u];
         if (apply) {
           af = af - tho_r[u] + thn_r[u];
           if (valid) {
@@ -699,7 +711,7 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
             else sto(P.AF, i, af);
           }
         }
-        const double y = T.y[r];
+        const double y = (double)T.y[r];
         const double w = (double)T.w[r];
         if (rss) {
           const double e = y - af;
@@ -757,8 +769,9 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
 // columns it reads are fixed by then (tree j's leaf ids, tree j+1's, and the ranks on both proposal variables); whether
 // tree j's birth was accepted only decides whether phase A uses the ranks on its variable. After the last tree it is
 // the closing pass's tile.
-template <int R, bool RES>
-__device__ __forceinline__ void load_next_first_tile(TileIn<R>& T, const Ctl& c, const RowPtrs& P, int dw, int lane,
+template <int R, bool RES, bool YF, bool AFT>
+__device__ __forceinline__ void load_next_first_tile(TileIn<R, YF, AFT>& T, const Ctl& c, const RowPtrs& P, int dw,
+                                                     int lane,
                                                      int j, int m, int cb) {
   const int n = P.n;
   const int iis = lane >> 2, ka = kLanesPerWave * dw + (lane & 3);
@@ -1253,7 +1266,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       cs->n_cap = n_cap;
     }
   } else {
-    TileIn<R> T;
+    TileIn<R, !PB, !RES> T;
     for (int it = a.it0; it < a.it1; it++) {
       const bool keep = it >= a.nskip;
       __syncthreads();

@@ -562,7 +562,18 @@ __device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, cons
 #pragma unroll
       for (int u = 0; u < 16; u++) s += tag_match(t4[u >> 2], 8 * (u & 3) + sh, wd, pat) ? x[u] : 0ll;
     }
-    for (; i < cnt; i++) s += tag_match(tag[i], sh, wd, pat) ? v[i] : 0ll;
+    for (; i < cnt; i += 8) {
+      // the last rows in blocks of 8: rows at or past cnt hold earlier values and are masked out
+      const int rem = cnt - i;
+      uint32_t t2[2];
+      long long x[8];
+#pragma unroll
+      for (int u = 0; u < 2; u++) t2[u] = *reinterpret_cast<const uint32_t*>(tag + i + 4 * u);
+#pragma unroll
+      for (int u = 0; u < 8; u++) x[u] = v[i + u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += (u < rem && tag_match(t2[u >> 2], 8 * (u & 3) + sh, wd, pat)) ? x[u] : 0ll;
+    }
     return __longlong_as_double(s);
   }
   double s = acc;
@@ -576,7 +587,21 @@ __device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, cons
 #pragma unroll
     for (int u = 0; u < 16; u++) s = fma((double)tag_match(t4[u >> 2], 8 * (u & 3) + sh, wd, pat), x[u], s);
   }
-  for (; i < cnt; i++) s = fma((double)tag_match(tag[i], sh, wd, pat), src[i], s);
+  for (; i < cnt; i += 8) {
+    // the last rows in blocks of 8: rows at or past cnt hold earlier values (any bit pattern) and add +0.0
+    const int rem = cnt - i;
+    uint32_t t2[2];
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 2; u++) t2[u] = *reinterpret_cast<const uint32_t*>(tag + i + 4 * u);
+#pragma unroll
+    for (int u = 0; u < 8; u++) x[u] = src[i + u];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const bool in = u < rem;
+      s = fma(in ? (double)tag_match(t2[u >> 2], 8 * (u & 3) + sh, wd, pat) : 0.0, in ? x[u] : 0.0, s);
+    }
+  }
   return s;
 }
 
@@ -737,16 +762,16 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
       }
     }
     wave_sync();
+    // ---- phase B: this thread's tasks walk lane kb's rows of the tile in row order. task = 2 rank + kind: kind 0
+    // sums w, kind 1 sums w*r; ranks below nleaf are leaves (rank field), then the split halves (side field 1 and
+    // 2); the closing pass sums every row (an empty field matches bit 0). A slot past ntask sums nothing.
     // the next tile's loads go out now and arrive during phase B
     if (tb + G::kTile < n) tile_load<R, RES>(T, P, la, lb, qa, qb, ii0 + G::kTW4, iis, ka);
-    // ---- phase B: this thread's tasks walk lane kb's rows of the tile in row order
     const int cnt = min(G::kTW4, max(0, nper_kb - ii0));
 #pragma unroll
     for (int g = 0; g < kGroups; g++) {
       const int task = slot + kTaskSlots * g;
       if (task < ntask) {
-        // task = 2 rank + kind: kind 0 sums w, kind 1 sums w*r; ranks below nleaf are leaves (rank field), then the
-        // split halves (side field 1 and 2); the closing pass sums every row (an empty field matches bit 0)
         const int q = task >> 1;
         const uint32_t sh = (!rss && q >= nleaf) ? 6u : 0u;
         const uint32_t wd = rss ? 0u : (q < nleaf ? 5u : 2u);

@@ -422,6 +422,8 @@ def main() -> int:
                     help="sweeps per persistent launch (the warmup and the timed sweeps split into equal launches)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ess-rows", type=int, default=0, help="rows per chain used for ESS (0 = all)")
+    ap.add_argument("--row-split", type=int, default=None,
+                    help="workgroups per chain (FX64 workloads; default: 8 for maxshape64, else 1)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration under rocprofv3)")
     ap.add_argument("--ndpost", type=int, default=None, help="kept sweeps (learned512; longer runs steady ESS)")
@@ -468,8 +470,9 @@ def main() -> int:
     seeds = [1 + c for c in range(chains)]
     rows_total = sum(p["q"].shape[1] for p in probs)
     log(f"{wl}: generated {chains} chains in {time.perf_counter() - t0:.1f} s; total rows {rows_total}")
+    split = args.row_split if args.row_split is not None else (8 if wl == "maxshape64" else 1)
     b = sampler.BatchSampler(probs, M, nskip, ndpost, seeds, probit=probit, keep_trees=1, stats_mode=stats_mode,
-                             node_cap=node_cap, sweeps_per_launch=spl)
+                             node_cap=node_cap, sweeps_per_launch=spl, row_split=split)
     b.sync()
     t_w = time.perf_counter()
     if W_:
@@ -517,7 +520,8 @@ def main() -> int:
         "ms_per_step": T / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": describe(wl, chains, nskip, ndpost), "chains": chains, "ntree": M,
-                   "rows_total": rows_total, "parallelism": "chains"},
+                   "rows_total": rows_total,
+                   "parallelism": "chains" if split <= 1 else f"chains x {split} workgroups (row split)"},
         "min_ess_per_sec": ess,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -33,7 +33,7 @@ class Opts(C.Structure):
     _fields_ = [("m", C.c_int), ("nskip", C.c_int), ("ndpost", C.c_int), ("probit", C.c_int),
                 ("keep_trees", C.c_int), ("pgtab", C.c_void_p), ("stats_mode", C.c_int), ("node_cap", C.c_int),
                 ("device", C.c_int), ("sweeps_per_launch", C.c_int), ("block_threads", C.c_int),
-                ("tile_rows", C.c_int)]
+                ("tile_rows", C.c_int), ("row_split", C.c_int)]
 
 
 class ChainOut(C.Structure):

@@ -49,6 +49,10 @@ struct ChainDesc {
   uint64_t tree_off;  // TreeDev index (chain * m)
   uint64_t sig_off;   // float index into sigma_out
   uint64_t tout_off;  // word index into trees_out ([ndpost][m][64])
+  // row split: this workgroup's chain is member `member` of `split` workgroups sharing logical chain `group`, whose
+  // rows number n_total (n is this member's share); keep = 1 if it writes the kept trees
+  int n_total, split, member, group, keep;
+  int pad_[3];
 };
 
 // Mutable per-chain sampler scalars carried between launches.
@@ -56,6 +60,8 @@ struct ChainState {
   uint32_t rng[4];
   double sigma;
   int it, n_birth, n_death, n_cap;
+  uint32_t xseq;  // row split: exchanges done so far
+  int xerr;       // row split: an exchange timed out (the results are invalid)
 };
 
 // Device images of the host libm on the generator's 2^24-point uniform grid u_k = (k + 1/2) 2^-24:

@@ -69,6 +69,8 @@ struct KArgs {
   int m, nskip, ndpost, probit, keep_trees, stats_mode, node_cap;
   int it0, it1;
   int af_off;     // byte offset of the resident running fit in dynamic LDS
+  unsigned long long* xbuf;  // row split: [group][parity][member][kXw] pass totals
+  uint32_t* xctr;            // row split: [group] arrivals
 };
 
 struct Rng {
@@ -1007,6 +1009,58 @@ __device__ __attribute__((noinline)) void probit_latents(int n_, double off_, co
   __syncthreads();
 }
 
+// ------------------------------------------------------------------ row split: pass totals across the workgroups
+// Every member of a split chain runs the same control steps on the same generator; after each pass the members add
+// their fixed-point totals (int64, order-free) so that every member continues from the same sums. Protocol, all on
+// device-scope (agent) atomics so that members on different XCDs see each other's values at the coherence point:
+// store this member's totals, wait until the stores have completed, count the arrival, wait for all members'
+// arrivals, read the totals of every member. Slots alternate between two buffers (a member can reuse a buffer only
+// after every member has arrived at the next exchange, i.e. has finished reading it). A wait that exceeds the spin
+// limit (members that are not resident together) sets xerr and stops waiting in this and later exchanges.
+constexpr int kXw = kMaxTasks + 2;    // task totals, then the two split counts
+constexpr int kSpinLimit = 1 << 20;   // about a second of polling
+__device__ void row_split_exchange(Ctl& c, const KArgs& a, int cb, bool rss, uint32_t& xseq, int& xerr, int lane) {
+  const int W = c.cd.split, me = c.cd.member, grp = c.cd.group;
+  const int ntask = rss ? 2 : 2 * (c.nleaf[cb] + (c.cmode[cb] == 1 ? 2 : 0));
+  unsigned long long* X = a.xbuf + ((size_t)grp * 2 + (xseq & 1u)) * (size_t)W * kXw;
+  for (int t = lane; t < kXw; t += 64) {
+    long long v = 0;
+    if (t < ntask)
+      for (int k = 0; k < kLanes; k++) v += __double_as_longlong(c.part[t][k]);
+    else if (t == kXw - 2)
+      v = c.nL;
+    else if (t == kXw - 1)
+      v = c.nR;
+    __hip_atomic_store(X + (size_t)me * kXw + t, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed before its arrival is counted
+  if (lane == 0) {
+    __hip_atomic_fetch_add(a.xctr + grp, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t target = (xseq + 1u) * (uint32_t)W;
+    for (int spin = 0; !xerr; spin++) {
+      if (__hip_atomic_load(a.xctr + grp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+      if (spin >= kSpinLimit) xerr = 1;
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  xerr = (int)ufl((uint32_t)__shfl(xerr, 0));
+  for (int t = lane; t < kXw; t += 64) {
+    long long v = 0;
+    for (int k = 0; k < W; k++)
+      v += (long long)__hip_atomic_load(X + (size_t)k * kXw + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t < ntask) {
+      c.part[t][0] = __longlong_as_double(v);
+      for (int k = 1; k < kLanes; k++) c.part[t][k] = 0.0;
+    } else if (t == kXw - 2) {
+      c.nL = (int)v;
+    } else if (t == kXw - 1) {
+      c.nR = (int)v;
+    }
+  }
+  xseq++;
+  wave_sync();
+}
+
 // ------------------------------------------------------------------ the sweep kernel
 // Two chains per CU: 2 x 5 waves over 4 SIMDs needs 3 waves per SIMD, i.e. at most 168 VGPRs. PB: probit (latent
 // redraw after each sweep, sigma fixed) -- a separate instantiation so that the wbart kernel carries none of its code.
@@ -1055,6 +1109,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
     Rng g{ufl(cs->rng[0]), ufl(cs->rng[1]), ufl(cs->rng[2]), ufl(cs->rng[3])};
     double sigma = cs->sigma;
     int n_birth = cs->n_birth, n_death = cs->n_death, n_cap = cs->n_cap;
+    uint32_t xseq = ufl(cs->xseq);
+    int xerr = (int)ufl((uint32_t)cs->xerr);
+    const bool split = FX && !PB && c.cd.split > 1;
     for (int it = a.it0; it < a.it1; it++) {
       const bool keep = it >= a.nskip;
       if (lane == 0) {
@@ -1071,7 +1128,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         // the previous update's tree (buffer cb ^ 1) is stored before that buffer receives tree j+1
         if (j > 0) {
           const TreeDev& tp = c.tree[cb ^ 1];
-          if (keep && a.keep_trees)
+          if (keep && a.keep_trees && c.cd.keep)
             a.trees_out[c.cd.tout_off + ((size_t)(it - a.nskip) * m + (j - 1)) * kNodes + lane] = pack_word(tp, lane);
           copy_tree(&TR[j - 1], &tp, lane);
         }
@@ -1118,6 +1175,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         if (lane == 0) c.lu = luv;
         __syncthreads();
         // ---- lane sums, Metropolis-Hastings (:469-518), leaf values (:520-528)
+        if (split) row_split_exchange(c, a, cb, false, xseq, xerr, lane);
         const int cmode = c.cmode[cb];
         if (FX) ctl_combine<true>(c, cb, false, cmode, lane);
         else ctl_combine<false>(c, cb, false, cmode, lane);
@@ -1249,7 +1307,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       }
       // closing pass (:536-541) on the data waves; meanwhile the last tree (buffer cb ^ 1) is stored
       const TreeDev& tp = c.tree[cb ^ 1];
-      if (keep && a.keep_trees)
+      if (keep && a.keep_trees && c.cd.keep)
         a.trees_out[c.cd.tout_off + ((size_t)(it - a.nskip) * m + (m - 1)) * kNodes + lane] = pack_word(tp, lane);
       copy_tree(&TR[m - 1], &tp, lane);
       __syncthreads();
@@ -1262,10 +1320,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
           c.pst_[3] = g.s3;
         }
       } else {
+        if (split) row_split_exchange(c, a, cb, true, xseq, xerr, lane);
         if (FX) ctl_combine<true>(c, cb, true, 0, lane);
         else ctl_combine<false>(c, cb, true, 0, lane);
         const float nul = c.cd.nu * c.cd.lambda;
-        const float nnu = (float)n + c.cd.nu;
+        const float nnu = (float)c.cd.n_total + c.cd.nu;
         sigma = __builtin_sqrt(((double)nul + c.rss) / (2.0 * rgamma_dev(g, (double)nnu / 2.0, tb)));
       }
       if (lane == 0) a.sigma_out[c.cd.sig_off + it] = (float)sigma;
@@ -1289,6 +1348,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       cs->n_birth = n_birth;
       cs->n_death = n_death;
       cs->n_cap = n_cap;
+      cs->xseq = xseq;
+      cs->xerr = xerr;
     }
   } else {
     TileIn<R, !PB, !RES> T;
@@ -1358,6 +1419,8 @@ __global__ __launch_bounds__(256) void k_init(KArgs a, const unsigned* seeds) {
     cs->sigma = (double)cd.sigest;
     cs->it = 0;
     cs->n_birth = cs->n_death = cs->n_cap = 0;
+    cs->xseq = 0;
+    cs->xerr = 0;
   }
 }
 
@@ -1960,7 +2023,7 @@ __global__ __launch_bounds__(kBlock) void k_lp_close(LpArgs A) {
     ctl_combine<FX>(c, 0, true, 0, lane);
     Rng g{ufl(S->rng[0]), ufl(S->rng[1]), ufl(S->rng[2]), ufl(S->rng[3])};
     const float nul = c.cd.nu * c.cd.lambda;
-    const float nnu = (float)n + c.cd.nu;
+    const float nnu = (float)c.cd.n_total + c.cd.nu;
     const double sigma = __builtin_sqrt(((double)nul + c.rss) / (2.0 * rgamma_dev(g, (double)nnu / 2.0, a.tb)));
     if (lane == 0) {
       a.sigma_out[c.cd.sig_off + it] = (float)sigma;
@@ -2050,7 +2113,13 @@ using namespace fbh;
 struct fbh_batch {
   int device = 0;
   hipStream_t stream = nullptr;
-  int C = 0, m = 0, nskip = 0, ndpost = 0, probit = 0, keep_trees = 0, stats_mode = 0, node_cap = kNodes;
+  // C device chains (workgroups); CL logical chains, each W consecutive members (W = 1 without a row split);
+  // logical chain c has rows [row0[c*W + k], row0[c*W + k + 1]) in member k
+  int C = 0, CL = 0, W = 1;
+  std::vector<int> row0;
+  unsigned long long* d_xbuf = nullptr;
+  uint32_t* d_xctr = nullptr;
+  int m = 0, nskip = 0, ndpost = 0, probit = 0, keep_trees = 0, stats_mode = 0, node_cap = kNodes;
   int sweeps_per_launch = 25;
   int it = 0;
   std::vector<ChainDesc> hdesc;
@@ -2102,7 +2171,7 @@ static void free_batch(fbh_batch* b) {
   void* ptrs[] = {b->d_desc, b->d_state, b->d_trees, b->d_q,   b->d_Y,   b->d_W,   b->d_yin,    b->d_AF,
                   b->d_TM,   b->d_ybin,  b->d_leaf,  b->d_sigma, b->d_tout, b->d_PG, b->d_trm, b->d_trace,
                   b->d_img,  b->d_lps,   b->d_afx,   b->d_X,     b->d_rg,   b->d_px, b->d_prg, b->d_ptg,
-                  b->d_pcount, b->d_idx};
+                  b->d_pcount, b->d_idx, b->d_xbuf, b->d_xctr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto e : b->ev) (void)hipEventDestroy(e);
@@ -2150,6 +2219,8 @@ static KArgs make_args(const fbh_batch* b) {
   a.node_cap = b->node_cap;
   a.it0 = a.it1 = 0;
   a.af_off = b->af_off;
+  a.xbuf = b->d_xbuf;
+  a.xctr = b->d_xctr;
   return a;
 }
 
@@ -2168,6 +2239,17 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
                           "block_threads or tile_rows)");
     return FBH_EINVAL;
   }
+  const int W = o->row_split > 1 ? o->row_split : 1;
+  if (W > 1 && (o->stats_mode != FBH_STATS_FX64 || o->probit || W > 64)) {
+    set_error(FBH_EINVAL, "fbh_batch_create: row_split needs FX64 statistics and wbart chains (and at most 64)");
+    return FBH_EINVAL;
+  }
+  for (int c = 0; W > 1 && c < nchains; c++)
+    if (chains[c].n < 64 * W) {
+      set_error(FBH_EINVAL, "fbh_batch_create: row_split " + std::to_string(W) + " needs n >= " +
+                                std::to_string(64 * W) + " in every chain");
+      return FBH_EINVAL;
+    }
   for (int c = 0; c < nchains; c++) {
     const fbh_chain_in& ci = chains[c];
     if (ci.n < 1 || ci.n > FBH_NMAX || ci.p < 1 || ci.p > FBH_PMAX || !ci.q || !ci.wt || !ci.numcut ||
@@ -2191,7 +2273,9 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
   if (rc) return rc;
   fbh_batch* b = new fbh_batch();
   b->device = o->device;
-  b->C = nchains;
+  b->CL = nchains;
+  b->W = W;
+  b->C = nchains * W;
   b->m = o->m;
   b->nskip = o->nskip;
   b->ndpost = o->ndpost;
@@ -2209,22 +2293,35 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     free_batch(b);
     return FBH_EHIP;
   }
-  // ---- pools
+  // ---- pools: one device chain per (logical chain, member); a member's rows are a contiguous range of the chain's
+  const int nchains_l = nchains;
+  nchains = b->C;
   b->hdesc.resize(nchains);
+  b->row0.assign((size_t)nchains + 1, 0);
   size_t qo = 0, ro = 0, lo = 0;
   const int total = b->nskip + b->ndpost;
-  for (int c = 0; c < nchains; c++) {
+  for (int dc = 0; dc < nchains; dc++) {
+    const int c = dc / W, k = dc % W;
     const fbh_chain_in& ci = chains[c];
-    ChainDesc& d = b->hdesc[c];
+    const int share = (ci.n + W - 1) / W;
+    const int r0 = std::min(ci.n, k * share), r1 = std::min(ci.n, (k + 1) * share);
+    b->row0[dc] = r0;
+    if (k == W - 1) b->row0[dc + 1] = ci.n;
+    ChainDesc& d = b->hdesc[dc];
     std::memset(&d, 0, sizeof d);
-    d.n = ci.n;
+    d.n = r1 - r0;
+    d.n_total = ci.n;
+    d.split = W;
+    d.member = k;
+    d.group = c;
+    d.keep = k == 0;
     d.p = ci.p;
     d.tau = ci.tau;
     d.lambda = ci.lambda;
     d.nu = ci.nu;
     d.sigest = ci.sigest;
     d.offset = ci.offset;
-    b->nmax = std::max(b->nmax, ci.n);
+    b->nmax = std::max(b->nmax, d.n);
     int rn = 0;
     for (int v = 0; v < FBH_PMAX; v++) {
       d.nc[v] = v < ci.p ? ci.numcut[v] : 0;
@@ -2234,13 +2331,13 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     d.q_off = qo;
     d.row_off = ro;
     d.leaf_off = lo;
-    d.tree_off = (uint64_t)c * b->m;
-    d.sig_off = (uint64_t)c * total;
-    d.tout_off = (uint64_t)c * b->ndpost * b->m * kNodes;
-    qo += (size_t)ci.p * ci.n;
-    ro += ((size_t)ci.n + 63) / 64 * 64;  // keep each chain's rows 512-byte aligned
-    lo += ((size_t)b->m * ci.n + 255) / 256 * 256;
-    b->bytes_per_sweep += 37.0 * ci.n * (b->m + 1);
+    d.tree_off = (uint64_t)dc * b->m;
+    d.sig_off = (uint64_t)dc * total;
+    d.tout_off = (uint64_t)c * b->ndpost * b->m * kNodes;  // only member 0 writes the kept trees
+    qo += (size_t)ci.p * d.n;
+    ro += ((size_t)d.n + 63) / 64 * 64;  // keep each chain's rows 512-byte aligned
+    lo += ((size_t)b->m * d.n + 255) / 256 * 256;
+    b->bytes_per_sweep += 37.0 * d.n * (b->m + 1);
   }
   b->q_total = qo;
   b->rows_total = ro;
@@ -2274,7 +2371,26 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
       return FBH_EHIP;
     }
   }
-  const size_t tout = b->keep_trees ? (size_t)nchains * b->ndpost * b->m * kNodes : 1;
+  const size_t tout = b->keep_trees ? (size_t)nchains_l * b->ndpost * b->m * kNodes : 1;
+  if (W > 1) {
+    // every member must be resident at once: the exchange waits for all of them
+    int per_cu = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(b->var.fn), kBlock,
+                                                     b->dyn_bytes) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess ||
+        (long)per_cu * ncu < (long)nchains) {
+      set_error(FBH_EINVAL, "fbh_batch_create: " + std::to_string(nchains) + " workgroups of a row split do not fit "
+                                "on the device at once (" + std::to_string(per_cu) + " per CU x " +
+                                std::to_string(ncu) + " CUs)");
+      free_batch(b);
+      return FBH_EINVAL;
+    }
+    if ((rc = dalloc(&b->d_xbuf, (size_t)nchains_l * 2 * W * kXw)) || (rc = dalloc(&b->d_xctr, (size_t)nchains_l)) ||
+        hipMemset(b->d_xctr, 0, sizeof(uint32_t) * nchains_l) != hipSuccess) {
+      free_batch(b);
+      return rc ? rc : FBH_EHIP;
+    }
+  }
   if ((rc = dalloc(&b->d_desc, nchains)) || (rc = dalloc(&b->d_state, nchains)) ||
       (rc = dalloc(&b->d_trees, (size_t)nchains * b->m)) || (rc = dalloc(&b->d_q, qo)) ||
       (rc = dalloc(&b->d_Y, ro)) || (rc = dalloc(&b->d_W, ro)) || (rc = dalloc(&b->d_yin, ro)) ||
@@ -2288,19 +2404,21 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
   {
     std::vector<uint8_t> hq(qo), hyb(ro, 0);
     std::vector<float> hw(ro, 0.f), hy(ro, 0.f);
-    for (int c = 0; c < nchains; c++) {
-      const fbh_chain_in& ci = chains[c];
-      const ChainDesc& d = b->hdesc[c];
-      std::memcpy(hq.data() + d.q_off, ci.q, (size_t)ci.p * ci.n);
-      std::memcpy(hw.data() + d.row_off, ci.wt, sizeof(float) * ci.n);
-      if (!o->probit) std::memcpy(hy.data() + d.row_off, ci.y, sizeof(float) * ci.n);
+    for (int dc = 0; dc < nchains; dc++) {
+      const fbh_chain_in& ci = chains[dc / W];
+      const ChainDesc& d = b->hdesc[dc];
+      const int r0 = b->row0[dc];
+      for (int v = 0; v < ci.p; v++)
+        std::memcpy(hq.data() + d.q_off + (size_t)v * d.n, ci.q + (size_t)v * ci.n + r0, (size_t)d.n);
+      std::memcpy(hw.data() + d.row_off, ci.wt + r0, sizeof(float) * d.n);
+      if (!o->probit) std::memcpy(hy.data() + d.row_off, ci.y + r0, sizeof(float) * d.n);
       if (o->probit)
-        for (int i = 0; i < ci.n; i++) hyb[d.row_off + i] = ci.ybin[i] != 0;
+        for (int i = 0; i < d.n; i++) hyb[d.row_off + i] = ci.ybin[r0 + i] != 0;
     }
     std::vector<double> pg(kNodes);
     for (int d = 0; d < kNodes; d++) pg[d] = (double)o->pgtab[d];
     std::vector<unsigned> seeds(nchains);
-    for (int c = 0; c < nchains; c++) seeds[c] = chains[c].seed;
+    for (int dc = 0; dc < nchains; dc++) seeds[dc] = chains[dc / W].seed;  // members share the generator
     unsigned* d_seeds = nullptr;
     if (hipMalloc(&d_seeds, sizeof(unsigned) * nchains) != hipSuccess) {
       set_error(FBH_ENOMEM, "hipMalloc failed");
@@ -2364,38 +2482,59 @@ extern "C" int fbh_batch_run(fbh_batch* b, int nsweeps) {
   return FBH_OK;
 }
 
+// a row split whose exchange timed out (members not resident together) has invalid results: report it
+static int check_split(fbh_batch* b) {
+  if (b->W <= 1) return FBH_OK;
+  std::vector<ChainState> cs(b->C);
+  FBH_HIP_CHECK(hipMemcpy(cs.data(), b->d_state, sizeof(ChainState) * b->C, hipMemcpyDeviceToHost));
+  for (int dc = 0; dc < b->C; dc++)
+    if (cs[dc].xerr) {
+      set_error(FBH_EHIP, "fbh_batch: a row-split exchange timed out (chain " + std::to_string(dc / b->W) +
+                              "); the workgroups of a chain were not resident together");
+      return FBH_EHIP;
+    }
+  return FBH_OK;
+}
+
 extern "C" int fbh_batch_sync(fbh_batch* b) {
   if (!b) return FBH_EINVAL;
   FBH_HIP_CHECK(hipSetDevice(b->device));
   FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
-  return FBH_OK;
+  return check_split(b);
 }
 
 extern "C" int fbh_batch_iter(const fbh_batch* b) { return b ? b->it : -1; }
 
 extern "C" int fbh_batch_fetch(fbh_batch* b, int c, const fbh_chain_out* out) {
-  if (!b || !out || c < 0 || c >= b->C) {
+  if (!b || !out || c < 0 || c >= b->CL) {
     set_error(FBH_EINVAL, "fbh_batch_fetch: bad argument");
     return FBH_EINVAL;
   }
   FBH_HIP_CHECK(hipSetDevice(b->device));
   FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
-  const ChainDesc& d = b->hdesc[c];
+  int rc = check_split(b);
+  if (rc) return rc;
+  const int d0 = c * b->W;  // member 0 holds the chain-level outputs (every member computes the same ones)
+  const ChainDesc& d = b->hdesc[d0];
   const int total = b->nskip + b->ndpost;
   if (out->sigma) FBH_HIP_CHECK(hipMemcpy(out->sigma, b->d_sigma + d.sig_off, sizeof(float) * total, hipMemcpyDeviceToHost));
   if (out->trmean) {
-    hipLaunchKernelGGL(k_trmean, dim3((d.n + 255) / 256), dim3(256), 0, b->stream, b->d_TM + d.row_off,
-                       b->d_trm + d.row_off, d.n, b->ndpost);
-    FBH_HIP_CHECK(hipGetLastError());
-    FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
-    FBH_HIP_CHECK(hipMemcpy(out->trmean, b->d_trm + d.row_off, sizeof(float) * d.n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < b->W; k++) {
+      const ChainDesc& dk = b->hdesc[d0 + k];
+      hipLaunchKernelGGL(k_trmean, dim3((dk.n + 255) / 256), dim3(256), 0, b->stream, b->d_TM + dk.row_off,
+                         b->d_trm + dk.row_off, dk.n, b->ndpost);
+      FBH_HIP_CHECK(hipGetLastError());
+      FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
+      FBH_HIP_CHECK(hipMemcpy(out->trmean + b->row0[d0 + k], b->d_trm + dk.row_off, sizeof(float) * dk.n,
+                              hipMemcpyDeviceToHost));
+    }
   }
   if (out->trees && b->keep_trees)
     FBH_HIP_CHECK(hipMemcpy(out->trees, b->d_tout + d.tout_off, sizeof(uint64_t) * b->ndpost * b->m * kNodes,
                             hipMemcpyDeviceToHost));
   if (out->stats) {
     ChainState cs;
-    FBH_HIP_CHECK(hipMemcpy(&cs, b->d_state + c, sizeof cs, hipMemcpyDeviceToHost));
+    FBH_HIP_CHECK(hipMemcpy(&cs, b->d_state + d0, sizeof cs, hipMemcpyDeviceToHost));
     out->stats[0] = cs.n_birth;
     out->stats[1] = cs.n_death;
     out->stats[2] = cs.n_cap;
@@ -2412,20 +2551,30 @@ static uint64_t pack_host(const TreeDev& t, int s) {
 
 extern "C" int fbh_batch_state(fbh_batch* b, int c, uint32_t* rng, double* sigma, double* af, double* latent,
                                uint8_t* leaf, uint64_t* words) {
-  if (!b || c < 0 || c >= b->C) {
+  if (!b || c < 0 || c >= b->CL) {
     set_error(FBH_EINVAL, "fbh_batch_state: bad argument");
     return FBH_EINVAL;
   }
   FBH_HIP_CHECK(hipSetDevice(b->device));
   FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
-  const ChainDesc& d = b->hdesc[c];
+  const int d0 = c * b->W;
+  const ChainDesc& d = b->hdesc[d0];
+  const int n_all = d.n_total;
   ChainState cs;
-  FBH_HIP_CHECK(hipMemcpy(&cs, b->d_state + c, sizeof cs, hipMemcpyDeviceToHost));
+  FBH_HIP_CHECK(hipMemcpy(&cs, b->d_state + d0, sizeof cs, hipMemcpyDeviceToHost));
   if (rng) std::memcpy(rng, cs.rng, 16);
   if (sigma) *sigma = cs.sigma;
-  if (af) FBH_HIP_CHECK(hipMemcpy(af, b->d_AF + d.row_off, sizeof(double) * d.n, hipMemcpyDeviceToHost));
-  if (latent) FBH_HIP_CHECK(hipMemcpy(latent, b->d_Y + d.row_off, sizeof(double) * d.n, hipMemcpyDeviceToHost));
-  if (leaf) FBH_HIP_CHECK(hipMemcpy(leaf, b->d_leaf + d.leaf_off, (size_t)b->m * d.n, hipMemcpyDeviceToHost));
+  for (int k = 0; k < b->W; k++) {
+    const ChainDesc& dk = b->hdesc[d0 + k];
+    const int r0 = b->row0[d0 + k];
+    if (af) FBH_HIP_CHECK(hipMemcpy(af + r0, b->d_AF + dk.row_off, sizeof(double) * dk.n, hipMemcpyDeviceToHost));
+    if (latent)
+      FBH_HIP_CHECK(hipMemcpy(latent + r0, b->d_Y + dk.row_off, sizeof(double) * dk.n, hipMemcpyDeviceToHost));
+    if (leaf)
+      for (int j = 0; j < b->m; j++)
+        FBH_HIP_CHECK(hipMemcpy(leaf + (size_t)j * n_all + r0, b->d_leaf + dk.leaf_off + (size_t)j * dk.n, dk.n,
+                                hipMemcpyDeviceToHost));
+  }
   if (words) {
     std::vector<TreeDev> tr(b->m);
     FBH_HIP_CHECK(hipMemcpy(tr.data(), b->d_trees + d.tree_off, sizeof(TreeDev) * b->m, hipMemcpyDeviceToHost));
@@ -2463,9 +2612,10 @@ extern "C" int fbh_batch_trace(fbh_batch* b, int cap, void* host_out, int chain)
     return FBH_OK;
   }
   if (host_out && b->d_trace) {
+    if (chain < 0 || chain >= b->CL) return FBH_EINVAL;
     FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
-    FBH_HIP_CHECK(hipMemcpy(host_out, b->d_trace + (size_t)chain * b->trace_cap, sizeof(TraceRec) * b->trace_cap,
-                            hipMemcpyDeviceToHost));
+    FBH_HIP_CHECK(hipMemcpy(host_out, b->d_trace + (size_t)chain * b->W * b->trace_cap,
+                            sizeof(TraceRec) * b->trace_cap, hipMemcpyDeviceToHost));
   }
   return FBH_OK;
 }
@@ -2479,14 +2629,29 @@ int yhat_device(const uint64_t* d_trees, const uint8_t* d_q, int n, int ndpost, 
 }
 
 extern "C" int fbh_batch_yhat(fbh_batch* b, int c, double* yhat) {
-  if (!b || c < 0 || c >= b->C || !yhat || !b->keep_trees) {
+  if (!b || c < 0 || c >= b->CL || !yhat || !b->keep_trees) {
     set_error(FBH_EINVAL, "fbh_batch_yhat: bad argument or trees not kept");
     return FBH_EINVAL;
   }
   FBH_HIP_CHECK(hipSetDevice(b->device));
   FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
-  const ChainDesc& d = b->hdesc[c];
-  return fbh::yhat_device(b->d_tout + d.tout_off, b->d_q + d.q_off, d.n, b->ndpost, b->m, b->stream, yhat);
+  const int d0 = c * b->W;
+  const ChainDesc& d = b->hdesc[d0];
+  if (b->W == 1)
+    return fbh::yhat_device(b->d_tout + d.tout_off, b->d_q + d.q_off, d.n, b->ndpost, b->m, b->stream, yhat);
+  // a row split: each member's rows against the chain's kept trees, placed at the member's row offset
+  const int n_all = d.n_total;
+  std::vector<double> part;
+  for (int k = 0; k < b->W; k++) {
+    const ChainDesc& dk = b->hdesc[d0 + k];
+    part.resize((size_t)b->ndpost * dk.n);
+    const int rc = fbh::yhat_device(b->d_tout + d.tout_off, b->d_q + dk.q_off, dk.n, b->ndpost, b->m, b->stream,
+                                    part.data());
+    if (rc) return rc;
+    for (int dr = 0; dr < b->ndpost; dr++)
+      std::memcpy(yhat + (size_t)dr * n_all + b->row0[d0 + k], part.data() + (size_t)dr * dk.n, sizeof(double) * dk.n);
+  }
+  return FBH_OK;
 }
 
 extern "C" int fbh_fit_batch(const fbh_chain_in* chains, int nchains, const fbh_opts* opts, const fbh_chain_out* outs) {
@@ -2593,8 +2758,8 @@ extern "C" int fbh_batch_learned_setup(fbh_batch* b, fbh_mlp* net, float alpha, 
                           "collecting pairs)");
     return FBH_EINVAL;
   }
-  if (b->probit) {
-    set_error(FBH_EINVAL, "fbh_batch_learned_setup: the learned proposal runs wbart chains only");
+  if (b->probit || b->W > 1) {
+    set_error(FBH_EINVAL, "fbh_batch_learned_setup: the learned proposal runs wbart chains without a row split only");
     return FBH_EINVAL;
   }
   if (net) {

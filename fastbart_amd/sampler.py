@@ -39,7 +39,7 @@ class BatchSampler:
     """Many independent chains sampled together on one GPU (fbh_batch_*)."""
 
     def __init__(self, problems, m, nskip, ndpost, seeds, probit=0, keep_trees=1, stats_mode=0, node_cap=0,
-                 device=0, sweeps_per_launch=0, pgtab=None, tile_rows=0):
+                 device=0, sweeps_per_launch=0, pgtab=None, tile_rows=0, row_split=0):
         L = lib()
         self.problems = problems
         self.C = len(problems)
@@ -52,7 +52,7 @@ class BatchSampler:
         pg = np.ascontiguousarray(pgtab if pgtab is not None else problems[0]["pgtab"], dtype=np.float32)
         self._keep.append(pg)
         o = Opts(m, nskip, ndpost, probit, keep_trees, ptr(pg), stats_mode, node_cap, device, sweeps_per_launch, 0,
-                 tile_rows)
+                 tile_rows, row_split)
         h = C.c_void_p()
         check(L.fbh_batch_create(arr, self.C, C.byref(o), C.byref(h)), "fbh_batch_create")
         self.h = h

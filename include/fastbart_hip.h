@@ -82,6 +82,11 @@ typedef struct fbh_opts {
   int block_threads;        /* 0 or 320: one control wave + four data waves per chain (the only shape built) */
   int tile_rows;            /* 0 = automatic; 1024 or 1536 rows staged in LDS per tile (1536 needs the running fit
                                resident in LDS, 1024 is the only size when it streams) */
+  int row_split;            /* 0 or 1 = one workgroup per chain. W > 1 (FX64 statistics, wbart only): the rows of
+                               every chain are split over W workgroups that run the same control steps and add their
+                               fixed-point pass totals after every pass -- exact, as fixed-point sums are order-free,
+                               so the results equal the one-workgroup FX64 chain bit for bit. Every chain needs
+                               n >= 64 W, and all C x W workgroups must fit on the device at once (FBH_EINVAL if not). */
 } fbh_opts;
 
 typedef struct fbh_chain_out {

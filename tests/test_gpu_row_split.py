@@ -1,0 +1,82 @@
+"""Row split (one chain over several workgroups, FX64 statistics): every member runs the same control steps and the
+members add their fixed-point pass totals after each pass. Fixed-point sums are order-free, so a split chain must equal
+the one-workgroup FX64 chain and the oracle's FX64 twin bit for bit: trees, sigma trace, trmean, move counts, per-draw
+training fits and the final chain state."""
+import numpy as np
+import pytest
+
+from conftest import same_bits
+from fastbart_amd import sampler
+from fastbart_amd import workloads as W
+from fastbart_amd._lib import FbhError
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[row split]", *a, flush=True)
+
+
+@pytest.mark.parametrize("split", [2, 3, 8])
+def test_gpu_row_split_matches_oracle_fx64(oracle, split):
+    probs = [W.small_problem(n, 4, 25, 61 + n, "uniform", ntree=20) for n in (1000, 1555, 2999)]
+    m, nskip, nd = 20, 12, 6
+    _say("split", split)
+    with sampler.BatchSampler(probs, m, nskip, nd, [3, 4, 5], stats_mode=1, row_split=split,
+                              sweeps_per_launch=7) as b:
+        b.run(nskip + nd)
+        outs = [b.fetch(c) for c in range(3)]
+        yh = [b.yhat(c) for c in range(3)]
+    with sampler.BatchSampler(probs, m, nskip, nd, [3, 4, 5], stats_mode=1) as b:
+        b.run(nskip + nd)
+        ref = [b.fetch(c) for c in range(3)]
+        yref = [b.yhat(c) for c in range(3)]
+    for c, pb in enumerate(probs):
+        e = oracle.fit(pb, m, nskip, nd, 3 + c, stats_mode=1)
+        for k in ("sigma", "trmean", "trees", "stats"):
+            assert same_bits(outs[c][k], e[k]), (c, k)
+            assert same_bits(outs[c][k], ref[c][k]), (c, k)
+        assert same_bits(yh[c], yref[c]), c
+
+
+def test_gpu_row_split_state_across_launches():
+    """State after partial runs (launch boundaries inside the fit) equals the unsplit chain's: generator, sigma, running
+    fit, leaf ids and tree words."""
+    probs = [W.small_problem(n, 5, 30, 71 + n, "extreme", ntree=15) for n in (777, 2048)]
+    m, nskip, nd = 15, 8, 4
+    _say("state across launches")
+    with sampler.BatchSampler(probs, m, nskip, nd, [9, 10], stats_mode=1, row_split=4, sweeps_per_launch=3) as a, \
+            sampler.BatchSampler(probs, m, nskip, nd, [9, 10], stats_mode=1, sweeps_per_launch=3) as b:
+        for _ in range(4):
+            a.run(3)
+            b.run(3)
+            for c in range(2):
+                sa, sb = a.state(c), b.state(c)
+                for k in ("rng", "af", "leaf", "trees"):
+                    assert same_bits(sa[k], sb[k]), k
+                assert sa["sigma"] == sb["sigma"]
+
+
+def test_gpu_row_split_maxshape_chain():
+    """A chain of the max-shape workload (n = 65,536, p = 32, node cap 64) split over 8 workgroups, a few sweeps,
+    against the same chain on one workgroup."""
+    pb = W.maxshape64_chain(1)
+    m, nskip, nd = 200, 2, 1
+    _say("max-shape chain split 8")
+    outs = []
+    for split in (8, 0):
+        with sampler.BatchSampler([pb], m, nskip, nd, [2], stats_mode=1, node_cap=64, row_split=split) as b:
+            b.run(nskip + nd)
+            outs.append(b.fetch(0))
+    for k in ("sigma", "trmean", "trees", "stats"):
+        assert same_bits(outs[0][k], outs[1][k]), k
+
+
+def test_gpu_row_split_rejects_what_it_cannot_run():
+    pb = W.small_problem(1000, 4, 25, 5, "uniform", ntree=10)
+    with pytest.raises(FbhError):                       # REF16 order needs one workgroup per lane sequence
+        sampler.BatchSampler([pb], 10, 2, 2, [1], stats_mode=0, row_split=2)
+    with pytest.raises(FbhError):                       # fewer than 64 rows per member
+        sampler.BatchSampler([pb], 10, 2, 2, [1], stats_mode=1, row_split=16)
+    with pytest.raises(FbhError):                       # more workgroups than fit on the device at once
+        sampler.BatchSampler([pb] * 1100, 10, 2, 2, list(range(1100)), stats_mode=1, row_split=2)

@@ -423,7 +423,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ess-rows", type=int, default=0, help="rows per chain used for ESS (0 = all)")
     ap.add_argument("--row-split", type=int, default=None,
-                    help="workgroups per chain (FX64 workloads; default: 8 for maxshape64, else 1)")
+                    help="workgroups per chain for FX64 workloads (-1 = one workgroup per CU; default: -1 for "
+                         "maxshape64, else 1)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration under rocprofv3)")
     ap.add_argument("--ndpost", type=int, default=None, help="kept sweeps (learned512; longer runs steady ESS)")
@@ -470,9 +471,10 @@ def main() -> int:
     seeds = [1 + c for c in range(chains)]
     rows_total = sum(p["q"].shape[1] for p in probs)
     log(f"{wl}: generated {chains} chains in {time.perf_counter() - t0:.1f} s; total rows {rows_total}")
-    split = args.row_split if args.row_split is not None else (8 if wl == "maxshape64" else 1)
+    split = args.row_split if args.row_split is not None else (-1 if wl == "maxshape64" else 1)
     b = sampler.BatchSampler(probs, M, nskip, ndpost, seeds, probit=probit, keep_trees=1, stats_mode=stats_mode,
                              node_cap=node_cap, sweeps_per_launch=spl, row_split=split)
+    split = b.row_split
     b.sync()
     t_w = time.perf_counter()
     if W_:

@@ -54,7 +54,7 @@ EXPORTS = ["bart_fit_cpu", "bart_wfit", "bart_set_rng", "fbh_last_status", "fbh_
            "fbh_compile_trees", "fbh_trees_info", "fbh_trees_arrays", "fbh_trees_free", "fbh_predict_draw",
            "fbh_predict_draws", "fbh_rows_upload", "fbh_rows_free", "fbh_predict_rows", "fbh_predict_timing",
            "fbh_predict_rows_device", "fbh_trees_rank_form", "fbh_device_alloc", "fbh_device_free",
-           "fbh_device_copy", "fbh_device_sync",
+           "fbh_device_copy", "fbh_device_sync", "fbh_batch_row_split",
            "fbh_debug_math", "fbh_debug_libm_host", "fbh_debug_copy", "fbh_batch_trace", "fbh_version",
            "fbh_mlp_create", "fbh_mlp_nparams", "fbh_mlp_params", "fbh_mlp_forward", "fbh_mlp_grad", "fbh_mlp_train",
            "fbh_mlp_timing", "fbh_mlp_destroy", "fbh_batch_learned_setup", "fbh_batch_run_learned",
@@ -109,6 +109,7 @@ def _declare(L):
     L.fbh_device_free.restype = None
     L.fbh_device_copy.argtypes = [vp, vp, C.c_size_t, i, i]
     L.fbh_device_sync.argtypes = [i]
+    L.fbh_batch_row_split.argtypes = [vp]
     L.fbh_debug_math.argtypes = [i, vp, vp, C.c_long, vp, vp, i]
     L.fbh_debug_libm_host.argtypes = [i, C.c_long, C.c_long, vp]
     L.fbh_debug_copy.argtypes = [C.c_long, i, C.POINTER(d)]

@@ -2239,7 +2239,14 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
                           "block_threads or tile_rows)");
     return FBH_EINVAL;
   }
-  const int W = o->row_split > 1 ? o->row_split : 1;
+  int W = o->row_split > 1 ? o->row_split : 1;
+  if (o->row_split < 0 && o->stats_mode == FBH_STATS_FX64 && !o->probit) {
+    // automatic: as many workgroups per chain as leave one workgroup per CU, within 64 rows per member
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, o->device) != hipSuccess) ncu = 0;
+    W = std::max(1, std::min(64, ncu / nchains));
+    for (int c = 0; c < nchains; c++) W = std::max(1, std::min(W, chains[c].n / 64));
+  }
   if (W > 1 && (o->stats_mode != FBH_STATS_FX64 || o->probit || W > 64)) {
     set_error(FBH_EINVAL, "fbh_batch_create: row_split needs FX64 statistics and wbart chains (and at most 64)");
     return FBH_EINVAL;
@@ -2504,6 +2511,8 @@ extern "C" int fbh_batch_sync(fbh_batch* b) {
 }
 
 extern "C" int fbh_batch_iter(const fbh_batch* b) { return b ? b->it : -1; }
+
+extern "C" int fbh_batch_row_split(const fbh_batch* b) { return b ? b->W : -1; }
 
 extern "C" int fbh_batch_fetch(fbh_batch* b, int c, const fbh_chain_out* out) {
   if (!b || !out || c < 0 || c >= b->CL) {

@@ -59,6 +59,11 @@ class BatchSampler:
         self.n = [int(pb["q"].shape[1]) for pb in problems]
         self.p = [int(pb["q"].shape[0]) for pb in problems]
 
+    @property
+    def row_split(self):
+        """Workgroups per chain (1 without a row split)."""
+        return int(lib().fbh_batch_row_split(self.h))
+
     def run(self, nsweeps):
         check(lib().fbh_batch_run(self.h, int(nsweeps)), "fbh_batch_run")
 

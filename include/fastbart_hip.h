@@ -86,7 +86,9 @@ typedef struct fbh_opts {
                                every chain are split over W workgroups that run the same control steps and add their
                                fixed-point pass totals after every pass -- exact, as fixed-point sums are order-free,
                                so the results equal the one-workgroup FX64 chain bit for bit. Every chain needs
-                               n >= 64 W, and all C x W workgroups must fit on the device at once (FBH_EINVAL if not). */
+                               n >= 64 W, and all C x W workgroups must fit on the device at once (FBH_EINVAL if not).
+                               -1 = automatic for FX64 wbart batches: the largest W with C x W <= the CU count and
+                               n >= 64 W (1 otherwise). */
 } fbh_opts;
 
 typedef struct fbh_chain_out {
@@ -102,9 +104,12 @@ typedef struct fbh_batch fbh_batch;
 int fbh_batch_create(const fbh_chain_in* chains, int nchains, const fbh_opts* opts, fbh_batch** out);
 /* Enqueue the next nsweeps sweeps of every chain on the batch's stream (asynchronous). */
 int fbh_batch_run(fbh_batch* b, int nsweeps);
+/* Wait for the batch's work; FBH_EHIP if a row-split exchange timed out (its results are invalid). */
 int fbh_batch_sync(fbh_batch* b);
 /* sweeps done so far */
 int fbh_batch_iter(const fbh_batch* b);
+/* workgroups per chain (the row split in effect; 1 without one) */
+int fbh_batch_row_split(const fbh_batch* b);
 /* Copy chain c's outputs to host buffers (synchronises). */
 int fbh_batch_fetch(fbh_batch* b, int chain, const fbh_chain_out* out);
 /* Copy chain c's sampler state (parity checks, checkpoints): rng[4], sigma, af[n], latent[n], leaf[m][n],

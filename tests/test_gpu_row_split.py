@@ -80,3 +80,22 @@ def test_gpu_row_split_rejects_what_it_cannot_run():
         sampler.BatchSampler([pb], 10, 2, 2, [1], stats_mode=1, row_split=16)
     with pytest.raises(FbhError):                       # more workgroups than fit on the device at once
         sampler.BatchSampler([pb] * 1100, 10, 2, 2, list(range(1100)), stats_mode=1, row_split=2)
+
+
+def test_gpu_row_split_automatic():
+    """row_split = -1: one workgroup per CU for FX64 batches (here bounded by 64 rows per member), none for REF16."""
+    probs = [W.small_problem(n, 4, 25, 31 + n, "uniform", ntree=12) for n in (1100, 1900)]
+    m, nskip, nd = 12, 6, 4
+    _say("automatic split")
+    outs = {}
+    for split in (-1, 0):
+        with sampler.BatchSampler(probs, m, nskip, nd, [5, 6], stats_mode=1, row_split=split) as b:
+            if split == -1:
+                assert b.row_split == 1100 // 64
+            b.run(nskip + nd)
+            outs[split] = [b.fetch(c) for c in range(2)]
+    for c in range(2):
+        for k in ("sigma", "trmean", "trees", "stats"):
+            assert same_bits(outs[-1][c][k], outs[0][c][k]), (c, k)
+    with sampler.BatchSampler(probs, m, nskip, nd, [5, 6], stats_mode=0, row_split=-1) as b:
+        assert b.row_split == 1

@@ -949,19 +949,32 @@ __device__ __attribute__((noinline)) void probit_latents(int n_, double off_, co
     __syncthreads();
     if (reff <= 0) break;
     if (wave == 0) {
-      // row l starts at offset r (the rejections so far) and accepts at the first set bit at or above it. Lane l packs
-      // its row's band start and mask so that the dependent scalar chain per row is a few instructions.
-      const uint32_t pk = lane < reff ? ((uint32_t)band_start(lane, rho16) << 16) | B.dmask[lane] : 0u;
+      // row l starts at offset r (the rejections so far) and accepts at the first set bit at or above it. A step
+      // resolves at once the run of rows from `done` that accept at the current offset (one ballot: their bit at r is
+      // set), then the first row that does not: its next set bit above r, which advances r. Steps = rows that reject
+      // at least once, not rows. Lane l holds its row's band start and mask; pk packs both for the readlane of row f.
+      const uint32_t bsl = lane < reff ? (uint32_t)band_start(lane, rho16) : 0u;
+      const uint32_t mkl = lane < reff ? B.dmask[lane] : 0u;
+      const uint32_t pk = (bsl << 16) | mkl;
+      const uint64_t below_reff = reff >= 64 ? ~0ull : ((1ull << reff) - 1ull);
       int r = 0, done = 0;
       uint32_t accv = 0;
 #pragma unroll 1
-      for (; done < reff; done++) {
-        const uint32_t x = __builtin_amdgcn_readlane(pk, done);
-        const uint32_t sh = (uint32_t)r - (x >> 16);
-        const uint32_t mk = sh < (uint32_t)kBand ? (x & 0xFFFFu) >> sh : 0u;
+      while (done < reff) {
+        const uint32_t sh = (uint32_t)r - bsl;
+        const bool ok = sh < (uint32_t)kBand && ((mkl >> sh) & 1u);
+        const uint64_t notok = below_reff & (~0ull << done) & ~(uint64_t)__ballot(ok);
+        const int f = notok ? (int)__builtin_ctzll(notok) : reff;
+        accv = (lane >= done && lane < f) ? (uint32_t)r : accv;
+        done = f;
+        if (done >= reff) break;
+        const uint32_t x = __builtin_amdgcn_readlane(pk, f);
+        const uint32_t shf = (uint32_t)r - (x >> 16);
+        const uint32_t mk = shf < (uint32_t)kBand ? (x & 0xFFFFu) >> shf : 0u;
         if (!mk) break;
         r += __builtin_ctz(mk);
-        accv = lane == done ? (uint32_t)r : accv;
+        accv = lane == f ? (uint32_t)r : accv;
+        done = f + 1;
       }
       if (lane < done) B.acc[lane] = (uint8_t)accv;
       int used = done + r;

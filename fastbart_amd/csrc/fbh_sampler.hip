@@ -916,14 +916,21 @@ __device__ __attribute__((noinline)) void probit_latents(int n_, double off_, co
       // decisions: data wave dw takes rows dw + 4i (i < 16), four rows per step; lane 16g + b decides offset
       // band_start(l) + b of row l = dw + 4 (4k + g)
       if (reff > 0) {
+        // each lane's own row's truncation point and rate once per window (an FP64 square root), handed to the
+        // sixteen lanes that decide that row's offsets
+        const LatentRow mine = latent_row(fv, ybv, off);
         for (int k = 0; k < kPR / (4 * kDataWaves); k++) {
           const int i = 4 * k + (lane >> 4);
           const int l = dw + kDataWaves * i;
-          const double f = __shfl(fv, i);
-          const int yb = __shfl(ybv, i);
+          LatentRow r;
+          r.tau = __shfl(mine.tau, i);
+          r.lam = __shfl(mine.lam, i);
+          r.tauf = __shfl(mine.tauf, i);
+          r.lamf = __shfl(mine.lamf, i);
+          r.mean = 0.;  // not used by the decision
+          r.pos = false;
           bool ok = false;
           if (l < reff) {
-            const LatentRow r = latent_row(f, yb, off);
             const uint32_t slot = S + (uint32_t)(l + band_start(l, rho16) + (lane & 15));
             ok = latent_accepts(r, slot_x1(B, slot), slot_x2(B, slot), tb);
           }

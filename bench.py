@@ -425,6 +425,11 @@ def main() -> int:
     ap.add_argument("--row-split", type=int, default=None,
                     help="workgroups per chain for FX64 workloads (-1 = one workgroup per CU; default: -1 for "
                          "maxshape64, else 1)")
+    ap.add_argument("--packing", default="auto",
+                    help="workgroup -> chain packing of the sweep kernel: 0 batch order, 1 serpentine, 2 largest "
+                         "first, auto = the autotuner picks packing and tile rows during the warmup")
+    ap.add_argument("--tile-rows", type=int, default=0, help="rows per LDS tile (0 automatic, 1024, 1536)")
+    ap.add_argument("--no-graph", action="store_true", help="plain stream launches instead of a captured hipGraph")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration under rocprofv3)")
     ap.add_argument("--ndpost", type=int, default=None, help="kept sweeps (learned512; longer runs steady ESS)")
@@ -465,6 +470,7 @@ def main() -> int:
         log("cpu baseline:", cpu)
 
     from fastbart_amd import sampler
+    from fastbart_amd._lib import FbhError
 
     t0 = time.perf_counter()
     probs = [make_chain(wl, c) for c in range(chains)]
@@ -472,16 +478,31 @@ def main() -> int:
     rows_total = sum(p["q"].shape[1] for p in probs)
     log(f"{wl}: generated {chains} chains in {time.perf_counter() - t0:.1f} s; total rows {rows_total}")
     split = args.row_split if args.row_split is not None else (-1 if wl == "maxshape64" else 1)
+    tune = args.packing == "auto"
     b = sampler.BatchSampler(probs, M, nskip, ndpost, seeds, probit=probit, keep_trees=1, stats_mode=stats_mode,
-                             node_cap=node_cap, sweeps_per_launch=spl, row_split=split)
+                             node_cap=node_cap, sweeps_per_launch=spl, row_split=split,
+                             packing=0 if tune else int(args.packing), tile_rows=args.tile_rows,
+                             graph=0 if args.no_graph else 1)
     split = b.row_split
     b.sync()
     t_w = time.perf_counter()
-    if W_:
-        b.run(W_)
-        b.sync()
+    tuned = None
+    if tune and W_:
+        # the autotuner's trials are real warmup sweeps (one sweep per trial launch); what is left of the warmup runs
+        # in the chosen configuration
+        try:
+            tuned = b.autotune(1)
+        except FbhError as e:
+            log("autotune skipped:", e)
+        if tuned:
+            log("autotune:", {k: tuned[k] for k in ("packing", "tile_rows", "sweeps")},
+                [(c["packing"], c["tile_rows"], round(c["ms_per_sweep"], 3)) for c in tuned["candidates"]])
+    if W_ > b.it:
+        b.run(W_ - b.it)
+    b.sync()
     t_w = time.perf_counter() - t_w
     log(f"warmup {W_} sweeps: {t_w:.2f} s")
+    packing, tile_rows, graph = b.config
     avg0, nl0, _ = b.timing()
     t1 = time.perf_counter()
     b.run(K)
@@ -523,7 +544,9 @@ def main() -> int:
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": describe(wl, chains, nskip, ndpost), "chains": chains, "ntree": M,
                    "rows_total": rows_total,
-                   "parallelism": "chains" if split <= 1 else f"chains x {split} workgroups (row split)"},
+                   "parallelism": "chains" if split <= 1 else f"chains x {split} workgroups (row split)",
+                   "packing": packing, "tile_rows": tile_rows, "graph": bool(graph),
+                   "autotuned": tuned is not None},
         "min_ess_per_sec": ess,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -33,7 +33,16 @@ class Opts(C.Structure):
     _fields_ = [("m", C.c_int), ("nskip", C.c_int), ("ndpost", C.c_int), ("probit", C.c_int),
                 ("keep_trees", C.c_int), ("pgtab", C.c_void_p), ("stats_mode", C.c_int), ("node_cap", C.c_int),
                 ("device", C.c_int), ("sweeps_per_launch", C.c_int), ("block_threads", C.c_int),
-                ("tile_rows", C.c_int), ("row_split", C.c_int)]
+                ("tile_rows", C.c_int), ("row_split", C.c_int), ("packing", C.c_int), ("graph", C.c_int)]
+
+
+TUNE_MAX = 8
+
+
+class Tune(C.Structure):
+    _fields_ = [("packing", C.c_int), ("tile_rows", C.c_int), ("ncand", C.c_int),
+                ("cand_packing", C.c_int * TUNE_MAX), ("cand_tile_rows", C.c_int * TUNE_MAX),
+                ("ms_per_sweep", C.c_double * TUNE_MAX), ("sweeps", C.c_int)]
 
 
 class ChainOut(C.Structure):
@@ -54,7 +63,9 @@ EXPORTS = ["bart_fit_cpu", "bart_wfit", "bart_set_rng", "fbh_last_status", "fbh_
            "fbh_compile_trees", "fbh_trees_info", "fbh_trees_arrays", "fbh_trees_free", "fbh_predict_draw",
            "fbh_predict_draws", "fbh_rows_upload", "fbh_rows_free", "fbh_predict_rows", "fbh_predict_timing",
            "fbh_predict_rows_device", "fbh_trees_rank_form", "fbh_device_alloc", "fbh_device_free",
-           "fbh_device_copy", "fbh_device_sync", "fbh_batch_row_split",
+           "fbh_device_copy", "fbh_device_sync", "fbh_batch_row_split", "fbh_batch_configure", "fbh_batch_config",
+           "fbh_batch_set_order",
+           "fbh_batch_autotune",
            "fbh_debug_math", "fbh_debug_libm_host", "fbh_debug_copy", "fbh_batch_trace", "fbh_version",
            "fbh_mlp_create", "fbh_mlp_nparams", "fbh_mlp_params", "fbh_mlp_forward", "fbh_mlp_grad", "fbh_mlp_train",
            "fbh_mlp_timing", "fbh_mlp_destroy", "fbh_batch_learned_setup", "fbh_batch_run_learned",
@@ -110,6 +121,10 @@ def _declare(L):
     L.fbh_device_copy.argtypes = [vp, vp, C.c_size_t, i, i]
     L.fbh_device_sync.argtypes = [i]
     L.fbh_batch_row_split.argtypes = [vp]
+    L.fbh_batch_configure.argtypes = [vp, i, i]
+    L.fbh_batch_set_order.argtypes = [vp, vp]
+    L.fbh_batch_config.argtypes = [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)]
+    L.fbh_batch_autotune.argtypes = [vp, i, C.POINTER(Tune)]
     L.fbh_debug_math.argtypes = [i, vp, vp, C.c_long, vp, vp, i]
     L.fbh_debug_libm_host.argtypes = [i, C.c_long, C.c_long, vp]
     L.fbh_debug_copy.argtypes = [C.c_long, i, C.POINTER(d)]

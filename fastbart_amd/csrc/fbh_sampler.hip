@@ -71,6 +71,7 @@ struct KArgs {
   int af_off;     // byte offset of the resident running fit in dynamic LDS
   unsigned long long* xbuf;  // row split: [group][parity][member][kXw] pass totals
   uint32_t* xctr;            // row split: [group] arrivals
+  const int* order;          // [C] device chain run by workgroup w of the sweep kernel
 };
 
 struct Rng {
@@ -1093,7 +1094,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: the role branches below are scalar
   const bool ctl = wave == 0;
   const int dw = wave - 1;
-  const int chain = blockIdx.x;
+  const int chain = a.order[blockIdx.x];  // the packing: which chain this workgroup runs
   if (ctl) {
     const uint32_t* src = reinterpret_cast<const uint32_t*>(a.desc + chain);
     uint32_t* dst = reinterpret_cast<uint32_t*>(&c.cd);
@@ -2163,10 +2164,26 @@ struct fbh_batch {
   Variant var{};
   size_t dyn_bytes = 0;
   int af_off = 0;
+  // the tile variants this batch may switch between at launch boundaries (the state between launches is the same)
+  Variant var_str{}, var_res{};
+  bool res_ok = false;
+  // the packing: workgroup w runs device chain order[w]
+  int packing = 0, ncu = 0;
+  std::vector<int> order;
+  int* d_order = nullptr;
+  // launches captured in a graph per fbh_batch_run call (kept until the batch is destroyed)
+  int graph = 0;
+  std::vector<hipGraph_t> graphs;
+  std::vector<hipGraphExec_t> gexecs;
   int nmax = 0;
   LibmTables tb{};
   size_t rows_total = 0, q_total = 0, leaf_total = 0;
-  std::vector<hipEvent_t> ev;  // start/stop pairs per launch
+  // timing: a start/stop event pair around each plain launch, or around each launch of a captured graph (nl launches)
+  struct Timed {
+    hipEvent_t e0, e1;
+    int nl;
+  };
+  std::vector<Timed> ev;
   double bytes_per_sweep = 0;
   // learned birth proposal
   fbh_mlp* net = nullptr;
@@ -2191,10 +2208,16 @@ static void free_batch(fbh_batch* b) {
   void* ptrs[] = {b->d_desc, b->d_state, b->d_trees, b->d_q,   b->d_Y,   b->d_W,   b->d_yin,    b->d_AF,
                   b->d_TM,   b->d_ybin,  b->d_leaf,  b->d_sigma, b->d_tout, b->d_PG, b->d_trm, b->d_trace,
                   b->d_img,  b->d_lps,   b->d_afx,   b->d_X,     b->d_rg,   b->d_px, b->d_prg, b->d_ptg,
-                  b->d_pcount, b->d_idx, b->d_xbuf, b->d_xctr};
+                  b->d_pcount, b->d_idx, b->d_xbuf, b->d_xctr, b->d_order};
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto g : b->gexecs) (void)hipGraphExecDestroy(g);
+  for (auto g : b->graphs) (void)hipGraphDestroy(g);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  for (auto e : b->ev) (void)hipEventDestroy(e);
+  for (auto& t : b->ev) {
+    (void)hipEventDestroy(t.e0);
+    (void)hipEventDestroy(t.e1);
+  }
   for (auto e : b->lev) (void)hipEventDestroy(e);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -2241,7 +2264,47 @@ static KArgs make_args(const fbh_batch* b) {
   a.af_off = b->af_off;
   a.xbuf = b->d_xbuf;
   a.xctr = b->d_xctr;
+  a.order = b->d_order;
   return a;
+}
+
+// The packing (workgroup -> device chain). 0: the batch order. 1: serpentine -- device chains by rows, largest first,
+// dealt to workgroups in rounds of one per CU with every second round reversed, so that where the dispatcher gives
+// workgroups w and w + ncu the same CU, a large chain shares its CU with a small one. 2: largest first (longest
+// processing time first, for batches with more chains than resident slots). Chains are independent, so the packing
+// changes timing only, never results.
+static void packing_order(const fbh_batch* b, int mode, std::vector<int>& ord) {
+  const int C = b->C;
+  ord.resize(C);
+  for (int w = 0; w < C; w++) ord[w] = w;
+  if (mode == 0) return;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return b->hdesc[x].n > b->hdesc[y].n; });
+  if (mode == 1) {
+    const int r = std::max(1, b->ncu);
+    for (int s0 = r, k = 1; s0 < C; s0 += r, k++)
+      if (k & 1) std::reverse(ord.begin() + s0, ord.begin() + std::min(C, s0 + r));
+  }
+}
+
+// switch the packing and the tile variant between launches (waits for the batch's queued work)
+static int configure(fbh_batch* b, int packing, bool resident) {
+  if (packing < 0 || packing > 2) {
+    set_error(FBH_EINVAL, "fbh_batch_configure: packing must be 0, 1 or 2");
+    return FBH_EINVAL;
+  }
+  if (resident && !b->res_ok) {
+    set_error(FBH_EINVAL, "fbh_batch_configure: 1536-row tiles need the running fit resident in LDS (n too large)");
+    return FBH_EINVAL;
+  }
+  FBH_HIP_CHECK(hipSetDevice(b->device));
+  FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
+  packing_order(b, packing, b->order);
+  FBH_HIP_CHECK(hipMemcpy(b->d_order, b->order.data(), sizeof(int) * b->C, hipMemcpyHostToDevice));
+  b->packing = packing;
+  b->var = resident ? b->var_res : b->var_str;
+  b->af_off = (int)align16(b->var.scratch);
+  b->dyn_bytes = (size_t)b->af_off + (resident ? 8 * afl_doubles((size_t)b->nmax) : 0);
+  return FBH_OK;
 }
 
 extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const fbh_opts* o, fbh_batch** out) {
@@ -2254,9 +2317,10 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
       (o->stats_mode != FBH_STATS_REF16 && o->stats_mode != FBH_STATS_FX64) ||
       (o->node_cap != 0 && (o->node_cap < 3 || o->node_cap > kNodes)) ||
       (o->block_threads != 0 && o->block_threads != kBlock) ||
-      (o->tile_rows != 0 && o->tile_rows != Geo<kRowsStream>::kTile && o->tile_rows != Geo<kRowsRes>::kTile)) {
+      (o->tile_rows != 0 && o->tile_rows != Geo<kRowsStream>::kTile && o->tile_rows != Geo<kRowsRes>::kTile) ||
+      o->packing < 0 || o->packing > 2) {
     set_error(FBH_EINVAL, "fbh_batch_create: bad options (m, nskip, ndpost, pgtab, stats_mode, node_cap, "
-                          "block_threads or tile_rows)");
+                          "block_threads, tile_rows or packing)");
     return FBH_EINVAL;
   }
   int W = o->row_split > 1 ? o->row_split : 1;
@@ -2377,26 +2441,35 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
       lds_max = 65536;
     const size_t budget = std::min<size_t>((size_t)lds_max, kLdsBudgetPerWG);
     const size_t need_res = kCtlBytes + align16(Geo<kRowsRes>::kScratchBytes) + 8 * afl_doubles((size_t)b->nmax);
-    const bool res = need_res <= budget && o->tile_rows != Geo<kRowsStream>::kTile;
+    b->res_ok = need_res <= budget;
+    const bool res = b->res_ok && o->tile_rows != Geo<kRowsStream>::kTile;
     if (o->tile_rows == Geo<kRowsRes>::kTile && !res) {
       set_error(FBH_EINVAL, "fbh_batch_create: 1536-row tiles need the running fit resident in LDS (n too large)");
       free_batch(b);
       return FBH_EINVAL;
     }
-    b->var = pick_variant(res, b->stats_mode == FBH_STATS_FX64, b->probit != 0);
+    const bool fx = b->stats_mode == FBH_STATS_FX64;
+    b->var_str = pick_variant(false, fx, b->probit != 0);
+    b->var_res = pick_variant(true, fx, b->probit != 0);
+    for (int r = 0; r < 2; r++) {
+      if (r == 1 && !b->res_ok) break;
+      const Variant& v = r ? b->var_res : b->var_str;
+      const size_t dyn = align16(v.scratch) + (r ? 8 * afl_doubles((size_t)b->nmax) : 0);
+      if (kCtlBytes + dyn > (size_t)lds_max) {
+        set_error(FBH_EINVAL, "fbh_batch_create: LDS need exceeds the device limit");
+        free_batch(b);
+        return FBH_EINVAL;
+      }
+      if (hipFuncSetAttribute(reinterpret_cast<const void*>(v.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)dyn) != hipSuccess) {
+        set_error(FBH_EHIP, "fbh_batch_create: hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
+        free_batch(b);
+        return FBH_EHIP;
+      }
+    }
+    b->var = res ? b->var_res : b->var_str;
     b->af_off = (int)align16(b->var.scratch);
     b->dyn_bytes = (size_t)b->af_off + (res ? 8 * afl_doubles((size_t)b->nmax) : 0);
-    if (kCtlBytes + b->dyn_bytes > (size_t)lds_max) {
-      set_error(FBH_EINVAL, "fbh_batch_create: LDS need exceeds the device limit");
-      free_batch(b);
-      return FBH_EINVAL;
-    }
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(b->var.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)b->dyn_bytes) != hipSuccess) {
-      set_error(FBH_EHIP, "fbh_batch_create: hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
-      free_batch(b);
-      return FBH_EHIP;
-    }
   }
   const size_t tout = b->keep_trees ? (size_t)nchains_l * b->ndpost * b->m * kNodes : 1;
   if (W > 1) {
@@ -2412,6 +2485,13 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
       free_batch(b);
       return FBH_EINVAL;
     }
+    // the other tile variant may be switched to only if its workgroups fit on the device at once as well
+    if (b->res_ok && b->var.fn != b->var_res.fn &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+             &per_cu, reinterpret_cast<const void*>(b->var_res.fn), kBlock,
+             align16(b->var_res.scratch) + 8 * afl_doubles((size_t)b->nmax)) != hipSuccess ||
+         (long)per_cu * ncu < (long)nchains))
+      b->res_ok = false;
     if ((rc = dalloc(&b->d_xbuf, (size_t)nchains_l * 2 * W * kXw)) || (rc = dalloc(&b->d_xctr, (size_t)nchains_l)) ||
         hipMemset(b->d_xctr, 0, sizeof(uint32_t) * nchains_l) != hipSuccess) {
       free_batch(b);
@@ -2423,10 +2503,15 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
       (rc = dalloc(&b->d_Y, ro)) || (rc = dalloc(&b->d_W, ro)) || (rc = dalloc(&b->d_yin, ro)) ||
       (rc = dalloc(&b->d_AF, ro)) || (rc = dalloc(&b->d_TM, ro)) || (rc = dalloc(&b->d_ybin, ro)) ||
       (rc = dalloc(&b->d_leaf, lo)) || (rc = dalloc(&b->d_sigma, (size_t)nchains * std::max(total, 1))) ||
-      (rc = dalloc(&b->d_tout, tout)) || (rc = dalloc(&b->d_PG, kNodes)) || (rc = dalloc(&b->d_trm, ro))) {
+      (rc = dalloc(&b->d_tout, tout)) || (rc = dalloc(&b->d_PG, kNodes)) || (rc = dalloc(&b->d_trm, ro)) ||
+      (rc = dalloc(&b->d_order, nchains))) {
     free_batch(b);
     return rc;
   }
+  if (hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess) b->ncu = 0;
+  b->packing = o->packing;
+  b->graph = o->graph ? 1 : 0;
+  packing_order(b, b->packing, b->order);
   // ---- staging on the host, one copy per pool
   {
     std::vector<uint8_t> hq(qo), hyb(ro, 0);
@@ -2463,6 +2548,7 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     cp(b->d_ybin, hyb.data(), ro);
     cp(b->d_PG, pg.data(), sizeof(double) * kNodes);
     cp(d_seeds, seeds.data(), sizeof(unsigned) * nchains);
+    cp(b->d_order, b->order.data(), sizeof(int) * nchains);
     if (e == hipSuccess) e = hipMemsetAsync(b->d_leaf, 0, lo, b->stream);
     if (e == hipSuccess) {
       KArgs a = make_args(b);
@@ -2488,23 +2574,165 @@ extern "C" int fbh_batch_run(fbh_batch* b, int nsweeps) {
   }
   FBH_HIP_CHECK(hipSetDevice(b->device));
   const int total = b->nskip + b->ndpost;
-  int end = std::min(total, b->it + nsweeps);
+  const int end = std::min(total, b->it + nsweeps);
+  if (b->it >= end) return FBH_OK;
+  // the launches of this call: plain, each between a pair of timing events; or, with graph = 1, captured into one
+  // graph of kernel nodes in stream order that is launched once between a pair of timing events
+  const int nl = (end - b->it + b->sweeps_per_launch - 1) / b->sweeps_per_launch;
+  const int npair = b->graph ? 1 : nl;
+  const size_t ev0 = b->ev.size();
+  for (int k = 0; k < npair; k++) {
+    fbh_batch::Timed t{nullptr, nullptr, b->graph ? nl : 1};
+    FBH_HIP_CHECK(hipEventCreate(&t.e0));
+    if (hipEventCreate(&t.e1) != hipSuccess) {
+      (void)hipEventDestroy(t.e0);
+      set_error(FBH_EHIP, "fbh_batch_run: hipEventCreate failed");
+      return FBH_EHIP;
+    }
+    b->ev.push_back(t);
+  }
+  hipError_t e = hipSuccess;
+  hipGraph_t g = nullptr;
+  if (b->graph) e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
   KArgs a = make_args(b);
-  while (b->it < end) {
-    const int it1 = std::min(end, b->it + b->sweeps_per_launch);
-    a.it0 = b->it;
-    a.it1 = it1;
-    hipEvent_t e0, e1;
-    FBH_HIP_CHECK(hipEventCreate(&e0));
-    FBH_HIP_CHECK(hipEventCreate(&e1));
-    b->ev.push_back(e0);
-    b->ev.push_back(e1);
-    FBH_HIP_CHECK(hipEventRecord(e0, b->stream));
+  for (int k = 0, it = b->it; k < nl && e == hipSuccess; k++) {
+    a.it0 = it;
+    a.it1 = std::min(end, it + b->sweeps_per_launch);
     void* args[] = {&a};
-    FBH_HIP_CHECK(hipLaunchKernel(reinterpret_cast<const void*>(b->var.fn), dim3(b->C), dim3(kBlock), args,
-                                  b->dyn_bytes, b->stream));
-    FBH_HIP_CHECK(hipEventRecord(e1, b->stream));
-    b->it = it1;
+    if (!b->graph) e = hipEventRecord(b->ev[ev0 + k].e0, b->stream);
+    if (e == hipSuccess)
+      e = hipLaunchKernel(reinterpret_cast<const void*>(b->var.fn), dim3(b->C), dim3(kBlock), args, b->dyn_bytes,
+                          b->stream);
+    if (e == hipSuccess && !b->graph) e = hipEventRecord(b->ev[ev0 + k].e1, b->stream);
+    it = a.it1;
+  }
+  if (b->graph) {
+    const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+    if (e == hipSuccess) e = ec;
+    if (g) b->graphs.push_back(g);
+    hipGraphExec_t x = nullptr;
+    if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (x) b->gexecs.push_back(x);
+    if (e == hipSuccess) e = hipEventRecord(b->ev[ev0].e0, b->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(x, b->stream);
+    if (e == hipSuccess) e = hipEventRecord(b->ev[ev0].e1, b->stream);
+  }
+  if (e != hipSuccess) {
+    set_error(FBH_EHIP, std::string("fbh_batch_run: ") + hipGetErrorString(e));
+    return FBH_EHIP;
+  }
+  b->it = end;
+  return FBH_OK;
+}
+
+extern "C" int fbh_batch_configure(fbh_batch* b, int packing, int tile_rows) {
+  if (!b || (tile_rows != 0 && tile_rows != Geo<kRowsStream>::kTile && tile_rows != Geo<kRowsRes>::kTile)) {
+    set_error(FBH_EINVAL, "fbh_batch_configure: bad argument");
+    return FBH_EINVAL;
+  }
+  if (b->net) {
+    set_error(FBH_EINVAL, "fbh_batch_configure: not for a batch set up for the learned proposal");
+    return FBH_EINVAL;
+  }
+  const bool res = tile_rows == 0 ? b->res_ok : tile_rows == Geo<kRowsRes>::kTile;
+  return configure(b, packing, res);
+}
+
+extern "C" int fbh_batch_set_order(fbh_batch* b, const int* order) {
+  if (!b || !order) {
+    set_error(FBH_EINVAL, "fbh_batch_set_order: null argument");
+    return FBH_EINVAL;
+  }
+  std::vector<int> seen(b->CL, 0);
+  for (int w = 0; w < b->CL; w++) {
+    if (order[w] < 0 || order[w] >= b->CL || seen[order[w]]++) {
+      set_error(FBH_EINVAL, "fbh_batch_set_order: not a permutation of the chains");
+      return FBH_EINVAL;
+    }
+  }
+  FBH_HIP_CHECK(hipSetDevice(b->device));
+  FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
+  // logical chain order[w] takes workgroups w W .. w W + W - 1 (its row-split members, in member order)
+  for (int w = 0; w < b->CL; w++)
+    for (int k = 0; k < b->W; k++) b->order[(size_t)w * b->W + k] = order[w] * b->W + k;
+  FBH_HIP_CHECK(hipMemcpy(b->d_order, b->order.data(), sizeof(int) * b->C, hipMemcpyHostToDevice));
+  b->packing = -1;
+  return FBH_OK;
+}
+
+extern "C" int fbh_batch_config(const fbh_batch* b, int* packing, int* tile_rows, int* graph) {
+  if (!b) {
+    set_error(FBH_EINVAL, "fbh_batch_config: null batch");
+    return FBH_EINVAL;
+  }
+  if (packing) *packing = b->packing;
+  if (tile_rows) *tile_rows = b->var.resident ? Geo<kRowsRes>::kTile : Geo<kRowsStream>::kTile;
+  if (graph) *graph = b->graph;
+  return FBH_OK;
+}
+
+// Autotuner: times every (packing, tile variant) candidate on the batch's own next sweeps -- real sweeps of every
+// chain, whose results no candidate changes -- and keeps the fastest. One untimed warm-up launch, then two rounds of
+// one launch of trial_sweeps sweeps per candidate, the second round in reverse order so that drift in the cost of a
+// sweep (trees growing during burn-in) falls on every candidate alike.
+extern "C" int fbh_batch_autotune(fbh_batch* b, int trial_sweeps, fbh_tune* out) {
+  if (!b || trial_sweeps < 1) {
+    set_error(FBH_EINVAL, "fbh_batch_autotune: bad argument");
+    return FBH_EINVAL;
+  }
+  if (b->net) {
+    set_error(FBH_EINVAL, "fbh_batch_autotune: not for a batch set up for the learned proposal");
+    return FBH_EINVAL;
+  }
+  int cp[FBH_TUNE_MAX], ct[FBH_TUNE_MAX], nc = 0;
+  for (int r = b->res_ok ? 1 : 0; r >= 0; r--)
+    for (int pk = 0; pk < 3; pk++) {
+      cp[nc] = pk;
+      ct[nc++] = r;
+    }
+  const int need = (1 + 2 * nc) * trial_sweeps;
+  if (b->it + need > b->nskip + b->ndpost) {
+    set_error(FBH_EINVAL, "fbh_batch_autotune: " + std::to_string(need) + " sweeps needed, " +
+                              std::to_string(b->nskip + b->ndpost - b->it) + " left");
+    return FBH_EINVAL;
+  }
+  const int spl = b->sweeps_per_launch, g = b->graph;
+  b->sweeps_per_launch = trial_sweeps;
+  b->graph = 0;
+  double ms[FBH_TUNE_MAX] = {0};
+  int rc = fbh_batch_run(b, trial_sweeps);
+  for (int round = 0; rc == FBH_OK && round < 2; round++)
+    for (int k = 0; rc == FBH_OK && k < nc; k++) {
+      const int i = round ? nc - 1 - k : k;
+      if ((rc = configure(b, cp[i], ct[i] != 0))) break;
+      if ((rc = fbh_batch_run(b, trial_sweeps))) break;
+      if ((rc = fbh_batch_sync(b))) break;
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, b->ev.back().e0, b->ev.back().e1) != hipSuccess) {
+        set_error(FBH_EHIP, "fbh_batch_autotune: hipEventElapsedTime failed");
+        rc = FBH_EHIP;
+        break;
+      }
+      ms[i] += t / (2.0 * trial_sweeps);
+    }
+  b->sweeps_per_launch = spl;
+  b->graph = g;
+  if (rc) return rc;
+  int best = 0;
+  for (int i = 1; i < nc; i++)
+    if (ms[i] < ms[best]) best = i;
+  if ((rc = configure(b, cp[best], ct[best] != 0))) return rc;
+  if (out) {
+    std::memset(out, 0, sizeof *out);
+    out->packing = cp[best];
+    out->tile_rows = ct[best] ? Geo<kRowsRes>::kTile : Geo<kRowsStream>::kTile;
+    out->ncand = nc;
+    out->sweeps = need;
+    for (int i = 0; i < nc; i++) {
+      out->cand_packing[i] = cp[i];
+      out->cand_tile_rows[i] = ct[i] ? Geo<kRowsRes>::kTile : Geo<kRowsStream>::kTile;
+      out->ms_per_sweep[i] = ms[i];
+    }
   }
   return FBH_OK;
 }
@@ -2617,11 +2845,13 @@ extern "C" int fbh_batch_timing(const fbh_batch* b, double* avg_ms, int* nlaunch
   if (!b) return FBH_EINVAL;
   double tot = 0;
   int k = 0;
-  for (size_t i = 0; i + 1 < b->ev.size(); i += 2) {
+  for (const auto& t : b->ev) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, b->ev[i], b->ev[i + 1]) == hipSuccess) {
+    if (hipEventElapsedTime(&ms, t.e0, t.e1) == hipSuccess) {
       tot += ms;
-      k++;
+      k += t.nl;
+    } else {
+      (void)hipGetLastError();  // a pair not yet complete: skipped, and its error not left for the next call
     }
   }
   if (avg_ms) *avg_ms = k ? tot / k : 0;

@@ -89,6 +89,11 @@ typedef struct fbh_opts {
                                n >= 64 W, and all C x W workgroups must fit on the device at once (FBH_EINVAL if not).
                                -1 = automatic for FX64 wbart batches: the largest W with C x W <= the CU count and
                                n >= 64 W (1 otherwise). */
+  int packing;              /* workgroup -> chain assignment of the sweep kernel: 0 = batch order, 1 = serpentine by
+                               rows (large and small chains share CUs), 2 = largest first. Timing only: results do not
+                               depend on it. fbh_batch_autotune measures and picks. */
+  int graph;                /* 1: fbh_batch_run captures each call's launches into a hipGraph and launches the graph
+                               (0: plain stream launches). Results do not depend on it. */
 } fbh_opts;
 
 typedef struct fbh_chain_out {
@@ -110,6 +115,29 @@ int fbh_batch_sync(fbh_batch* b);
 int fbh_batch_iter(const fbh_batch* b);
 /* workgroups per chain (the row split in effect; 1 without one) */
 int fbh_batch_row_split(const fbh_batch* b);
+
+/* ---------------------------------------------------------------- launch configuration and autotuner (new; the
+ * reference runs one chain per process and has no launch to configure) */
+#define FBH_TUNE_MAX 8
+typedef struct fbh_tune {
+  int packing, tile_rows;          /* the configuration chosen (in effect on return) */
+  int ncand;                       /* candidates timed */
+  int cand_packing[FBH_TUNE_MAX], cand_tile_rows[FBH_TUNE_MAX];
+  double ms_per_sweep[FBH_TUNE_MAX];
+  int sweeps;                      /* sweeps the tuning ran: real sweeps of every chain, results unchanged */
+} fbh_tune;
+/* Switch packing (0, 1, 2 as fbh_opts.packing) and tile rows (1024, 1536, 0 = the automatic choice) between runs;
+ * waits for the batch's queued work. Results do not depend on either. */
+int fbh_batch_configure(fbh_batch* b, int packing, int tile_rows);
+/* A packing of the caller's: logical chain order[w] runs on workgroup w (on workgroups w W .. w W + W - 1 with a row
+ * split of W); order is a permutation of 0..nchains-1. Waits for the batch's queued work; results do not depend on it. */
+int fbh_batch_set_order(fbh_batch* b, const int* order);
+/* the configuration in effect (packing -1 after fbh_batch_set_order) */
+int fbh_batch_config(const fbh_batch* b, int* packing, int* tile_rows, int* graph);
+/* Time every (packing, tile rows) candidate on the batch's next sweeps -- one warm-up launch, then two launches of
+ * trial_sweeps sweeps per candidate -- and keep the fastest. The sweeps are real sweeps of every chain and count
+ * toward nskip + ndpost (FBH_EINVAL if too few are left); no candidate changes any result. out may be null. */
+int fbh_batch_autotune(fbh_batch* b, int trial_sweeps, fbh_tune* out);
 /* Copy chain c's outputs to host buffers (synchronises). */
 int fbh_batch_fetch(fbh_batch* b, int chain, const fbh_chain_out* out);
 /* Copy chain c's sampler state (parity checks, checkpoints): rng[4], sigma, af[n], latent[n], leaf[m][n],

@@ -1,0 +1,134 @@
+"""Launch configuration of the sweep kernel: the workgroup -> chain packing, the tile rows, graph capture and the
+autotuner that picks among them. Chains are independent and the state between launches is complete in device memory,
+so none of these may change a result: every configuration, and a switch between configurations at any launch
+boundary, must reproduce the oracle's chains bit for bit."""
+import pytest
+
+from conftest import same_bits
+from fastbart_amd import sampler
+from fastbart_amd import workloads as W
+from fastbart_amd._lib import FbhError
+
+pytestmark = pytest.mark.gpu
+
+NS = (300, 1700, 901, 2500, 1203, 640)
+
+
+def _say(*a):
+    print("[launch config]", *a, flush=True)
+
+
+def _probs(probit=False, ntree=12):
+    return [W.small_problem(n, 4, 20, 510 + n, "uniform", probit=probit, ntree=ntree) for n in NS]
+
+
+def _check_oracle(oracle, b, probs, m, nskip, nd, seeds, probit=0):
+    for c, pb in enumerate(probs):
+        e = oracle.fit(pb, m, nskip, nd, seeds[c], probit=probit)
+        out = b.fetch(c)
+        for k in ("sigma", "trmean", "trees", "stats"):
+            assert same_bits(out[k], e[k]), (c, k)
+
+
+@pytest.mark.parametrize("packing,graph", [(1, 1), (2, 0), (0, 1)])
+def test_gpu_packing_and_graph_match_oracle(oracle, packing, graph):
+    probs = _probs()
+    m, nskip, nd = 12, 14, 6
+    seeds = [31 + c for c in range(len(probs))]
+    _say("packing", packing, "graph", graph)
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds, packing=packing, graph=graph, sweeps_per_launch=4) as b:
+        assert b.config[0] == packing and b.config[2] == graph
+        b.run(7)
+        b.run(nskip + nd - 7)
+        b.sync()
+        avg, nl, _ = b.timing()
+        assert nl == 2 + 4 and avg > 0  # 7 sweeps in 2 launches, 13 in 4 (a graph is timed as a whole)
+        _check_oracle(oracle, b, probs, m, nskip, nd, seeds)
+
+
+def test_gpu_probit_packing_and_graph_match_oracle(oracle):
+    probs = _probs(probit=True, ntree=8)
+    m, nskip, nd = 8, 9, 5
+    seeds = [41 + c for c in range(len(probs))]
+    _say("probit, packing 1, graph")
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds, probit=1, packing=1, graph=1, sweeps_per_launch=3) as b:
+        b.run(nskip + nd)
+        b.sync()
+        _check_oracle(oracle, b, probs, m, nskip, nd, seeds, probit=1)
+
+
+def test_gpu_configuration_switched_between_launches(oracle):
+    """Packing and tile rows changed at every launch boundary: the state carried between launches is the same in
+    every configuration."""
+    probs = _probs()
+    m, nskip, nd = 12, 15, 6
+    seeds = [51 + c for c in range(len(probs))]
+    cycle = [(0, 1024), (1, 1536), (2, 1024), (2, 1536), (1, 1024), (0, 1536), (1, 0)]
+    _say("switched configurations")
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds, sweeps_per_launch=3) as b:
+        for k, (pk, tr) in enumerate(cycle):
+            b.configure(pk, tr)
+            got = b.config
+            assert got[0] == pk and got[1] == (tr or 1536)
+            b.run(3)
+        b.sync()
+        assert b.it == nskip + nd
+        _check_oracle(oracle, b, probs, m, nskip, nd, seeds)
+
+
+def test_gpu_caller_order_matches_oracle(oracle):
+    probs = _probs()
+    m, nskip, nd = 12, 10, 5
+    seeds = [71 + c for c in range(len(probs))]
+    _say("caller's order")
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds, sweeps_per_launch=3) as b:
+        b.set_order([3, 0, 5, 1, 4, 2])
+        assert b.config[0] == -1
+        b.run(8)
+        b.set_order([5, 4, 3, 2, 1, 0])
+        b.run(nskip + nd - 8)
+        b.sync()
+        _check_oracle(oracle, b, probs, m, nskip, nd, seeds)
+        for bad in ([0, 1, 2, 3, 4, 4], [0, 1, 2, 3, 4, 6], [-1, 1, 2, 3, 4, 5]):
+            with pytest.raises(FbhError):
+                b.set_order(bad)
+
+
+def test_gpu_autotune_runs_real_sweeps_and_keeps_results(oracle):
+    probs = _probs()
+    m, nskip, nd = 12, 16, 6
+    seeds = [61 + c for c in range(len(probs))]
+    _say("autotune")
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds, sweeps_per_launch=5, graph=1) as b:
+        t = b.autotune(1)
+        assert len(t["candidates"]) == 6  # three packings x two tile sizes (the running fit fits in LDS here)
+        assert t["sweeps"] == 13 and b.it == 13
+        assert all(c["ms_per_sweep"] > 0 for c in t["candidates"])
+        best = min(t["candidates"], key=lambda c: c["ms_per_sweep"])
+        assert (t["packing"], t["tile_rows"]) == (best["packing"], best["tile_rows"])
+        assert b.config[:2] == (t["packing"], t["tile_rows"])
+        assert b.config[2] == 1  # graph capture is left as it was
+        b.run(nskip + nd - 13)
+        b.sync()
+        _check_oracle(oracle, b, probs, m, nskip, nd, seeds)
+
+
+def test_gpu_launch_config_rejections():
+    probs = _probs()[:2]
+    with sampler.BatchSampler(probs, 8, 4, 2, [1, 2]) as b:
+        with pytest.raises(FbhError):
+            b.configure(3, 0)
+        with pytest.raises(FbhError):
+            b.configure(0, 512)
+        with pytest.raises(FbhError):
+            b.autotune(1)  # 13 sweeps needed, 6 left
+        assert b.it == 0
+    big = [W.small_problem(12000, 2, 10, 7, "uniform", ntree=4)]
+    with sampler.BatchSampler(big, 4, 2, 1, [1]) as b:
+        assert b.config[1] == 1024  # the running fit does not fit in LDS
+        with pytest.raises(FbhError):
+            b.configure(0, 1536)
+        with pytest.raises(FbhError):
+            b.autotune(1)  # three candidates (no 1536-row tiles): 7 sweeps needed, 3 left
+    with pytest.raises(FbhError):
+        sampler.BatchSampler(big, 4, 2, 1, [1], packing=5)

@@ -641,7 +641,8 @@ __device__ __forceinline__ void tile_load(TileIn<R, YF, AFT>& T, const RowPtrs& 
   }
 }
 
-template <int R, bool RES, bool FX, bool YF, bool AFT>
+// D: the dense build (two chains per CU, 128 VGPRs), which gathers one row at a time in phase A
+template <int R, bool RES, bool FX, bool D = false, bool YF, bool AFT>
 __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int dw, int lane, int jp, int j, int cb,
                           bool apply, bool stats, bool rss, bool keep, TileIn<R, YF, AFT>& T, bool first_loaded) {
   static_assert(AFT || RES, "a running fit outside LDS is part of the tile");
@@ -691,7 +692,7 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
     // after tree j-1's change and every LDS gather of the tile's rows (leaf values, ranks), all in flight together;
     // then the arithmetic and the stores. Rows past n compute on row n-1's loads and store nothing global.
     const uint32_t abase = opaque((uint32_t)(iis * kLanes + ka));
-    constexpr int RC = (R % 3 == 0) ? 3 : 2;  // rows per gather batch (batches keep the registers in budget)
+    constexpr int RC = D ? 1 : (R % 3 == 0) ? 3 : 2;  // rows per gather batch (batches keep the registers in budget)
 #pragma unroll
     for (int r0 = 0; r0 < R; r0 += RC) {
       // a batch whose first row for this wave is past n (the last tile) is skipped, and so are the later ones
@@ -1083,10 +1084,14 @@ __device__ void row_split_exchange(Ctl& c, const KArgs& a, int cb, bool rss, uin
 }
 
 // ------------------------------------------------------------------ the sweep kernel
-// Two chains per CU: 2 x 5 waves over 4 SIMDs needs 3 waves per SIMD, i.e. at most 168 VGPRs. PB: probit (latent
-// redraw after each sweep, sigma fixed) -- a separate instantiation so that the wbart kernel carries none of its code.
-template <int R, bool RES, bool FX, bool PB>
-__global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(3))) void k_sweeps(
+// Residency: the dispatcher reserves registers for a 5-wave workgroup as if each SIMD held two of its waves, so a
+// second workgroup fits on a CU only if four waves fit per SIMD, i.e. at most 128 VGPRs (with 168, 512 workgroups on
+// 256 CUs were measured to run as two consecutive rounds of 256). D = 1, the dense build: 128 VGPRs, two chains per
+// CU, for batches with more workgroups than CUs. D = 0: 168 VGPRs (three waves per SIMD), one chain per CU, fewer
+// spills. PB: probit (latent redraw after each sweep, sigma fixed) -- a separate instantiation so that the wbart
+// kernel carries none of its code.
+template <int R, bool RES, bool FX, bool PB, bool D>
+__global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(D ? 4 : 3))) void k_sweeps(
     KArgs a) {
   __shared__ Ctl c;
   unsigned char* const dyn = fbh_dyn;
@@ -1380,7 +1385,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       int cb = 0;
       bool pre = false;
       for (int j = 0; j < m; j++) {
-        data_pass<R, RES, FX>(c, dyn, P, dw, lane, j - 1, j, cb, j > 0, true, false, false, T, pre);
+        data_pass<R, RES, FX, D>(c, dyn, P, dw, lane, j - 1, j, cb, j > 0, true, false, false, T, pre);
         __syncthreads();
         // while the control wave decides tree j, the next pass's first tile is loaded
         load_next_first_tile<R, RES>(T, c, P, dw, lane, j, m, cb);
@@ -1388,7 +1393,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         __syncthreads();
         cb ^= 1;
       }
-      data_pass<R, RES, FX>(c, dyn, P, dw, lane, m - 1, m, cb, true, false, true, keep, T, pre);
+      data_pass<R, RES, FX, D>(c, dyn, P, dw, lane, m - 1, m, cb, true, false, true, keep, T, pre);
       __syncthreads();
       if (PB) {
         __syncthreads();
@@ -1465,18 +1470,20 @@ struct Variant {
   SweepKernel fn;
   int scratch;
   bool resident;
+  bool dense;
 };
 
-template <bool PB>
-inline Variant pick_variant_pb(bool resident, bool fx) {
+template <bool PB, bool D>
+inline Variant pick_variant_pd(bool resident, bool fx) {
   if (resident)
-    return fx ? Variant{k_sweeps<kRowsRes, true, true, PB>, Geo<kRowsRes>::kScratchBytes, true}
-              : Variant{k_sweeps<kRowsRes, true, false, PB>, Geo<kRowsRes>::kScratchBytes, true};
-  return fx ? Variant{k_sweeps<kRowsStream, false, true, PB>, Geo<kRowsStream>::kScratchBytes, false}
-            : Variant{k_sweeps<kRowsStream, false, false, PB>, Geo<kRowsStream>::kScratchBytes, false};
+    return fx ? Variant{k_sweeps<kRowsRes, true, true, PB, D>, Geo<kRowsRes>::kScratchBytes, true, D}
+              : Variant{k_sweeps<kRowsRes, true, false, PB, D>, Geo<kRowsRes>::kScratchBytes, true, D};
+  return fx ? Variant{k_sweeps<kRowsStream, false, true, PB, D>, Geo<kRowsStream>::kScratchBytes, false, D}
+            : Variant{k_sweeps<kRowsStream, false, false, PB, D>, Geo<kRowsStream>::kScratchBytes, false, D};
 }
-inline Variant pick_variant(bool resident, bool fx, bool probit) {
-  return probit ? pick_variant_pb<true>(resident, fx) : pick_variant_pb<false>(resident, fx);
+inline Variant pick_variant(bool resident, bool fx, bool probit, bool dense) {
+  if (probit) return dense ? pick_variant_pd<true, true>(resident, fx) : pick_variant_pd<true, false>(resident, fx);
+  return dense ? pick_variant_pd<false, true>(resident, fx) : pick_variant_pd<false, false>(resident, fx);
 }
 
 constexpr int kRowsStreamL = kRowsStream;  // the learned pipeline streams the running fit
@@ -2164,9 +2171,11 @@ struct fbh_batch {
   Variant var{};
   size_t dyn_bytes = 0;
   int af_off = 0;
-  // the tile variants this batch may switch between at launch boundaries (the state between launches is the same)
-  Variant var_str{}, var_res{};
+  // the kernel variants this batch may switch between at launch boundaries (the state between launches is the same):
+  // [dense][resident]; fits = all workgroups resident at once (required by a row split)
+  Variant vars[2][2]{};
   bool res_ok = false;
+  bool fits[2][2]{};
   // the packing: workgroup w runs device chain order[w]
   int packing = 0, ncu = 0;
   std::vector<int> order;
@@ -2286,8 +2295,19 @@ static void packing_order(const fbh_batch* b, int mode, std::vector<int>& ord) {
   }
 }
 
-// switch the packing and the tile variant between launches (waits for the batch's queued work)
-static int configure(fbh_batch* b, int packing, bool resident) {
+static size_t dyn_of(const fbh_batch* b, bool resident) {
+  return align16(resident ? Geo<kRowsRes>::kScratchBytes : Geo<kRowsStream>::kScratchBytes) +
+         (resident ? 8 * afl_doubles((size_t)b->nmax) : 0);
+}
+
+static void use_variant(fbh_batch* b, bool resident, bool dense) {
+  b->var = b->vars[dense][resident];
+  b->af_off = (int)align16(b->var.scratch);
+  b->dyn_bytes = dyn_of(b, resident);
+}
+
+// switch the packing, the tile variant and the build between launches (waits for the batch's queued work)
+static int configure(fbh_batch* b, int packing, bool resident, bool dense) {
   if (packing < 0 || packing > 2) {
     set_error(FBH_EINVAL, "fbh_batch_configure: packing must be 0, 1 or 2");
     return FBH_EINVAL;
@@ -2296,14 +2316,16 @@ static int configure(fbh_batch* b, int packing, bool resident) {
     set_error(FBH_EINVAL, "fbh_batch_configure: 1536-row tiles need the running fit resident in LDS (n too large)");
     return FBH_EINVAL;
   }
+  if (b->W > 1 && !b->fits[dense][resident]) {
+    set_error(FBH_EINVAL, "fbh_batch_configure: the row split's workgroups would not all be resident at once");
+    return FBH_EINVAL;
+  }
   FBH_HIP_CHECK(hipSetDevice(b->device));
   FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
   packing_order(b, packing, b->order);
   FBH_HIP_CHECK(hipMemcpy(b->d_order, b->order.data(), sizeof(int) * b->C, hipMemcpyHostToDevice));
   b->packing = packing;
-  b->var = resident ? b->var_res : b->var_str;
-  b->af_off = (int)align16(b->var.scratch);
-  b->dyn_bytes = (size_t)b->af_off + (resident ? 8 * afl_doubles((size_t)b->nmax) : 0);
+  use_variant(b, resident, dense);
   return FBH_OK;
 }
 
@@ -2318,9 +2340,9 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
       (o->node_cap != 0 && (o->node_cap < 3 || o->node_cap > kNodes)) ||
       (o->block_threads != 0 && o->block_threads != kBlock) ||
       (o->tile_rows != 0 && o->tile_rows != Geo<kRowsStream>::kTile && o->tile_rows != Geo<kRowsRes>::kTile) ||
-      o->packing < 0 || o->packing > 2) {
+      o->packing < 0 || o->packing > 2 || o->chains_per_cu < 0 || o->chains_per_cu > 2) {
     set_error(FBH_EINVAL, "fbh_batch_create: bad options (m, nskip, ndpost, pgtab, stats_mode, node_cap, "
-                          "block_threads, tile_rows or packing)");
+                          "block_threads, tile_rows, packing or chains_per_cu)");
     return FBH_EINVAL;
   }
   int W = o->row_split > 1 ? o->row_split : 1;
@@ -2448,50 +2470,52 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
       free_batch(b);
       return FBH_EINVAL;
     }
+    if (hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess) b->ncu = 0;
     const bool fx = b->stats_mode == FBH_STATS_FX64;
-    b->var_str = pick_variant(false, fx, b->probit != 0);
-    b->var_res = pick_variant(true, fx, b->probit != 0);
-    for (int r = 0; r < 2; r++) {
-      if (r == 1 && !b->res_ok) break;
-      const Variant& v = r ? b->var_res : b->var_str;
-      const size_t dyn = align16(v.scratch) + (r ? 8 * afl_doubles((size_t)b->nmax) : 0);
-      if (kCtlBytes + dyn > (size_t)lds_max) {
-        set_error(FBH_EINVAL, "fbh_batch_create: LDS need exceeds the device limit");
-        free_batch(b);
-        return FBH_EINVAL;
+    for (int d = 0; d < 2; d++)
+      for (int r = 0; r < 2; r++) {
+        Variant& v = b->vars[d][r];
+        v = pick_variant(r != 0, fx, b->probit != 0, d != 0);
+        if (r == 1 && !b->res_ok) continue;
+        const size_t dyn = dyn_of(b, r != 0);
+        if (kCtlBytes + dyn > (size_t)lds_max) {
+          set_error(FBH_EINVAL, "fbh_batch_create: LDS need exceeds the device limit");
+          free_batch(b);
+          return FBH_EINVAL;
+        }
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(v.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)dyn) != hipSuccess) {
+          set_error(FBH_EHIP, "fbh_batch_create: hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
+          free_batch(b);
+          return FBH_EHIP;
+        }
+        // workgroups resident per CU: the occupancy calculator's figure, capped by the dispatcher's reservation of
+        // two waves per SIMD for a 5-wave workgroup (one workgroup per CU at 168 VGPRs, two at 128)
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(v.fn), kBlock, dyn) !=
+            hipSuccess)
+          per_cu = 0;
+        per_cu = std::min(per_cu, d ? 2 : 1);
+        b->fits[d][r] = (long)per_cu * b->ncu >= (long)nchains;
       }
-      if (hipFuncSetAttribute(reinterpret_cast<const void*>(v.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)dyn) != hipSuccess) {
-        set_error(FBH_EHIP, "fbh_batch_create: hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
-        free_batch(b);
-        return FBH_EHIP;
-      }
-    }
-    b->var = res ? b->var_res : b->var_str;
-    b->af_off = (int)align16(b->var.scratch);
-    b->dyn_bytes = (size_t)b->af_off + (res ? 8 * afl_doubles((size_t)b->nmax) : 0);
+    const bool dense = o->chains_per_cu == 2 || (o->chains_per_cu == 0 && nchains > b->ncu);
+    use_variant(b, res, dense);
   }
   const size_t tout = b->keep_trees ? (size_t)nchains_l * b->ndpost * b->m * kNodes : 1;
   if (W > 1) {
-    // every member must be resident at once: the exchange waits for all of them
-    int per_cu = 0, ncu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(b->var.fn), kBlock,
-                                                     b->dyn_bytes) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess ||
-        (long)per_cu * ncu < (long)nchains) {
+    // every member must be resident at once: the exchange waits for all of them. An automatic choice of build takes
+    // the other build if only that one fits.
+    const bool res = b->var.resident;
+    bool dense = b->var.dense;
+    if (!b->fits[dense][res] && o->chains_per_cu == 0 && b->fits[!dense][res]) dense = !dense;
+    if (!b->fits[dense][res]) {
       set_error(FBH_EINVAL, "fbh_batch_create: " + std::to_string(nchains) + " workgroups of a row split do not fit "
-                                "on the device at once (" + std::to_string(per_cu) + " per CU x " +
-                                std::to_string(ncu) + " CUs)");
+                                "on the device at once (" + std::to_string(dense ? 2 : 1) + " per CU x " +
+                                std::to_string(b->ncu) + " CUs)");
       free_batch(b);
       return FBH_EINVAL;
     }
-    // the other tile variant may be switched to only if its workgroups fit on the device at once as well
-    if (b->res_ok && b->var.fn != b->var_res.fn &&
-        (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-             &per_cu, reinterpret_cast<const void*>(b->var_res.fn), kBlock,
-             align16(b->var_res.scratch) + 8 * afl_doubles((size_t)b->nmax)) != hipSuccess ||
-         (long)per_cu * ncu < (long)nchains))
-      b->res_ok = false;
+    use_variant(b, res, dense);
     if ((rc = dalloc(&b->d_xbuf, (size_t)nchains_l * 2 * W * kXw)) || (rc = dalloc(&b->d_xctr, (size_t)nchains_l)) ||
         hipMemset(b->d_xctr, 0, sizeof(uint32_t) * nchains_l) != hipSuccess) {
       free_batch(b);
@@ -2508,7 +2532,6 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     free_batch(b);
     return rc;
   }
-  if (hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess) b->ncu = 0;
   b->packing = o->packing;
   b->graph = o->graph ? 1 : 0;
   packing_order(b, b->packing, b->order);
@@ -2625,8 +2648,9 @@ extern "C" int fbh_batch_run(fbh_batch* b, int nsweeps) {
   return FBH_OK;
 }
 
-extern "C" int fbh_batch_configure(fbh_batch* b, int packing, int tile_rows) {
-  if (!b || (tile_rows != 0 && tile_rows != Geo<kRowsStream>::kTile && tile_rows != Geo<kRowsRes>::kTile)) {
+extern "C" int fbh_batch_configure(fbh_batch* b, int packing, int tile_rows, int chains_per_cu) {
+  if (!b || (tile_rows != 0 && tile_rows != Geo<kRowsStream>::kTile && tile_rows != Geo<kRowsRes>::kTile) ||
+      chains_per_cu < 0 || chains_per_cu > 2) {
     set_error(FBH_EINVAL, "fbh_batch_configure: bad argument");
     return FBH_EINVAL;
   }
@@ -2635,7 +2659,8 @@ extern "C" int fbh_batch_configure(fbh_batch* b, int packing, int tile_rows) {
     return FBH_EINVAL;
   }
   const bool res = tile_rows == 0 ? b->res_ok : tile_rows == Geo<kRowsRes>::kTile;
-  return configure(b, packing, res);
+  const bool dense = chains_per_cu == 0 ? b->var.dense : chains_per_cu == 2;
+  return configure(b, packing, res, dense);
 }
 
 extern "C" int fbh_batch_set_order(fbh_batch* b, const int* order) {
@@ -2660,21 +2685,24 @@ extern "C" int fbh_batch_set_order(fbh_batch* b, const int* order) {
   return FBH_OK;
 }
 
-extern "C" int fbh_batch_config(const fbh_batch* b, int* packing, int* tile_rows, int* graph) {
+extern "C" int fbh_batch_config(const fbh_batch* b, int* packing, int* tile_rows, int* chains_per_cu, int* graph) {
   if (!b) {
     set_error(FBH_EINVAL, "fbh_batch_config: null batch");
     return FBH_EINVAL;
   }
   if (packing) *packing = b->packing;
   if (tile_rows) *tile_rows = b->var.resident ? Geo<kRowsRes>::kTile : Geo<kRowsStream>::kTile;
+  if (chains_per_cu) *chains_per_cu = b->var.dense ? 2 : 1;
   if (graph) *graph = b->graph;
   return FBH_OK;
 }
 
-// Autotuner: times every (packing, tile variant) candidate on the batch's own next sweeps -- real sweeps of every
-// chain, whose results no candidate changes -- and keeps the fastest. One untimed warm-up launch, then two rounds of
-// one launch of trial_sweeps sweeps per candidate, the second round in reverse order so that drift in the cost of a
-// sweep (trees growing during burn-in) falls on every candidate alike.
+// Autotuner: times candidate launch configurations on the batch's own next sweeps -- real sweeps of every chain, whose
+// results no candidate changes -- and keeps the fastest. Two stages: the build (one or two chains per CU; two only
+// with more workgroups than CUs) x the tile rows, at largest-first packing; then the packing with the stage's winner.
+// One untimed warm-up launch, then per stage two rounds of one launch of trial_sweeps sweeps per candidate, the
+// second round in reverse order so that drift in the cost of a sweep (trees growing during burn-in) falls on every
+// candidate alike.
 extern "C" int fbh_batch_autotune(fbh_batch* b, int trial_sweeps, fbh_tune* out) {
   if (!b || trial_sweeps < 1) {
     set_error(FBH_EINVAL, "fbh_batch_autotune: bad argument");
@@ -2684,13 +2712,17 @@ extern "C" int fbh_batch_autotune(fbh_batch* b, int trial_sweeps, fbh_tune* out)
     set_error(FBH_EINVAL, "fbh_batch_autotune: not for a batch set up for the learned proposal");
     return FBH_EINVAL;
   }
-  int cp[FBH_TUNE_MAX], ct[FBH_TUNE_MAX], nc = 0;
-  for (int r = b->res_ok ? 1 : 0; r >= 0; r--)
-    for (int pk = 0; pk < 3; pk++) {
-      cp[nc] = pk;
-      ct[nc++] = r;
-    }
-  const int need = (1 + 2 * nc) * trial_sweeps;
+  struct Cand {
+    int packing;
+    bool res, dense;
+    double ms;
+  };
+  std::vector<Cand> s1;
+  for (int d = (b->C > b->ncu) ? 1 : 0; d >= 0; d--)
+    for (int r = b->res_ok ? 1 : 0; r >= 0; r--)
+      if (b->W <= 1 || b->fits[d][r]) s1.push_back(Cand{2, r != 0, d != 0, 0.0});
+  if (s1.empty()) s1.push_back(Cand{2, b->var.resident, b->var.dense, 0.0});
+  const int need = (1 + 2 * (int)s1.size() + 2 * 3) * trial_sweeps;
   if (b->it + need > b->nskip + b->ndpost) {
     set_error(FBH_EINVAL, "fbh_batch_autotune: " + std::to_string(need) + " sweeps needed, " +
                               std::to_string(b->nskip + b->ndpost - b->it) + " left");
@@ -2699,43 +2731,62 @@ extern "C" int fbh_batch_autotune(fbh_batch* b, int trial_sweeps, fbh_tune* out)
   const int spl = b->sweeps_per_launch, g = b->graph;
   b->sweeps_per_launch = trial_sweeps;
   b->graph = 0;
-  double ms[FBH_TUNE_MAX] = {0};
-  int rc = fbh_batch_run(b, trial_sweeps);
-  for (int round = 0; rc == FBH_OK && round < 2; round++)
-    for (int k = 0; rc == FBH_OK && k < nc; k++) {
-      const int i = round ? nc - 1 - k : k;
-      if ((rc = configure(b, cp[i], ct[i] != 0))) break;
-      if ((rc = fbh_batch_run(b, trial_sweeps))) break;
-      if ((rc = fbh_batch_sync(b))) break;
-      float t = 0.f;
-      if (hipEventElapsedTime(&t, b->ev.back().e0, b->ev.back().e1) != hipSuccess) {
-        set_error(FBH_EHIP, "fbh_batch_autotune: hipEventElapsedTime failed");
-        rc = FBH_EHIP;
-        break;
+  auto stage = [&](std::vector<Cand>& cs) -> int {
+    for (int round = 0; round < 2; round++)
+      for (size_t k = 0; k < cs.size(); k++) {
+        Cand& cd = cs[round ? cs.size() - 1 - k : k];
+        int rc = configure(b, cd.packing, cd.res, cd.dense);
+        if (!rc) rc = fbh_batch_run(b, trial_sweeps);
+        if (!rc) rc = fbh_batch_sync(b);
+        if (rc) return rc;
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, b->ev.back().e0, b->ev.back().e1) != hipSuccess) {
+          (void)hipGetLastError();
+          set_error(FBH_EHIP, "fbh_batch_autotune: hipEventElapsedTime failed");
+          return FBH_EHIP;
+        }
+        cd.ms += t / (2.0 * trial_sweeps);
       }
-      ms[i] += t / (2.0 * trial_sweeps);
-    }
+    return FBH_OK;
+  };
+  auto best_of = [](const std::vector<Cand>& cs) {
+    size_t k = 0;
+    for (size_t i = 1; i < cs.size(); i++)
+      if (cs[i].ms < cs[k].ms) k = i;
+    return cs[k];
+  };
+  int rc = fbh_batch_run(b, trial_sweeps);
+  std::vector<Cand> s2;
+  if (!rc) rc = stage(s1);
+  if (!rc) {
+    const Cand w = best_of(s1);
+    for (int pk = 0; pk < 3; pk++) s2.push_back(Cand{pk, w.res, w.dense, 0.0});
+    rc = stage(s2);
+  }
   b->sweeps_per_launch = spl;
   b->graph = g;
   if (rc) return rc;
-  int best = 0;
-  for (int i = 1; i < nc; i++)
-    if (ms[i] < ms[best]) best = i;
-  if ((rc = configure(b, cp[best], ct[best] != 0))) return rc;
+  const Cand w = best_of(s2);
+  if ((rc = configure(b, w.packing, w.res, w.dense))) return rc;
   if (out) {
     std::memset(out, 0, sizeof *out);
-    out->packing = cp[best];
-    out->tile_rows = ct[best] ? Geo<kRowsRes>::kTile : Geo<kRowsStream>::kTile;
-    out->ncand = nc;
+    out->packing = w.packing;
+    out->tile_rows = w.res ? Geo<kRowsRes>::kTile : Geo<kRowsStream>::kTile;
+    out->chains_per_cu = w.dense ? 2 : 1;
     out->sweeps = need;
-    for (int i = 0; i < nc; i++) {
-      out->cand_packing[i] = cp[i];
-      out->cand_tile_rows[i] = ct[i] ? Geo<kRowsRes>::kTile : Geo<kRowsStream>::kTile;
-      out->ms_per_sweep[i] = ms[i];
+    std::vector<Cand> all(s1);
+    all.insert(all.end(), s2.begin(), s2.end());
+    out->ncand = std::min<int>((int)all.size(), FBH_TUNE_MAX);
+    for (int i = 0; i < out->ncand; i++) {
+      out->cand_packing[i] = all[i].packing;
+      out->cand_tile_rows[i] = all[i].res ? Geo<kRowsRes>::kTile : Geo<kRowsStream>::kTile;
+      out->cand_chains_per_cu[i] = all[i].dense ? 2 : 1;
+      out->ms_per_sweep[i] = all[i].ms;
     }
   }
   return FBH_OK;
 }
+
 
 // a row split whose exchange timed out (members not resident together) has invalid results: report it
 static int check_split(fbh_batch* b) {

@@ -94,6 +94,9 @@ typedef struct fbh_opts {
                                depend on it. fbh_batch_autotune measures and picks. */
   int graph;                /* 1: fbh_batch_run captures each call's launches into a hipGraph and launches the graph
                                (0: plain stream launches). Results do not depend on it. */
+  int chains_per_cu;        /* build of the sweep kernel: 1 = 168 VGPRs, one workgroup per CU; 2 = 128 VGPRs, two
+                               workgroups share a CU; 0 = automatic (2 when the batch has more workgroups than the
+                               device has CUs). Results do not depend on it. */
 } fbh_opts;
 
 typedef struct fbh_chain_out {
@@ -120,23 +123,24 @@ int fbh_batch_row_split(const fbh_batch* b);
  * reference runs one chain per process and has no launch to configure) */
 #define FBH_TUNE_MAX 8
 typedef struct fbh_tune {
-  int packing, tile_rows;          /* the configuration chosen (in effect on return) */
-  int ncand;                       /* candidates timed */
-  int cand_packing[FBH_TUNE_MAX], cand_tile_rows[FBH_TUNE_MAX];
+  int packing, tile_rows, chains_per_cu;  /* the configuration chosen (in effect on return) */
+  int ncand;                              /* candidates timed (both stages) */
+  int cand_packing[FBH_TUNE_MAX], cand_tile_rows[FBH_TUNE_MAX], cand_chains_per_cu[FBH_TUNE_MAX];
   double ms_per_sweep[FBH_TUNE_MAX];
-  int sweeps;                      /* sweeps the tuning ran: real sweeps of every chain, results unchanged */
+  int sweeps;                             /* sweeps the tuning ran: real sweeps of every chain, results unchanged */
 } fbh_tune;
-/* Switch packing (0, 1, 2 as fbh_opts.packing) and tile rows (1024, 1536, 0 = the automatic choice) between runs;
- * waits for the batch's queued work. Results do not depend on either. */
-int fbh_batch_configure(fbh_batch* b, int packing, int tile_rows);
+/* Switch packing (0, 1, 2 as fbh_opts.packing), tile rows (1024, 1536, 0 = the automatic choice) and the build
+ * (chains_per_cu 1 or 2, 0 = keep) between runs; waits for the batch's queued work. Results do not depend on them. */
+int fbh_batch_configure(fbh_batch* b, int packing, int tile_rows, int chains_per_cu);
 /* A packing of the caller's: logical chain order[w] runs on workgroup w (on workgroups w W .. w W + W - 1 with a row
  * split of W); order is a permutation of 0..nchains-1. Waits for the batch's queued work; results do not depend on it. */
 int fbh_batch_set_order(fbh_batch* b, const int* order);
 /* the configuration in effect (packing -1 after fbh_batch_set_order) */
-int fbh_batch_config(const fbh_batch* b, int* packing, int* tile_rows, int* graph);
-/* Time every (packing, tile rows) candidate on the batch's next sweeps -- one warm-up launch, then two launches of
- * trial_sweeps sweeps per candidate -- and keep the fastest. The sweeps are real sweeps of every chain and count
- * toward nskip + ndpost (FBH_EINVAL if too few are left); no candidate changes any result. out may be null. */
+int fbh_batch_config(const fbh_batch* b, int* packing, int* tile_rows, int* chains_per_cu, int* graph);
+/* Time launch configurations on the batch's next sweeps and keep the fastest: first build x tile rows at largest-first
+ * packing, then the three packings with the winner; one warm-up launch, then two launches of trial_sweeps sweeps per
+ * candidate. The sweeps are real sweeps of every chain and count toward nskip + ndpost (FBH_EINVAL if too few are
+ * left); no candidate changes any result. out may be null. */
 int fbh_batch_autotune(fbh_batch* b, int trial_sweeps, fbh_tune* out);
 /* Copy chain c's outputs to host buffers (synchronises). */
 int fbh_batch_fetch(fbh_batch* b, int chain, const fbh_chain_out* out);

@@ -1,5 +1,5 @@
-"""Launch configuration of the sweep kernel: the workgroup -> chain packing, the tile rows, graph capture and the
-autotuner that picks among them. Chains are independent and the state between launches is complete in device memory,
+"""Launch configuration of the sweep kernel: the workgroup -> chain packing, the tile rows, the build (one chain per
+CU at 168 VGPRs or two at 128), graph capture and the autotuner that picks among them. Chains are independent and the state between launches is complete in device memory,
 so none of these may change a result: every configuration, and a switch between configurations at any launch
 boundary, must reproduce the oracle's chains bit for bit."""
 import pytest
@@ -30,14 +30,15 @@ def _check_oracle(oracle, b, probs, m, nskip, nd, seeds, probit=0):
             assert same_bits(out[k], e[k]), (c, k)
 
 
-@pytest.mark.parametrize("packing,graph", [(1, 1), (2, 0), (0, 1)])
-def test_gpu_packing_and_graph_match_oracle(oracle, packing, graph):
+@pytest.mark.parametrize("packing,graph,per_cu", [(1, 1, 2), (2, 0, 1), (0, 1, 0)])
+def test_gpu_packing_and_graph_match_oracle(oracle, packing, graph, per_cu):
     probs = _probs()
     m, nskip, nd = 12, 14, 6
     seeds = [31 + c for c in range(len(probs))]
-    _say("packing", packing, "graph", graph)
-    with sampler.BatchSampler(probs, m, nskip, nd, seeds, packing=packing, graph=graph, sweeps_per_launch=4) as b:
-        assert b.config[0] == packing and b.config[2] == graph
+    _say("packing", packing, "graph", graph, "chains per CU", per_cu)
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds, packing=packing, graph=graph, sweeps_per_launch=4,
+                              chains_per_cu=per_cu) as b:
+        assert b.config["packing"] == packing and b.config["graph"] == graph
         b.run(7)
         b.run(nskip + nd - 7)
         b.sync()
@@ -51,25 +52,29 @@ def test_gpu_probit_packing_and_graph_match_oracle(oracle):
     m, nskip, nd = 8, 9, 5
     seeds = [41 + c for c in range(len(probs))]
     _say("probit, packing 1, graph")
-    with sampler.BatchSampler(probs, m, nskip, nd, seeds, probit=1, packing=1, graph=1, sweeps_per_launch=3) as b:
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds, probit=1, packing=1, graph=1, sweeps_per_launch=3,
+                              chains_per_cu=2) as b:
         b.run(nskip + nd)
         b.sync()
         _check_oracle(oracle, b, probs, m, nskip, nd, seeds, probit=1)
 
 
 def test_gpu_configuration_switched_between_launches(oracle):
-    """Packing and tile rows changed at every launch boundary: the state carried between launches is the same in
-    every configuration."""
+    """Packing, tile rows and build changed at every launch boundary: the state carried between launches is the same
+    in every configuration."""
     probs = _probs()
     m, nskip, nd = 12, 15, 6
     seeds = [51 + c for c in range(len(probs))]
-    cycle = [(0, 1024), (1, 1536), (2, 1024), (2, 1536), (1, 1024), (0, 1536), (1, 0)]
+    cycle = [(0, 1024, 1), (1, 1536, 2), (2, 1024, 2), (2, 1536, 1), (1, 1024, 0), (0, 1536, 2), (1, 0, 1)]
     _say("switched configurations")
     with sampler.BatchSampler(probs, m, nskip, nd, seeds, sweeps_per_launch=3) as b:
-        for k, (pk, tr) in enumerate(cycle):
-            b.configure(pk, tr)
+        assert b.config["chains_per_cu"] == 1  # automatic: six workgroups, fewer than the CUs
+        prev = 1
+        for k, (pk, tr, cp) in enumerate(cycle):
+            b.configure(pk, tr, cp)
             got = b.config
-            assert got[0] == pk and got[1] == (tr or 1536)
+            assert (got["packing"], got["tile_rows"], got["chains_per_cu"]) == (pk, tr or 1536, cp or prev)
+            prev = got["chains_per_cu"]
             b.run(3)
         b.sync()
         assert b.it == nskip + nd
@@ -83,7 +88,7 @@ def test_gpu_caller_order_matches_oracle(oracle):
     _say("caller's order")
     with sampler.BatchSampler(probs, m, nskip, nd, seeds, sweeps_per_launch=3) as b:
         b.set_order([3, 0, 5, 1, 4, 2])
-        assert b.config[0] == -1
+        assert b.config["packing"] == -1
         b.run(8)
         b.set_order([5, 4, 3, 2, 1, 0])
         b.run(nskip + nd - 8)
@@ -101,14 +106,21 @@ def test_gpu_autotune_runs_real_sweeps_and_keeps_results(oracle):
     _say("autotune")
     with sampler.BatchSampler(probs, m, nskip, nd, seeds, sweeps_per_launch=5, graph=1) as b:
         t = b.autotune(1)
-        assert len(t["candidates"]) == 6  # three packings x two tile sizes (the running fit fits in LDS here)
-        assert t["sweeps"] == 13 and b.it == 13
+        # stage 1: two tile sizes at one chain per CU (six workgroups, fewer than the CUs); stage 2: three packings
+        c1, c2 = t["candidates"][:2], t["candidates"][2:]
+        assert len(c1) == 2 and len(c2) == 3
+        assert {c["tile_rows"] for c in c1} == {1024, 1536} and all(c["chains_per_cu"] == 1 for c in c1)
+        assert [c["packing"] for c in c2] == [0, 1, 2]
+        assert t["sweeps"] == 11 and b.it == 11
         assert all(c["ms_per_sweep"] > 0 for c in t["candidates"])
-        best = min(t["candidates"], key=lambda c: c["ms_per_sweep"])
-        assert (t["packing"], t["tile_rows"]) == (best["packing"], best["tile_rows"])
-        assert b.config[:2] == (t["packing"], t["tile_rows"])
-        assert b.config[2] == 1  # graph capture is left as it was
-        b.run(nskip + nd - 13)
+        w1 = min(c1, key=lambda c: c["ms_per_sweep"])
+        assert all(c["tile_rows"] == w1["tile_rows"] for c in c2)
+        best = min(c2, key=lambda c: c["ms_per_sweep"])
+        assert (t["packing"], t["tile_rows"], t["chains_per_cu"]) == (best["packing"], best["tile_rows"], 1)
+        cfg = b.config
+        assert (cfg["packing"], cfg["tile_rows"], cfg["chains_per_cu"]) == (t["packing"], t["tile_rows"], 1)
+        assert cfg["graph"] == 1  # graph capture is left as it was
+        b.run(nskip + nd - 11)
         b.sync()
         _check_oracle(oracle, b, probs, m, nskip, nd, seeds)
 
@@ -121,14 +133,18 @@ def test_gpu_launch_config_rejections():
         with pytest.raises(FbhError):
             b.configure(0, 512)
         with pytest.raises(FbhError):
-            b.autotune(1)  # 13 sweeps needed, 6 left
+            b.configure(0, 0, 3)
+        with pytest.raises(FbhError):
+            b.autotune(1)  # 11 sweeps needed, 6 left
         assert b.it == 0
     big = [W.small_problem(12000, 2, 10, 7, "uniform", ntree=4)]
     with sampler.BatchSampler(big, 4, 2, 1, [1]) as b:
-        assert b.config[1] == 1024  # the running fit does not fit in LDS
+        assert b.config["tile_rows"] == 1024  # the running fit does not fit in LDS
         with pytest.raises(FbhError):
             b.configure(0, 1536)
         with pytest.raises(FbhError):
-            b.autotune(1)  # three candidates (no 1536-row tiles): 7 sweeps needed, 3 left
+            b.autotune(1)  # one stage-1 candidate: 9 sweeps needed, 3 left
     with pytest.raises(FbhError):
         sampler.BatchSampler(big, 4, 2, 1, [1], packing=5)
+    with pytest.raises(FbhError):
+        sampler.BatchSampler(big, 4, 2, 1, [1], chains_per_cu=3)

@@ -52,7 +52,8 @@ struct ChainDesc {
   // row split: this workgroup's chain is member `member` of `split` workgroups sharing logical chain `group`, whose
   // rows number n_total (n is this member's share); keep = 1 if it writes the kept trees
   int n_total, split, member, group, keep;
-  int pad_[3];
+  int ns;  // stride of the q and leaf-id columns: n rounded up to a multiple of 16 (16-byte aligned columns)
+  int pad_[2];
 };
 
 // Mutable per-chain sampler scalars carried between launches.

@@ -20,7 +20,7 @@ namespace fbh {
 // walked with q <= c going left. The draw's words are read through the cache (every thread of the draw reads the same
 // words).
 __global__ __launch_bounds__(256) void k_yhat(const uint64_t* __restrict__ trees, const uint8_t* __restrict__ q, int n,
-                                              int m, double* __restrict__ out) {
+                                              int ldq, int m, double* __restrict__ out) {
   const int dr = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -31,7 +31,7 @@ __global__ __launch_bounds__(256) void k_yhat(const uint64_t* __restrict__ trees
     uint64_t w = T[0];
     for (int step = 0; step < kNodes && ((w >> 16) & 255); step++) {
       const int v = (int)(w & 255), c = (int)((w >> 8) & 255);
-      const int s = q[(size_t)v * n + i] <= c ? (int)((w >> 16) & 255) : (int)((w >> 24) & 255);
+      const int s = q[(size_t)v * ldq + i] <= c ? (int)((w >> 16) & 255) : (int)((w >> 24) & 255);
       w = T[s];
     }
     f += (double)__uint_as_float((uint32_t)(w >> 32));
@@ -218,13 +218,14 @@ using namespace fbh;
 
 // ==================================================================== yhat
 namespace fbh {
-int yhat_device(const uint64_t* d_trees, const uint8_t* d_q, int n, int ndpost, int m, hipStream_t stream,
+// q [p][ldq] (column v at q + v ldq)
+int yhat_device(const uint64_t* d_trees, const uint8_t* d_q, int n, int ldq, int ndpost, int m, hipStream_t stream,
                 double* host_out) {
   if (ndpost == 0 || n == 0) return FBH_OK;
   double* d_out = nullptr;
   FBH_HIP_CHECK(hipMalloc(&d_out, sizeof(double) * (size_t)ndpost * n));
   dim3 grid((n + 255) / 256, ndpost);
-  hipLaunchKernelGGL(k_yhat, grid, dim3(256), 0, stream, d_trees, d_q, n, m, d_out);
+  hipLaunchKernelGGL(k_yhat, grid, dim3(256), 0, stream, d_trees, d_q, n, ldq, m, d_out);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e == hipSuccess)
@@ -261,7 +262,7 @@ extern "C" int fbh_yhat(const uint64_t* trees, const uint8_t* q, int n, int p, i
   FBH_HIP_CHECK(hipMalloc(&dq, (size_t)p * n));
   FBH_HIP_CHECK(hipMemcpy(dt, trees, sizeof(uint64_t) * nw, hipMemcpyHostToDevice));
   FBH_HIP_CHECK(hipMemcpy(dq, q, (size_t)p * n, hipMemcpyHostToDevice));
-  rc = yhat_device(dt, dq, n, ndpost, m, nullptr, yhat);
+  rc = yhat_device(dt, dq, n, n, ndpost, m, nullptr, yhat);
   (void)hipFree(dt);
   (void)hipFree(dq);
   return rc;

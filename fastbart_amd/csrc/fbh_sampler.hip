@@ -539,6 +539,7 @@ struct RowPtrs {
   const uint8_t* Q;
   uint8_t* LEAF;
   int n, probit;
+  int ns;  // column stride of Q and LEAF
 };
 
 // Walk one lane's rows in row order, adding src[i] to the task's sum where the task matches the row's tag: bit f of the
@@ -610,38 +611,64 @@ __device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, cons
 
 __device__ __forceinline__ long long fx_round(double x, double scale) { return (long long)rint(x * scale); }
 
-// One phase-A thread's loads of a tile, kept raw (one register per byte) so that nothing consumes them before phase A:
-// the loads of tile t+1 are issued before phase B of tile t and arrive while it runs. YF: the outcome is read as the
-// float the caller passed (wbart; (double) of it is the reference's y), else the double latent (probit). AFT: the
-// running fit is part of the tile (streamed from HBM); when it is resident in LDS, phase A reads it with its gathers.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+struct D4 {  // four consecutive doubles
+  d2v a, b;
+};
+__device__ __forceinline__ double d4_at(const D4& v, int u) { return u < 2 ? v.a[u] : v.b[u - 2]; }
+
+// One phase-A thread's loads of a tile. Thread t of data wave dw takes the row slots t + 64 s of the tile (S of them)
+// and, in each, the rows of the wave's four summation lanes -- four consecutive rows, so one dword of every byte column,
+// one 16-byte vector of every float column and two of every double column (columns are 16-byte aligned). Kept raw so
+// that nothing consumes them before phase A: the loads of tile t+1 are issued before phase B of tile t and arrive while
+// it runs. YF: the outcome is read as the float the caller passed (wbart; (double) of it is the reference's y), else
+// the double latent (probit). AFT: the running fit is part of the tile (streamed from HBM); when it is resident in
+// LDS, phase A reads it with its gathers.
 template <int R, bool YF = false, bool AFT = true>
 struct TileIn {
-  uint32_t la[R], lb[R], qa[R], qb[R];  // leaf id of tree j-1, of tree j; rank on v_prev, on v_cur
-  float w[R];
-  typename std::conditional<YF, float, double>::type y[R];
-  double af[AFT ? R : 1];
+  static constexpr int S = (16 * R + 63) / 64;  // row slots per thread
+  uint32_t la[S], lb[S], qa[S], qb[S];          // leaf id of tree j-1, of tree j; rank on v_prev, on v_cur (a byte a row)
+  f4v w[S];
+  typename std::conditional<YF, f4v, D4>::type y[S];
+  D4 af[AFT ? S : 1];
 };
 
+// The first row of the thread's quad in slot s of the tile starting at row slot ii0: tq = 16 t + 4 dw. Addresses are
+// clamped to qlast, the wave's last quad that starts below n, so that every load stays inside the chain's columns
+// (padded to a multiple of 16 rows); a clamped quad's values are not used.
 template <int R, bool RES, bool YF, bool AFT>
 __device__ __forceinline__ void tile_load(TileIn<R, YF, AFT>& T, const RowPtrs& P, const uint8_t* __restrict__ la,
                                           const uint8_t* __restrict__ lb, const uint8_t* __restrict__ qa,
-                                          const uint8_t* __restrict__ qb, int ii0, int iis, int ka) {
-  const uint32_t base = opaque((uint32_t)(iis * kLanes + ka));  // row (ii0 + iis + 16 r) * 16 + ka
+                                          const uint8_t* __restrict__ qb, int ii0, uint32_t tq, uint32_t qlast) {
+  constexpr int S = TileIn<R, YF, AFT>::S;
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    const uint32_t i = min(base + (uint32_t)((ii0 + 16 * r) * kLanes), (uint32_t)(P.n - 1));  // past n: row n-1
-    T.la[r] = ldo(la, i);
-    T.lb[r] = ldo(lb, i);
-    T.qa[r] = ldo(qa, i);
-    T.qb[r] = ldo(qb, i);
-    if (YF) T.y[r] = ldo(P.yf, i);
-    else T.y[r] = ldo(P.Y, i);
-    T.w[r] = ldo(P.W, i);
-    if (AFT) T.af[r] = RES ? P.AFl[afl(i)] : ldo((const double*)P.AF, i);
+  for (int s = 0; s < S; s++) {
+    const uint32_t r0 = min((uint32_t)((ii0 + 64 * s) * kLanes) + tq, qlast);
+    T.la[s] = ldo(reinterpret_cast<const uint32_t*>(la), r0 >> 2);
+    T.lb[s] = ldo(reinterpret_cast<const uint32_t*>(lb), r0 >> 2);
+    T.qa[s] = ldo(reinterpret_cast<const uint32_t*>(qa), r0 >> 2);
+    T.qb[s] = ldo(reinterpret_cast<const uint32_t*>(qb), r0 >> 2);
+    T.w[s] = ldo(reinterpret_cast<const f4v*>(P.W), r0 >> 2);
+    if constexpr (YF) {
+      T.y[s] = ldo(reinterpret_cast<const f4v*>(P.yf), r0 >> 2);
+    } else {
+      T.y[s].a = ldo(reinterpret_cast<const d2v*>(P.Y), r0 >> 1);
+      T.y[s].b = ldo(reinterpret_cast<const d2v*>(P.Y), (r0 >> 1) + 1);
+    }
+    if constexpr (AFT) {
+      T.af[s].a = ldo(reinterpret_cast<const d2v*>(P.AF), r0 >> 1);
+      T.af[s].b = ldo(reinterpret_cast<const d2v*>(P.AF), (r0 >> 1) + 1);
+    }
   }
 }
 
-// D: the dense build (two chains per CU, 128 VGPRs), which gathers one row at a time in phase A
+// the wave's last quad of rows that starts below n (its first quad when the chain has no row in the wave's lanes)
+__device__ __forceinline__ uint32_t last_quad(int n, int dw) {
+  const int k0 = kLanesPerWave * dw;
+  return n > k0 ? (uint32_t)(((n - 1 - k0) & ~(kLanes - 1)) + k0) : (uint32_t)k0;
+}
+
 template <int R, bool RES, bool FX, bool D = false, bool YF, bool AFT>
 __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int dw, int lane, int jp, int j, int cb,
                           bool apply, bool stats, bool rss, bool keep, TileIn<R, YF, AFT>& T, bool first_loaded) {
@@ -654,10 +681,11 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
   const int cmode = stats ? (int)ufl(c.cmode[cb]) : 0, cnx = ufl(c.cnx[cb]), cc = ufl(c.cc[cb]), cv = ufl(c.cv[cb]);
   const int nleaf = ufl(c.nleaf[cb]);
   const int ntask = rss ? 2 : 2 * (nleaf + (cmode == 1 ? 2 : 0));
-  uint8_t* __restrict__ lprev = apply ? P.LEAF + (size_t)jp * n : nullptr;
-  const uint8_t* __restrict__ lcur = stats ? P.LEAF + (size_t)j * n : nullptr;
-  const uint8_t* __restrict__ qp = (apply && pmode == 1) ? P.Q + (size_t)pv * n : nullptr;
-  const uint8_t* __restrict__ qc = (stats && cmode == 1) ? P.Q + (size_t)cv * n : nullptr;
+  const size_t ns = (size_t)P.ns;
+  uint8_t* __restrict__ lprev = apply ? P.LEAF + (size_t)jp * ns : nullptr;
+  const uint8_t* __restrict__ lcur = stats ? P.LEAF + (size_t)j * ns : nullptr;
+  const uint8_t* __restrict__ qp = (apply && pmode == 1) ? P.Q + (size_t)pv * ns : nullptr;
+  const uint8_t* __restrict__ qc = (stats && cmode == 1) ? P.Q + (size_t)cv * ns : nullptr;
   // unused streams read a valid dummy column so that every load is unconditional
   const uint8_t* __restrict__ la = lprev ? lprev : P.LEAF;
   const uint8_t* __restrict__ lb = lcur ? lcur : P.LEAF;
@@ -666,15 +694,17 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
   const double* __restrict__ thc = c.thc[cb];
   const uint8_t* __restrict__ rank_of = c.rank_of[cb];
 
-  // phase-A coordinates: local lane al, row slot iis; phase-B coordinates: local lane bl, task slot
-  const int al = lane & 3, iis = lane >> 2, ka = kLanesPerWave * dw + al;
+  // phase-A coordinates: the thread's row quad (16 lane + 4 dw: the wave's four lanes in a row slot); phase-B
+  // coordinates: local lane bl, task slot
+  const uint32_t tq = opaque((uint32_t)(kLanes * lane + kLanesPerWave * dw));
+  const uint32_t qlast = last_quad(n, dw);
   const int bl = lane >> 4, slot = lane & 15, kb = kLanesPerWave * dw + bl;
   double* v0 = reinterpret_cast<double*>(scratch);
   double* v1 = v0 + kLanes * G::kStride + G::kV1Pad;
   uint8_t* tg = reinterpret_cast<uint8_t*>(v1 + kLanes * G::kStride);
-  double* v0a = v0 + (kLanesPerWave * dw + al) * G::kStride;
-  double* v1a = v1 + (kLanesPerWave * dw + al) * G::kStride;
-  uint8_t* tga = tg + (kLanesPerWave * dw + al) * G::kTagStride;
+  double* v0a = v0 + kLanesPerWave * dw * G::kStride;  // lane 4 dw + u of the quad at + u kStride
+  double* v1a = v1 + kLanesPerWave * dw * G::kStride;
+  uint8_t* tga = tg + kLanesPerWave * dw * G::kTagStride;
   const double* v0b = v0 + kb * G::kStride;
   const double* v1b = v1 + kb * G::kStride;
   const uint8_t* tgb = tg + kb * G::kTagStride;
@@ -685,83 +715,98 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
   const int nper_kb = kb < n ? (n - kb + kLanes - 1) / kLanes : 0;
   int my_nl = 0, my_nr = 0;
 
-  if (!first_loaded) tile_load<R, RES>(T, P, la, lb, qa, qb, 0, iis, ka);
+  if (!first_loaded) tile_load<R, RES>(T, P, la, lb, qa, qb, 0, tq, qlast);
   for (int tb = 0; tb < n; tb += G::kTile) {
     const int ii0 = tb / kLanes;
-    // ---- phase A: the arithmetic on the tile's loads (issued before the previous tile's phase B). First the leaf ids
-    // after tree j-1's change and every LDS gather of the tile's rows (leaf values, ranks), all in flight together;
-    // then the arithmetic and the stores. Rows past n compute on row n-1's loads and store nothing global.
-    const uint32_t abase = opaque((uint32_t)(iis * kLanes + ka));
-    constexpr int RC = D ? 1 : (R % 3 == 0) ? 3 : 2;  // rows per gather batch (batches keep the registers in budget)
+    // ---- phase A: the arithmetic on the tile's loads (issued before the previous tile's phase B). Per batch of RC rows
+    // of a quad, first the leaf ids after tree j-1's change and every LDS gather (leaf values, ranks, the resident
+    // fit), all in flight together; then the arithmetic and the stores. Rows past n compute on the loaded values and
+    // store nothing global; a quad past n, or a slot past the tile, does nothing.
+    constexpr int S = TileIn<R, YF, AFT>::S;
+    constexpr int RC = D ? 1 : 2;  // rows per gather batch (batches keep the registers in budget)
 #pragma unroll
-    for (int r0 = 0; r0 < R; r0 += RC) {
-      // a batch whose first row for this wave is past n (the last tile) is skipped, and so are the later ones
-      if ((uint32_t)((ii0 + 16 * r0) * kLanes + kLanesPerWave * dw) >= (uint32_t)n) break;
-      int lfn[RC];
-      double tho_r[RC], thn_r[RC], thc_r[RC], af_r[RC];
-      uint32_t rank_r[RC];
+    for (int s = 0; s < S; s++) {
+      const int idx = lane + 64 * s;  // the slot within the tile
+      const uint32_t r0 = (uint32_t)((ii0 + 64 * s) * kLanes) + tq;
+      if (idx < G::kTW4 && r0 < (uint32_t)n) {
+        uint32_t lnew = 0;  // the quad's leaf ids after the change, a byte a row
 #pragma unroll
-      for (int u = 0; u < RC; u++) {
-        const int r = r0 + u;
-        const int lf = (int)T.la[r];
-        int lf2 = lf;
-        if (pmode == 1)
-          lf2 = lf == pnx ? ((int)T.qa[r] <= pc ? pL : pR) : lf;
-        else if (pmode == 2)
-          lf2 = (lf == pl || lf == pr) ? pnx : lf;
-        lfn[u] = lf2;
-        if (apply) {
-          tho_r[u] = c.tho[lf];
-          thn_r[u] = c.thn[lf2];
-        }
-        if (!rss) {
-          thc_r[u] = thc[T.lb[r]];
-          rank_r[u] = rank_of[T.lb[r]];
-        }
-        if (AFT) {
-          af_r[u] = T.af[r];
-        } else {
-          const uint32_t i = min(abase + (uint32_t)((ii0 + 16 * r) * kLanes), (uint32_t)(n - 1));
-          af_r[u] = P.AFl[afl(i)];
-        }
-      }
+        for (int u0 = 0; u0 < kLanesPerWave; u0 += RC) {
+          int lfn[RC];
+          double tho_r[RC], thn_r[RC], thc_r[RC], af_r[RC];
+          uint32_t rank_r[RC];
 #pragma unroll
-      for (int u = 0; u < RC; u++) {
-        const int r = r0 + u;
-        const int idx = iis + 16 * r;
-        const uint32_t i = abase + (uint32_t)((ii0 + 16 * r) * kLanes);
-        const bool valid = i < (uint32_t)n;
-        double af = af_r[u];
-        if (apply) {
-          af = af - tho_r[u] + thn_r[u];
-          if (valid) {
-            sto(lprev, i, (uint8_t)lfn[u]);
-            if (RES) P.AFl[afl(i)] = af;
-            else sto(P.AF, i, af);
+          for (int k = 0; k < RC; k++) {
+            const int u = u0 + k;
+            const int lf = (int)((T.la[s] >> (8 * u)) & 255u);
+            int lf2 = lf;
+            if (pmode == 1)
+              lf2 = lf == pnx ? ((int)((T.qa[s] >> (8 * u)) & 255u) <= pc ? pL : pR) : lf;
+            else if (pmode == 2)
+              lf2 = (lf == pl || lf == pr) ? pnx : lf;
+            lfn[k] = lf2;
+            if (apply) {
+              tho_r[k] = c.tho[lf];
+              thn_r[k] = c.thn[lf2];
+            }
+            if (!rss) {
+              const uint32_t lb_u = (T.lb[s] >> (8 * u)) & 255u;
+              thc_r[k] = thc[lb_u];
+              rank_r[k] = rank_of[lb_u];
+            }
+            if constexpr (AFT) af_r[k] = d4_at(T.af[s], u);
+            else af_r[k] = P.AFl[afl(min(r0 + (uint32_t)u, (uint32_t)(n - 1)))];
+          }
+#pragma unroll
+          for (int k = 0; k < RC; k++) {
+            const int u = u0 + k;
+            const uint32_t i = r0 + (uint32_t)u;
+            const bool valid = i < (uint32_t)n;
+            double af = af_r[k];
+            if (apply) {
+              af = af - tho_r[k] + thn_r[k];
+              lnew |= (uint32_t)lfn[k] << (8 * u);
+              if (valid) {
+                if (RES) P.AFl[afl(i)] = af;
+                else sto(P.AF, i, af);
+              }
+            }
+            double y;
+            if constexpr (YF) y = (double)T.y[s][u];
+            else y = d4_at(T.y[s], u);
+            const double w = (double)T.w[s][u];
+            if (rss) {
+              const double e = y - af;
+              const double c2 = (w * e) * e;
+              v0a[u * G::kStride + idx] = 0.0;
+              v1a[u * G::kStride + idx] = FX ? __longlong_as_double(fx_round(c2, 16777216.0)) : c2;
+              tga[u * G::kTagStride + idx] = 0;
+              if (keep && valid) sto(P.TM, i, ldo((const double*)P.TM, i) + af);
+            } else {
+              const int lf = (int)((T.lb[s] >> (8 * u)) & 255u);
+              const double rr = y - af + thc_r[k];
+              const double wr = w * rr;
+              int side = 0;
+              if (cmode == 1 && lf == cnx && valid) {
+                side = (int)((T.qb[s] >> (8 * u)) & 255u) <= cc ? 1 : 2;
+                my_nl += side == 1;
+                my_nr += side == 2;
+              }
+              v0a[u * G::kStride + idx] = FX ? __longlong_as_double(fx_round(w, 4294967296.0)) : w;
+              v1a[u * G::kStride + idx] = FX ? __longlong_as_double(fx_round(wr, 4294967296.0)) : wr;
+              tga[u * G::kTagStride + idx] = (uint8_t)(rank_r[k] | (side << 6));
+            }
           }
         }
-        const double y = (double)T.y[r];
-        const double w = (double)T.w[r];
-        if (rss) {
-          const double e = y - af;
-          const double c2 = (w * e) * e;
-          v0a[idx] = 0.0;
-          v1a[idx] = FX ? __longlong_as_double(fx_round(c2, 16777216.0)) : c2;
-          tga[idx] = 0;
-          if (keep && valid) sto(P.TM, i, ldo((const double*)P.TM, i) + af);
-        } else {
-          const int lf = (int)T.lb[r];
-          const double rr = y - af + thc_r[u];
-          const double wr = w * rr;
-          int side = 0;
-          if (cmode == 1 && lf == cnx && valid) {
-            side = (int)T.qb[r] <= cc ? 1 : 2;
-            my_nl += side == 1;
-            my_nr += side == 2;
+        // the quad's new leaf ids: one dword store when the whole quad is below n, else a byte per row
+        if (apply) {
+          if (r0 + 3u < (uint32_t)n) {
+            sto(reinterpret_cast<uint32_t*>(lprev), r0 >> 2, lnew);
+          } else {
+#pragma unroll
+            for (int u = 0; u < kLanesPerWave; u++)
+              if (r0 + (uint32_t)u < (uint32_t)n) sto(lprev, r0 + (uint32_t)u, (uint8_t)(lnew >> (8 * u)));
           }
-          v0a[idx] = FX ? __longlong_as_double(fx_round(w, 4294967296.0)) : w;
-          v1a[idx] = FX ? __longlong_as_double(fx_round(wr, 4294967296.0)) : wr;
-          tga[idx] = (uint8_t)(rank_r[u] | (side << 6));
         }
       }
     }
@@ -770,7 +815,7 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
     // sums w, kind 1 sums w*r; ranks below nleaf are leaves (rank field), then the split halves (side field 1 and
     // 2); the closing pass sums every row (an empty field matches bit 0). A slot past ntask sums nothing.
     // the next tile's loads go out now and arrive during phase B
-    if (tb + G::kTile < n) tile_load<R, RES>(T, P, la, lb, qa, qb, ii0 + G::kTW4, iis, ka);
+    if (tb + G::kTile < n) tile_load<R, RES>(T, P, la, lb, qa, qb, ii0 + G::kTW4, tq, qlast);
     const int cnt = min(G::kTW4, max(0, nper_kb - ii0));
 #pragma unroll
     for (int g = 0; g < kGroups; g++) {
@@ -803,13 +848,14 @@ __device__ __forceinline__ void load_next_first_tile(TileIn<R, YF, AFT>& T, cons
                                                      int lane,
                                                      int j, int m, int cb) {
   const int n = P.n;
-  const int iis = lane >> 2, ka = kLanesPerWave * dw + (lane & 3);
+  const uint32_t tq = opaque((uint32_t)(kLanes * lane + kLanesPerWave * dw));
   const bool last = j + 1 == m;
-  const uint8_t* la = P.LEAF + (size_t)j * n;
-  const uint8_t* lb = last ? P.LEAF : P.LEAF + (size_t)(j + 1) * n;
-  const uint8_t* qa = ufl(c.cmode[cb]) == 1 ? P.Q + (size_t)ufl(c.cv[cb]) * n : P.Q;
-  const uint8_t* qb = (!last && ufl(c.cmode[cb ^ 1]) == 1) ? P.Q + (size_t)ufl(c.cv[cb ^ 1]) * n : P.Q;
-  tile_load<R, RES>(T, P, la, lb, qa, qb, 0, iis, ka);
+  const size_t ns = (size_t)P.ns;
+  const uint8_t* la = P.LEAF + (size_t)j * ns;
+  const uint8_t* lb = last ? P.LEAF : P.LEAF + (size_t)(j + 1) * ns;
+  const uint8_t* qa = ufl(c.cmode[cb]) == 1 ? P.Q + (size_t)ufl(c.cv[cb]) * ns : P.Q;
+  const uint8_t* qb = (!last && ufl(c.cmode[cb ^ 1]) == 1) ? P.Q + (size_t)ufl(c.cv[cb ^ 1]) * ns : P.Q;
+  tile_load<R, RES>(T, P, la, lb, qa, qb, 0, tq, last_quad(n, dw));
 }
 
 // ------------------------------------------------------------------ probit latent redraw (all waves)
@@ -1121,6 +1167,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
   P.LEAF = a.leaf + ufl_t(c.cd.leaf_off);
   P.n = n;
   P.probit = a.probit;
+  P.ns = ufl(c.cd.ns);
   TreeDev* TR = a.trees + c.cd.tree_off;
   ChainState* cs = a.state + chain;
   const LibmTables tb = a.tb;
@@ -1551,6 +1598,7 @@ __device__ __forceinline__ RowPtrs lp_rows(const KArgs& a, const ChainDesc& cd) 
   P.LEAF = a.leaf + cd.leaf_off;
   P.n = cd.n;
   P.probit = 0;
+  P.ns = cd.ns;
   return P;
 }
 
@@ -1558,12 +1606,12 @@ __device__ __forceinline__ RowPtrs lp_rows(const KArgs& a, const ChainDesc& cd) 
 // them); each data wave takes the rows of its own summation lanes, as the data pass does
 __device__ void lp_afx_apply(const Ctl& c, const RowPtrs& P, long long* __restrict__ AFX, int jp, int dw, int lane) {
   const int n = P.n;
-  const uint8_t* lp = P.LEAF + (size_t)jp * n;
+  const uint8_t* lp = P.LEAF + (size_t)jp * P.ns;
   for (int i = 16 * (lane >> 2) + kLanesPerWave * dw + (lane & 3); i < n; i += 256) {
     const int lf = lp[i];
     int lf2 = lf;
     if (c.pmode == 1 && lf == c.pnx)
-      lf2 = (int)P.Q[(size_t)c.pv * n + i] <= c.pc ? c.pL : c.pR;
+      lf2 = (int)P.Q[(size_t)c.pv * P.ns + i] <= c.pc ? c.pL : c.pR;
     else if (c.pmode == 2 && (lf == c.pl || lf == c.pr))
       lf2 = c.pnx;
     AFX[i] += fit_fx(c.thn[lf2]) - fit_fx(c.tho[lf]);
@@ -1662,14 +1710,14 @@ __global__ __launch_bounds__(kBlock) void k_lp_stage1(LpArgs A) {
     // the node's rows: the leaf nx (birth) or the two children of nx (death)
     const TreeDev& t = c.tree[0];
     const int l = mode == 2 ? t.l[nx] : nx, r = mode == 2 ? t.r[nx] : nx;
-    const uint8_t* lc = P.LEAF + (size_t)j * n;
+    const uint8_t* lc = P.LEAF + (size_t)j * P.ns;
     for (int i = tid; i < n; i += kBlock) {
       const int lf = lc[i];
       if (lf != l && lf != r) continue;
       const double res = (double)P.Y[i] - (double)(AFX[i] - fit_fx(c.thc[0][lf])) * (1.0 / kFxFit);
       const double w = (double)P.W[i];
       const unsigned long long wr = (unsigned long long)(long long)rint(w * res * kFxFeat);
-      for (int v = 0; v < p; v++) atomicAdd(&H[v * kFeatBins + (P.Q[(size_t)v * n + i] >> 4)], wr);
+      for (int v = 0; v < p; v++) atomicAdd(&H[v * kFeatBins + (P.Q[(size_t)v * P.ns + i] >> 4)], wr);
       atomicAdd(&H[kFeat - 1], (unsigned long long)(long long)rint(w * kFxFeat));
     }
   }
@@ -2100,7 +2148,7 @@ __global__ __launch_bounds__(kBlock) void k_lp_begin(LpArgs A) {
   long long* AFX = A.AFX + c.cd.row_off;
   for (int i = tid; i < n; i += kBlock) {
     long long s = 0;
-    for (int jj = 0; jj < m; jj++) s += fit_fx((double)TR[jj].th[LEAF[(size_t)jj * n + i]]);
+    for (int jj = 0; jj < m; jj++) s += fit_fx((double)TR[jj].th[LEAF[(size_t)jj * c.cd.ns + i]]);
     AFX[i] = s;
   }
   __syncthreads();
@@ -2423,6 +2471,7 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     ChainDesc& d = b->hdesc[dc];
     std::memset(&d, 0, sizeof d);
     d.n = r1 - r0;
+    d.ns = (d.n + 15) / 16 * 16;
     d.n_total = ci.n;
     d.split = W;
     d.member = k;
@@ -2447,9 +2496,9 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     d.tree_off = (uint64_t)dc * b->m;
     d.sig_off = (uint64_t)dc * total;
     d.tout_off = (uint64_t)c * b->ndpost * b->m * kNodes;  // only member 0 writes the kept trees
-    qo += (size_t)ci.p * d.n;
+    qo += (size_t)ci.p * d.ns;
     ro += ((size_t)d.n + 63) / 64 * 64;  // keep each chain's rows 512-byte aligned
-    lo += ((size_t)b->m * d.n + 255) / 256 * 256;
+    lo += ((size_t)b->m * d.ns + 255) / 256 * 256;
     b->bytes_per_sweep += 37.0 * d.n * (b->m + 1);
   }
   b->q_total = qo;
@@ -2544,7 +2593,7 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
       const ChainDesc& d = b->hdesc[dc];
       const int r0 = b->row0[dc];
       for (int v = 0; v < ci.p; v++)
-        std::memcpy(hq.data() + d.q_off + (size_t)v * d.n, ci.q + (size_t)v * ci.n + r0, (size_t)d.n);
+        std::memcpy(hq.data() + d.q_off + (size_t)v * d.ns, ci.q + (size_t)v * ci.n + r0, (size_t)d.n);
       std::memcpy(hw.data() + d.row_off, ci.wt + r0, sizeof(float) * d.n);
       if (!o->probit) std::memcpy(hy.data() + d.row_off, ci.y + r0, sizeof(float) * d.n);
       if (o->probit)
@@ -2573,6 +2622,9 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     cp(d_seeds, seeds.data(), sizeof(unsigned) * nchains);
     cp(b->d_order, b->order.data(), sizeof(int) * nchains);
     if (e == hipSuccess) e = hipMemsetAsync(b->d_leaf, 0, lo, b->stream);
+    // rows past n in the row pools are read (never used) by the last quad of rows of a tile: keep them zero
+    if (e == hipSuccess) e = hipMemsetAsync(b->d_AF, 0, sizeof(double) * ro, b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b->d_Y, 0, sizeof(double) * ro, b->stream);
     if (e == hipSuccess) {
       KArgs a = make_args(b);
       hipLaunchKernelGGL(k_init, dim3(nchains), dim3(256), 0, b->stream, a, d_seeds);
@@ -2880,7 +2932,7 @@ extern "C" int fbh_batch_state(fbh_batch* b, int c, uint32_t* rng, double* sigma
       FBH_HIP_CHECK(hipMemcpy(latent + r0, b->d_Y + dk.row_off, sizeof(double) * dk.n, hipMemcpyDeviceToHost));
     if (leaf)
       for (int j = 0; j < b->m; j++)
-        FBH_HIP_CHECK(hipMemcpy(leaf + (size_t)j * n_all + r0, b->d_leaf + dk.leaf_off + (size_t)j * dk.n, dk.n,
+        FBH_HIP_CHECK(hipMemcpy(leaf + (size_t)j * n_all + r0, b->d_leaf + dk.leaf_off + (size_t)j * dk.ns, dk.n,
                                 hipMemcpyDeviceToHost));
   }
   if (words) {
@@ -2934,7 +2986,7 @@ extern "C" void fbh_batch_destroy(fbh_batch* b) { free_batch(b); }
 
 // the device yhat walk (fbh_predict.hip)
 namespace fbh {
-int yhat_device(const uint64_t* d_trees, const uint8_t* d_q, int n, int ndpost, int m, hipStream_t stream,
+int yhat_device(const uint64_t* d_trees, const uint8_t* d_q, int n, int ldq, int ndpost, int m, hipStream_t stream,
                 double* host_out);
 }
 
@@ -2948,15 +3000,15 @@ extern "C" int fbh_batch_yhat(fbh_batch* b, int c, double* yhat) {
   const int d0 = c * b->W;
   const ChainDesc& d = b->hdesc[d0];
   if (b->W == 1)
-    return fbh::yhat_device(b->d_tout + d.tout_off, b->d_q + d.q_off, d.n, b->ndpost, b->m, b->stream, yhat);
+    return fbh::yhat_device(b->d_tout + d.tout_off, b->d_q + d.q_off, d.n, d.ns, b->ndpost, b->m, b->stream, yhat);
   // a row split: each member's rows against the chain's kept trees, placed at the member's row offset
   const int n_all = d.n_total;
   std::vector<double> part;
   for (int k = 0; k < b->W; k++) {
     const ChainDesc& dk = b->hdesc[d0 + k];
     part.resize((size_t)b->ndpost * dk.n);
-    const int rc = fbh::yhat_device(b->d_tout + d.tout_off, b->d_q + dk.q_off, dk.n, b->ndpost, b->m, b->stream,
-                                    part.data());
+    const int rc = fbh::yhat_device(b->d_tout + d.tout_off, b->d_q + dk.q_off, dk.n, dk.ns, b->ndpost, b->m,
+                                    b->stream, part.data());
     if (rc) return rc;
     for (int dr = 0; dr < b->ndpost; dr++)
       std::memcpy(yhat + (size_t)dr * n_all + b->row0[d0 + k], part.data() + (size_t)dr * dk.n, sizeof(double) * dk.n);

@@ -429,6 +429,8 @@ def main() -> int:
                     help="workgroup -> chain packing of the sweep kernel: 0 batch order, 1 serpentine, 2 largest "
                          "first, auto = the autotuner picks packing, tile rows and build during the warmup")
     ap.add_argument("--tile-rows", type=int, default=0, help="rows per LDS tile (0 automatic, 1024, 1536)")
+    ap.add_argument("--resident", type=int, default=0,
+                    help="running fit during a launch: 1 streamed, 2 in LDS, 0 automatic")
     ap.add_argument("--chains-per-cu", type=int, default=0,
                     help="sweep-kernel build: 1 (168 VGPRs) or 2 (128 VGPRs, two workgroups per CU); 0 automatic")
     ap.add_argument("--no-graph", action="store_true", help="plain stream launches instead of a captured hipGraph")
@@ -484,7 +486,8 @@ def main() -> int:
     b = sampler.BatchSampler(probs, M, nskip, ndpost, seeds, probit=probit, keep_trees=1, stats_mode=stats_mode,
                              node_cap=node_cap, sweeps_per_launch=spl, row_split=split,
                              packing=0 if tune else int(args.packing), tile_rows=args.tile_rows,
-                             graph=0 if args.no_graph else 1, chains_per_cu=args.chains_per_cu)
+                             graph=0 if args.no_graph else 1, chains_per_cu=args.chains_per_cu,
+                             resident=args.resident)
     split = b.row_split
     b.sync()
     t_w = time.perf_counter()
@@ -497,8 +500,8 @@ def main() -> int:
         except FbhError as e:
             log("autotune skipped:", e)
         if tuned:
-            log("autotune:", {k: tuned[k] for k in ("packing", "tile_rows", "chains_per_cu", "sweeps")},
-                [(c["packing"], c["tile_rows"], c["chains_per_cu"], round(c["ms_per_sweep"], 3))
+            log("autotune:", {k: tuned[k] for k in ("packing", "tile_rows", "resident", "chains_per_cu", "sweeps")},
+                [(c["packing"], c["tile_rows"], c["resident"], c["chains_per_cu"], round(c["ms_per_sweep"], 3))
                  for c in tuned["candidates"]])
     if W_ > b.it:
         b.run(W_ - b.it)
@@ -549,6 +552,7 @@ def main() -> int:
                    "rows_total": rows_total,
                    "parallelism": "chains" if split <= 1 else f"chains x {split} workgroups (row split)",
                    "packing": launch_cfg["packing"], "tile_rows": launch_cfg["tile_rows"],
+                   "fit_in_lds": launch_cfg["resident"] == 2,
                    "chains_per_cu": launch_cfg["chains_per_cu"], "graph": bool(launch_cfg["graph"]),
                    "autotuned": tuned is not None},
         "min_ess_per_sec": ess,

@@ -34,17 +34,17 @@ class Opts(C.Structure):
                 ("keep_trees", C.c_int), ("pgtab", C.c_void_p), ("stats_mode", C.c_int), ("node_cap", C.c_int),
                 ("device", C.c_int), ("sweeps_per_launch", C.c_int), ("block_threads", C.c_int),
                 ("tile_rows", C.c_int), ("row_split", C.c_int), ("packing", C.c_int), ("graph", C.c_int),
-                ("chains_per_cu", C.c_int)]
+                ("chains_per_cu", C.c_int), ("resident", C.c_int)]
 
 
-TUNE_MAX = 8
+TUNE_MAX = 12
 
 
 class Tune(C.Structure):
-    _fields_ = [("packing", C.c_int), ("tile_rows", C.c_int), ("chains_per_cu", C.c_int), ("ncand", C.c_int),
-                ("cand_packing", C.c_int * TUNE_MAX), ("cand_tile_rows", C.c_int * TUNE_MAX),
-                ("cand_chains_per_cu", C.c_int * TUNE_MAX), ("ms_per_sweep", C.c_double * TUNE_MAX),
-                ("sweeps", C.c_int)]
+    _fields_ = [("packing", C.c_int), ("tile_rows", C.c_int), ("resident", C.c_int), ("chains_per_cu", C.c_int),
+                ("ncand", C.c_int), ("cand_packing", C.c_int * TUNE_MAX), ("cand_tile_rows", C.c_int * TUNE_MAX),
+                ("cand_resident", C.c_int * TUNE_MAX), ("cand_chains_per_cu", C.c_int * TUNE_MAX),
+                ("ms_per_sweep", C.c_double * TUNE_MAX), ("sweeps", C.c_int)]
 
 
 class ChainOut(C.Structure):
@@ -123,9 +123,9 @@ def _declare(L):
     L.fbh_device_copy.argtypes = [vp, vp, C.c_size_t, i, i]
     L.fbh_device_sync.argtypes = [i]
     L.fbh_batch_row_split.argtypes = [vp]
-    L.fbh_batch_configure.argtypes = [vp, i, i, i]
+    L.fbh_batch_configure.argtypes = [vp, i, i, i, i]
     L.fbh_batch_set_order.argtypes = [vp, vp]
-    L.fbh_batch_config.argtypes = [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)]
+    L.fbh_batch_config.argtypes = [vp] + [C.POINTER(i)] * 5
     L.fbh_batch_autotune.argtypes = [vp, i, C.POINTER(Tune)]
     L.fbh_debug_math.argtypes = [i, vp, vp, C.c_long, vp, vp, i]
     L.fbh_debug_libm_host.argtypes = [i, C.c_long, C.c_long, vp]

@@ -1505,10 +1505,15 @@ __global__ void k_trmean(const double* TM, float* out, int n, int ndpost) {
 
 // ------------------------------------------------------------------ launch variants
 using SweepKernel = void (*)(KArgs);
-constexpr int kRowsRes = 6;    // 1536-row tiles when the running fit is resident in LDS
-constexpr int kRowsStream = 4; // 1024-row tiles otherwise
+constexpr int kRowsRes = 6;    // 1536-row tiles (the running fit resident in LDS)
+constexpr int kRowsStream = 4; // 1024-row tiles (the running fit streamed, or resident in LDS)
 static_assert(sizeof(ProbitRing) <= Geo<kRowsStream>::kScratchBytes && sizeof(ProbitRing) <= Geo<kRowsRes>::kScratchBytes,
               "the probit ring reuses the tile scratch");
+// tile kinds: 0 = 1024-row tiles, running fit streamed; 1 = 1024-row tiles, fit in LDS; 2 = 1536-row tiles, fit in LDS
+constexpr int kKinds = 3;
+constexpr bool kind_resident(int k) { return k != 0; }
+constexpr int kind_rows(int k) { return k == 2 ? kRowsRes : kRowsStream; }
+inline int kind_scratch(int k) { return k == 2 ? Geo<kRowsRes>::kScratchBytes : Geo<kRowsStream>::kScratchBytes; }
 constexpr int kCtlBytes = sizeof(Ctl);
 constexpr size_t kLdsBudgetPerWG = 80 * 1024;
 inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
@@ -1516,21 +1521,25 @@ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 struct Variant {
   SweepKernel fn;
   int scratch;
+  int kind;
   bool resident;
   bool dense;
 };
 
-template <bool PB, bool D>
-inline Variant pick_variant_pd(bool resident, bool fx) {
-  if (resident)
-    return fx ? Variant{k_sweeps<kRowsRes, true, true, PB, D>, Geo<kRowsRes>::kScratchBytes, true, D}
-              : Variant{k_sweeps<kRowsRes, true, false, PB, D>, Geo<kRowsRes>::kScratchBytes, true, D};
-  return fx ? Variant{k_sweeps<kRowsStream, false, true, PB, D>, Geo<kRowsStream>::kScratchBytes, false, D}
-            : Variant{k_sweeps<kRowsStream, false, false, PB, D>, Geo<kRowsStream>::kScratchBytes, false, D};
+template <int R, bool RES, bool PB, bool D>
+inline Variant variant_of(int kind, bool fx) {
+  return fx ? Variant{k_sweeps<R, RES, true, PB, D>, Geo<R>::kScratchBytes, kind, RES, D}
+            : Variant{k_sweeps<R, RES, false, PB, D>, Geo<R>::kScratchBytes, kind, RES, D};
 }
-inline Variant pick_variant(bool resident, bool fx, bool probit, bool dense) {
-  if (probit) return dense ? pick_variant_pd<true, true>(resident, fx) : pick_variant_pd<true, false>(resident, fx);
-  return dense ? pick_variant_pd<false, true>(resident, fx) : pick_variant_pd<false, false>(resident, fx);
+template <bool PB, bool D>
+inline Variant pick_variant_pd(int kind, bool fx) {
+  if (kind == 2) return variant_of<kRowsRes, true, PB, D>(kind, fx);
+  if (kind == 1) return variant_of<kRowsStream, true, PB, D>(kind, fx);
+  return variant_of<kRowsStream, false, PB, D>(kind, fx);
+}
+inline Variant pick_variant(int kind, bool fx, bool probit, bool dense) {
+  if (probit) return dense ? pick_variant_pd<true, true>(kind, fx) : pick_variant_pd<true, false>(kind, fx);
+  return dense ? pick_variant_pd<false, true>(kind, fx) : pick_variant_pd<false, false>(kind, fx);
 }
 
 constexpr int kRowsStreamL = kRowsStream;  // the learned pipeline streams the running fit
@@ -2221,9 +2230,9 @@ struct fbh_batch {
   int af_off = 0;
   // the kernel variants this batch may switch between at launch boundaries (the state between launches is the same):
   // [dense][resident]; fits = all workgroups resident at once (required by a row split)
-  Variant vars[2][2]{};
-  bool res_ok = false;
-  bool fits[2][2]{};
+  Variant vars[2][kKinds]{};
+  bool kind_ok[kKinds]{};
+  bool fits[2][kKinds]{};
   // the packing: workgroup w runs device chain order[w]
   int packing = 0, ncu = 0;
   std::vector<int> order;
@@ -2343,28 +2352,41 @@ static void packing_order(const fbh_batch* b, int mode, std::vector<int>& ord) {
   }
 }
 
-static size_t dyn_of(const fbh_batch* b, bool resident) {
-  return align16(resident ? Geo<kRowsRes>::kScratchBytes : Geo<kRowsStream>::kScratchBytes) +
-         (resident ? 8 * afl_doubles((size_t)b->nmax) : 0);
+static size_t dyn_of(const fbh_batch* b, int kind) {
+  return align16(kind_scratch(kind)) + (kind_resident(kind) ? 8 * afl_doubles((size_t)b->nmax) : 0);
 }
 
-static void use_variant(fbh_batch* b, bool resident, bool dense) {
-  b->var = b->vars[dense][resident];
+static void use_variant(fbh_batch* b, int kind, bool dense) {
+  b->var = b->vars[dense][kind];
   b->af_off = (int)align16(b->var.scratch);
-  b->dyn_bytes = dyn_of(b, resident);
+  b->dyn_bytes = dyn_of(b, kind);
 }
 
-// switch the packing, the tile variant and the build between launches (waits for the batch's queued work)
-static int configure(fbh_batch* b, int packing, bool resident, bool dense) {
+// the tile kind for tile_rows (0, 1024, 1536) and resident (0 automatic, 1 streamed, 2 in LDS); -1 if none fits.
+// Automatic: the running fit in LDS when it fits -- 1024-row tiles for the dense build, 1536 otherwise -- except that
+// tile_rows = 1024 alone keeps the streamed kind.
+static int pick_kind(const fbh_batch* b, int tile_rows, int resident, bool dense) {
+  if (resident == 1) return tile_rows == 1536 ? -1 : 0;
+  if (tile_rows == 1536) return b->kind_ok[2] ? 2 : -1;
+  if (tile_rows == 1024) return resident == 2 ? (b->kind_ok[1] ? 1 : -1) : 0;
+  const int pref[2] = {dense ? 1 : 2, dense ? 2 : 1};
+  for (int k : pref)
+    if (b->kind_ok[k]) return k;
+  return resident == 2 ? -1 : 0;
+}
+
+// switch the packing, the tile kind and the build between launches (waits for the batch's queued work)
+static int configure(fbh_batch* b, int packing, int kind, bool dense) {
   if (packing < 0 || packing > 2) {
     set_error(FBH_EINVAL, "fbh_batch_configure: packing must be 0, 1 or 2");
     return FBH_EINVAL;
   }
-  if (resident && !b->res_ok) {
-    set_error(FBH_EINVAL, "fbh_batch_configure: 1536-row tiles need the running fit resident in LDS (n too large)");
+  if (kind < 0 || !b->kind_ok[kind]) {
+    set_error(FBH_EINVAL, "fbh_batch_configure: no such tile configuration for this batch (a running fit in LDS "
+                          "needs the longest chain's rows to fit beside the tile, 1536-row tiles need it in LDS)");
     return FBH_EINVAL;
   }
-  if (b->W > 1 && !b->fits[dense][resident]) {
+  if (b->W > 1 && !b->fits[dense][kind]) {
     set_error(FBH_EINVAL, "fbh_batch_configure: the row split's workgroups would not all be resident at once");
     return FBH_EINVAL;
   }
@@ -2373,7 +2395,7 @@ static int configure(fbh_batch* b, int packing, bool resident, bool dense) {
   packing_order(b, packing, b->order);
   FBH_HIP_CHECK(hipMemcpy(b->d_order, b->order.data(), sizeof(int) * b->C, hipMemcpyHostToDevice));
   b->packing = packing;
-  use_variant(b, resident, dense);
+  use_variant(b, kind, dense);
   return FBH_OK;
 }
 
@@ -2388,9 +2410,10 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
       (o->node_cap != 0 && (o->node_cap < 3 || o->node_cap > kNodes)) ||
       (o->block_threads != 0 && o->block_threads != kBlock) ||
       (o->tile_rows != 0 && o->tile_rows != Geo<kRowsStream>::kTile && o->tile_rows != Geo<kRowsRes>::kTile) ||
-      o->packing < 0 || o->packing > 2 || o->chains_per_cu < 0 || o->chains_per_cu > 2) {
+      o->packing < 0 || o->packing > 2 || o->chains_per_cu < 0 || o->chains_per_cu > 2 || o->resident < 0 ||
+      o->resident > 2) {
     set_error(FBH_EINVAL, "fbh_batch_create: bad options (m, nskip, ndpost, pgtab, stats_mode, node_cap, "
-                          "block_threads, tile_rows, packing or chains_per_cu)");
+                          "block_threads, tile_rows, packing, chains_per_cu or resident)");
     return FBH_EINVAL;
   }
   int W = o->row_split > 1 ? o->row_split : 1;
@@ -2511,22 +2534,17 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device) != hipSuccess)
       lds_max = 65536;
     const size_t budget = std::min<size_t>((size_t)lds_max, kLdsBudgetPerWG);
-    const size_t need_res = kCtlBytes + align16(Geo<kRowsRes>::kScratchBytes) + 8 * afl_doubles((size_t)b->nmax);
-    b->res_ok = need_res <= budget;
-    const bool res = b->res_ok && o->tile_rows != Geo<kRowsStream>::kTile;
-    if (o->tile_rows == Geo<kRowsRes>::kTile && !res) {
-      set_error(FBH_EINVAL, "fbh_batch_create: 1536-row tiles need the running fit resident in LDS (n too large)");
-      free_batch(b);
-      return FBH_EINVAL;
-    }
+    for (int k = 0; k < kKinds; k++)
+      b->kind_ok[k] = !kind_resident(k) ||
+                      kCtlBytes + align16(kind_scratch(k)) + 8 * afl_doubles((size_t)b->nmax) <= budget;
     if (hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess) b->ncu = 0;
     const bool fx = b->stats_mode == FBH_STATS_FX64;
     for (int d = 0; d < 2; d++)
-      for (int r = 0; r < 2; r++) {
-        Variant& v = b->vars[d][r];
-        v = pick_variant(r != 0, fx, b->probit != 0, d != 0);
-        if (r == 1 && !b->res_ok) continue;
-        const size_t dyn = dyn_of(b, r != 0);
+      for (int k = 0; k < kKinds; k++) {
+        Variant& v = b->vars[d][k];
+        v = pick_variant(k, fx, b->probit != 0, d != 0);
+        if (!b->kind_ok[k]) continue;
+        const size_t dyn = dyn_of(b, k);
         if (kCtlBytes + dyn > (size_t)lds_max) {
           set_error(FBH_EINVAL, "fbh_batch_create: LDS need exceeds the device limit");
           free_batch(b);
@@ -2545,26 +2563,32 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
             hipSuccess)
           per_cu = 0;
         per_cu = std::min(per_cu, d ? 2 : 1);
-        b->fits[d][r] = (long)per_cu * b->ncu >= (long)nchains;
+        b->fits[d][k] = (long)per_cu * b->ncu >= (long)nchains;
       }
-    const bool dense = o->chains_per_cu == 2 || (o->chains_per_cu == 0 && nchains > b->ncu);
-    use_variant(b, res, dense);
-  }
-  const size_t tout = b->keep_trees ? (size_t)nchains_l * b->ndpost * b->m * kNodes : 1;
-  if (W > 1) {
-    // every member must be resident at once: the exchange waits for all of them. An automatic choice of build takes
-    // the other build if only that one fits.
-    const bool res = b->var.resident;
-    bool dense = b->var.dense;
-    if (!b->fits[dense][res] && o->chains_per_cu == 0 && b->fits[!dense][res]) dense = !dense;
-    if (!b->fits[dense][res]) {
-      set_error(FBH_EINVAL, "fbh_batch_create: " + std::to_string(nchains) + " workgroups of a row split do not fit "
-                                "on the device at once (" + std::to_string(dense ? 2 : 1) + " per CU x " +
-                                std::to_string(b->ncu) + " CUs)");
+    bool dense = o->chains_per_cu == 2 || (o->chains_per_cu == 0 && nchains > b->ncu);
+    int kind = pick_kind(b, o->tile_rows, o->resident, dense);
+    if (kind < 0) {
+      set_error(FBH_EINVAL, "fbh_batch_create: tile_rows / resident ask for the running fit in LDS, and the longest "
+                            "chain's rows do not fit there beside the tile");
       free_batch(b);
       return FBH_EINVAL;
     }
-    use_variant(b, res, dense);
+    if (W > 1) {
+      // every member must be resident at once: the exchange waits for all of them. An automatic choice of build
+      // takes the other build if only that one fits.
+      if (!b->fits[dense][kind] && o->chains_per_cu == 0 && b->fits[!dense][kind]) dense = !dense;
+      if (!b->fits[dense][kind]) {
+        set_error(FBH_EINVAL, "fbh_batch_create: " + std::to_string(nchains) + " workgroups of a row split do not "
+                                  "fit on the device at once (" + std::to_string(dense ? 2 : 1) + " per CU x " +
+                                  std::to_string(b->ncu) + " CUs)");
+        free_batch(b);
+        return FBH_EINVAL;
+      }
+    }
+    use_variant(b, kind, dense);
+  }
+  const size_t tout = b->keep_trees ? (size_t)nchains_l * b->ndpost * b->m * kNodes : 1;
+  if (W > 1) {
     if ((rc = dalloc(&b->d_xbuf, (size_t)nchains_l * 2 * W * kXw)) || (rc = dalloc(&b->d_xctr, (size_t)nchains_l)) ||
         hipMemset(b->d_xctr, 0, sizeof(uint32_t) * nchains_l) != hipSuccess) {
       free_batch(b);
@@ -2700,9 +2724,9 @@ extern "C" int fbh_batch_run(fbh_batch* b, int nsweeps) {
   return FBH_OK;
 }
 
-extern "C" int fbh_batch_configure(fbh_batch* b, int packing, int tile_rows, int chains_per_cu) {
+extern "C" int fbh_batch_configure(fbh_batch* b, int packing, int tile_rows, int resident, int chains_per_cu) {
   if (!b || (tile_rows != 0 && tile_rows != Geo<kRowsStream>::kTile && tile_rows != Geo<kRowsRes>::kTile) ||
-      chains_per_cu < 0 || chains_per_cu > 2) {
+      resident < 0 || resident > 2 || chains_per_cu < 0 || chains_per_cu > 2) {
     set_error(FBH_EINVAL, "fbh_batch_configure: bad argument");
     return FBH_EINVAL;
   }
@@ -2710,9 +2734,8 @@ extern "C" int fbh_batch_configure(fbh_batch* b, int packing, int tile_rows, int
     set_error(FBH_EINVAL, "fbh_batch_configure: not for a batch set up for the learned proposal");
     return FBH_EINVAL;
   }
-  const bool res = tile_rows == 0 ? b->res_ok : tile_rows == Geo<kRowsRes>::kTile;
   const bool dense = chains_per_cu == 0 ? b->var.dense : chains_per_cu == 2;
-  return configure(b, packing, res, dense);
+  return configure(b, packing, pick_kind(b, tile_rows, resident, dense), dense);
 }
 
 extern "C" int fbh_batch_set_order(fbh_batch* b, const int* order) {
@@ -2737,13 +2760,15 @@ extern "C" int fbh_batch_set_order(fbh_batch* b, const int* order) {
   return FBH_OK;
 }
 
-extern "C" int fbh_batch_config(const fbh_batch* b, int* packing, int* tile_rows, int* chains_per_cu, int* graph) {
+extern "C" int fbh_batch_config(const fbh_batch* b, int* packing, int* tile_rows, int* resident, int* chains_per_cu,
+                                int* graph) {
   if (!b) {
     set_error(FBH_EINVAL, "fbh_batch_config: null batch");
     return FBH_EINVAL;
   }
   if (packing) *packing = b->packing;
-  if (tile_rows) *tile_rows = b->var.resident ? Geo<kRowsRes>::kTile : Geo<kRowsStream>::kTile;
+  if (tile_rows) *tile_rows = 256 * kind_rows(b->var.kind);
+  if (resident) *resident = b->var.resident ? 2 : 1;
   if (chains_per_cu) *chains_per_cu = b->var.dense ? 2 : 1;
   if (graph) *graph = b->graph;
   return FBH_OK;
@@ -2751,7 +2776,8 @@ extern "C" int fbh_batch_config(const fbh_batch* b, int* packing, int* tile_rows
 
 // Autotuner: times candidate launch configurations on the batch's own next sweeps -- real sweeps of every chain, whose
 // results no candidate changes -- and keeps the fastest. Two stages: the build (one or two chains per CU; two only
-// with more workgroups than CUs) x the tile rows, at largest-first packing; then the packing with the stage's winner.
+// with more workgroups than CUs) x the tile kind (1024-row tiles with the running fit streamed or in LDS, 1536-row
+// tiles with it in LDS), at largest-first packing; then the packing with the stage's winner.
 // One untimed warm-up launch, then per stage two rounds of one launch of trial_sweeps sweeps per candidate, the
 // second round in reverse order so that drift in the cost of a sweep (trees growing during burn-in) falls on every
 // candidate alike.
@@ -2765,15 +2791,15 @@ extern "C" int fbh_batch_autotune(fbh_batch* b, int trial_sweeps, fbh_tune* out)
     return FBH_EINVAL;
   }
   struct Cand {
-    int packing;
-    bool res, dense;
+    int packing, kind;
+    bool dense;
     double ms;
   };
   std::vector<Cand> s1;
   for (int d = (b->C > b->ncu) ? 1 : 0; d >= 0; d--)
-    for (int r = b->res_ok ? 1 : 0; r >= 0; r--)
-      if (b->W <= 1 || b->fits[d][r]) s1.push_back(Cand{2, r != 0, d != 0, 0.0});
-  if (s1.empty()) s1.push_back(Cand{2, b->var.resident, b->var.dense, 0.0});
+    for (int k = kKinds - 1; k >= 0; k--)
+      if (b->kind_ok[k] && (b->W <= 1 || b->fits[d][k])) s1.push_back(Cand{2, k, d != 0, 0.0});
+  if (s1.empty()) s1.push_back(Cand{2, b->var.kind, b->var.dense, 0.0});
   const int need = (1 + 2 * (int)s1.size() + 2 * 3) * trial_sweeps;
   if (b->it + need > b->nskip + b->ndpost) {
     set_error(FBH_EINVAL, "fbh_batch_autotune: " + std::to_string(need) + " sweeps needed, " +
@@ -2787,7 +2813,7 @@ extern "C" int fbh_batch_autotune(fbh_batch* b, int trial_sweeps, fbh_tune* out)
     for (int round = 0; round < 2; round++)
       for (size_t k = 0; k < cs.size(); k++) {
         Cand& cd = cs[round ? cs.size() - 1 - k : k];
-        int rc = configure(b, cd.packing, cd.res, cd.dense);
+        int rc = configure(b, cd.packing, cd.kind, cd.dense);
         if (!rc) rc = fbh_batch_run(b, trial_sweeps);
         if (!rc) rc = fbh_batch_sync(b);
         if (rc) return rc;
@@ -2812,18 +2838,19 @@ extern "C" int fbh_batch_autotune(fbh_batch* b, int trial_sweeps, fbh_tune* out)
   if (!rc) rc = stage(s1);
   if (!rc) {
     const Cand w = best_of(s1);
-    for (int pk = 0; pk < 3; pk++) s2.push_back(Cand{pk, w.res, w.dense, 0.0});
+    for (int pk = 0; pk < 3; pk++) s2.push_back(Cand{pk, w.kind, w.dense, 0.0});
     rc = stage(s2);
   }
   b->sweeps_per_launch = spl;
   b->graph = g;
   if (rc) return rc;
   const Cand w = best_of(s2);
-  if ((rc = configure(b, w.packing, w.res, w.dense))) return rc;
+  if ((rc = configure(b, w.packing, w.kind, w.dense))) return rc;
   if (out) {
     std::memset(out, 0, sizeof *out);
     out->packing = w.packing;
-    out->tile_rows = w.res ? Geo<kRowsRes>::kTile : Geo<kRowsStream>::kTile;
+    out->tile_rows = 256 * kind_rows(w.kind);
+    out->resident = kind_resident(w.kind) ? 2 : 1;
     out->chains_per_cu = w.dense ? 2 : 1;
     out->sweeps = need;
     std::vector<Cand> all(s1);
@@ -2831,7 +2858,8 @@ extern "C" int fbh_batch_autotune(fbh_batch* b, int trial_sweeps, fbh_tune* out)
     out->ncand = std::min<int>((int)all.size(), FBH_TUNE_MAX);
     for (int i = 0; i < out->ncand; i++) {
       out->cand_packing[i] = all[i].packing;
-      out->cand_tile_rows[i] = all[i].res ? Geo<kRowsRes>::kTile : Geo<kRowsStream>::kTile;
+      out->cand_tile_rows[i] = 256 * kind_rows(all[i].kind);
+      out->cand_resident[i] = kind_resident(all[i].kind) ? 2 : 1;
       out->cand_chains_per_cu[i] = all[i].dense ? 2 : 1;
       out->ms_per_sweep[i] = all[i].ms;
     }

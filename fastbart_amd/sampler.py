@@ -39,7 +39,8 @@ class BatchSampler:
     """Many independent chains sampled together on one GPU (fbh_batch_*)."""
 
     def __init__(self, problems, m, nskip, ndpost, seeds, probit=0, keep_trees=1, stats_mode=0, node_cap=0,
-                 device=0, sweeps_per_launch=0, pgtab=None, tile_rows=0, row_split=0, packing=0, graph=0, chains_per_cu=0):
+                 device=0, sweeps_per_launch=0, pgtab=None, tile_rows=0, row_split=0, packing=0, graph=0, chains_per_cu=0,
+                 resident=0):
         L = lib()
         self.problems = problems
         self.C = len(problems)
@@ -52,7 +53,7 @@ class BatchSampler:
         pg = np.ascontiguousarray(pgtab if pgtab is not None else problems[0]["pgtab"], dtype=np.float32)
         self._keep.append(pg)
         o = Opts(m, nskip, ndpost, probit, keep_trees, ptr(pg), stats_mode, node_cap, device, sweeps_per_launch, 0,
-                 tile_rows, row_split, packing, int(bool(graph)), chains_per_cu)
+                 tile_rows, row_split, packing, int(bool(graph)), chains_per_cu, resident)
         h = C.c_void_p()
         check(L.fbh_batch_create(arr, self.C, C.byref(o), C.byref(h)), "fbh_batch_create")
         self.h = h
@@ -64,10 +65,10 @@ class BatchSampler:
         """Workgroups per chain (1 without a row split)."""
         return int(lib().fbh_batch_row_split(self.h))
 
-    def configure(self, packing=0, tile_rows=0, chains_per_cu=0):
-        """Switch the workgroup -> chain packing, the tile rows and the build (one or two chains per CU; 0 keeps it)
-        between runs. Results do not depend on them."""
-        check(lib().fbh_batch_configure(self.h, int(packing), int(tile_rows), int(chains_per_cu)),
+    def configure(self, packing=0, tile_rows=0, chains_per_cu=0, resident=0):
+        """Switch the workgroup -> chain packing, the tile rows, the running fit's place (1 streamed, 2 in LDS, 0
+        automatic) and the build (one or two chains per CU; 0 keeps it) between runs. Results do not depend on them."""
+        check(lib().fbh_batch_configure(self.h, int(packing), int(tile_rows), int(resident), int(chains_per_cu)),
               "fbh_batch_configure")
 
     def set_order(self, order):
@@ -77,20 +78,21 @@ class BatchSampler:
 
     @property
     def config(self):
-        """The launch configuration in effect: packing, tile_rows, chains_per_cu, graph."""
-        pk, tr, cp, g = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        check(lib().fbh_batch_config(self.h, C.byref(pk), C.byref(tr), C.byref(cp), C.byref(g)), "fbh_batch_config")
-        return dict(packing=pk.value, tile_rows=tr.value, chains_per_cu=cp.value, graph=g.value)
+        """The launch configuration in effect: packing, tile_rows, resident (1 streamed, 2 in LDS), chains_per_cu,
+        graph."""
+        v = [C.c_int() for _ in range(5)]
+        check(lib().fbh_batch_config(self.h, *[C.byref(x) for x in v]), "fbh_batch_config")
+        return dict(zip(("packing", "tile_rows", "resident", "chains_per_cu", "graph"), (x.value for x in v)))
 
     def autotune(self, trial_sweeps=1):
         """Time every (packing, tile rows) candidate on the next sweeps (real sweeps of every chain) and keep the
         fastest. Returns the choice and each candidate's ms per sweep."""
         t = Tune()
         check(lib().fbh_batch_autotune(self.h, int(trial_sweeps), C.byref(t)), "fbh_batch_autotune")
-        cands = [dict(packing=t.cand_packing[k], tile_rows=t.cand_tile_rows[k], chains_per_cu=t.cand_chains_per_cu[k],
-                      ms_per_sweep=t.ms_per_sweep[k]) for k in range(t.ncand)]
-        return dict(packing=t.packing, tile_rows=t.tile_rows, chains_per_cu=t.chains_per_cu, sweeps=t.sweeps,
-                    candidates=cands)
+        cands = [dict(packing=t.cand_packing[k], tile_rows=t.cand_tile_rows[k], resident=t.cand_resident[k],
+                      chains_per_cu=t.cand_chains_per_cu[k], ms_per_sweep=t.ms_per_sweep[k]) for k in range(t.ncand)]
+        return dict(packing=t.packing, tile_rows=t.tile_rows, resident=t.resident, chains_per_cu=t.chains_per_cu,
+                    sweeps=t.sweeps, candidates=cands)
 
     def run(self, nsweeps):
         check(lib().fbh_batch_run(self.h, int(nsweeps)), "fbh_batch_run")

@@ -81,7 +81,7 @@ typedef struct fbh_opts {
   int sweeps_per_launch;    /* 0 = automatic */
   int block_threads;        /* 0 or 320: one control wave + four data waves per chain (the only shape built) */
   int tile_rows;            /* 0 = automatic; 1024 or 1536 rows staged in LDS per tile (1536 needs the running fit
-                               resident in LDS, 1024 is the only size when it streams) */
+                               resident in LDS; 1024 alone, with resident = 0, streams it) */
   int row_split;            /* 0 or 1 = one workgroup per chain. W > 1 (FX64 statistics, wbart only): the rows of
                                every chain are split over W workgroups that run the same control steps and add their
                                fixed-point pass totals after every pass -- exact, as fixed-point sums are order-free,
@@ -97,6 +97,9 @@ typedef struct fbh_opts {
   int chains_per_cu;        /* build of the sweep kernel: 1 = 168 VGPRs, one workgroup per CU; 2 = 128 VGPRs, two
                                workgroups share a CU; 0 = automatic (2 when the batch has more workgroups than the
                                device has CUs). Results do not depend on it. */
+  int resident;             /* the running fit during a launch: 1 = streamed from HBM, 2 = resident in LDS (needs the
+                               longest chain's 8 n bytes beside the tile within 80 KiB), 0 = automatic (in LDS when
+                               it fits: 1024-row tiles for the two-per-CU build, 1536 otherwise) */
 } fbh_opts;
 
 typedef struct fbh_chain_out {
@@ -121,25 +124,27 @@ int fbh_batch_row_split(const fbh_batch* b);
 
 /* ---------------------------------------------------------------- launch configuration and autotuner (new; the
  * reference runs one chain per process and has no launch to configure) */
-#define FBH_TUNE_MAX 8
+#define FBH_TUNE_MAX 12
 typedef struct fbh_tune {
-  int packing, tile_rows, chains_per_cu;  /* the configuration chosen (in effect on return) */
-  int ncand;                              /* candidates timed (both stages) */
-  int cand_packing[FBH_TUNE_MAX], cand_tile_rows[FBH_TUNE_MAX], cand_chains_per_cu[FBH_TUNE_MAX];
+  int packing, tile_rows, resident, chains_per_cu;  /* the configuration chosen (in effect on return) */
+  int ncand;                                        /* candidates timed (both stages) */
+  int cand_packing[FBH_TUNE_MAX], cand_tile_rows[FBH_TUNE_MAX], cand_resident[FBH_TUNE_MAX],
+      cand_chains_per_cu[FBH_TUNE_MAX];
   double ms_per_sweep[FBH_TUNE_MAX];
-  int sweeps;                             /* sweeps the tuning ran: real sweeps of every chain, results unchanged */
+  int sweeps;                                       /* sweeps the tuning ran: real sweeps, results unchanged */
 } fbh_tune;
-/* Switch packing (0, 1, 2 as fbh_opts.packing), tile rows (1024, 1536, 0 = the automatic choice) and the build
- * (chains_per_cu 1 or 2, 0 = keep) between runs; waits for the batch's queued work. Results do not depend on them. */
-int fbh_batch_configure(fbh_batch* b, int packing, int tile_rows, int chains_per_cu);
+/* Switch packing (0, 1, 2 as fbh_opts.packing), tile rows and resident (as in fbh_opts; 0 = the automatic choice)
+ * and the build (chains_per_cu 1 or 2, 0 = keep) between runs; waits for the batch's queued work. Results do not
+ * depend on any of them. */
+int fbh_batch_configure(fbh_batch* b, int packing, int tile_rows, int resident, int chains_per_cu);
 /* A packing of the caller's: logical chain order[w] runs on workgroup w (on workgroups w W .. w W + W - 1 with a row
  * split of W); order is a permutation of 0..nchains-1. Waits for the batch's queued work; results do not depend on it. */
 int fbh_batch_set_order(fbh_batch* b, const int* order);
-/* the configuration in effect (packing -1 after fbh_batch_set_order) */
-int fbh_batch_config(const fbh_batch* b, int* packing, int* tile_rows, int* chains_per_cu, int* graph);
-/* Time launch configurations on the batch's next sweeps and keep the fastest: first build x tile rows at largest-first
- * packing, then the three packings with the winner; one warm-up launch, then two launches of trial_sweeps sweeps per
- * candidate. The sweeps are real sweeps of every chain and count toward nskip + ndpost (FBH_EINVAL if too few are
+/* the configuration in effect (packing -1 after fbh_batch_set_order; resident 1 streamed, 2 in LDS) */
+int fbh_batch_config(const fbh_batch* b, int* packing, int* tile_rows, int* resident, int* chains_per_cu, int* graph);
+/* Time launch configurations on the batch's next sweeps and keep the fastest: first build x tile kind (1024-row tiles
+ * streamed or in LDS, 1536-row tiles in LDS) at largest-first packing, then the three packings with the winner; one
+ * warm-up launch, then two launches of trial_sweeps sweeps per candidate. The sweeps are real sweeps of every chain and count toward nskip + ndpost (FBH_EINVAL if too few are
  * left); no candidate changes any result. out may be null. */
 int fbh_batch_autotune(fbh_batch* b, int trial_sweeps, fbh_tune* out);
 /* Copy chain c's outputs to host buffers (synchronises). */

@@ -60,20 +60,26 @@ def test_gpu_probit_packing_and_graph_match_oracle(oracle):
 
 
 def test_gpu_configuration_switched_between_launches(oracle):
-    """Packing, tile rows and build changed at every launch boundary: the state carried between launches is the same
+    """Packing, tile kind and build changed at every launch boundary: the state carried between launches is the same
     in every configuration."""
     probs = _probs()
-    m, nskip, nd = 12, 15, 6
+    m, nskip, nd = 12, 18, 6
     seeds = [51 + c for c in range(len(probs))]
-    cycle = [(0, 1024, 1), (1, 1536, 2), (2, 1024, 2), (2, 1536, 1), (1, 1024, 0), (0, 1536, 2), (1, 0, 1)]
+    # (packing, tile_rows, resident, chains_per_cu) asked -> (tile_rows, resident) in effect
+    cycle = [((0, 1024, 1, 1), (1024, 1)), ((1, 1536, 0, 2), (1536, 2)), ((2, 1024, 2, 2), (1024, 2)),
+             ((2, 1536, 2, 1), (1536, 2)), ((1, 1024, 0, 0), (1024, 1)), ((0, 0, 2, 2), (1024, 2)),
+             ((1, 0, 0, 1), (1536, 2)), ((2, 1024, 2, 1), (1024, 2))]
     _say("switched configurations")
     with sampler.BatchSampler(probs, m, nskip, nd, seeds, sweeps_per_launch=3) as b:
-        assert b.config["chains_per_cu"] == 1  # automatic: six workgroups, fewer than the CUs
+        cfg = b.config
+        # automatic: six workgroups (fewer than the CUs) -> one chain per CU, 1536-row tiles with the fit in LDS
+        assert (cfg["chains_per_cu"], cfg["tile_rows"], cfg["resident"]) == (1, 1536, 2)
         prev = 1
-        for k, (pk, tr, cp) in enumerate(cycle):
-            b.configure(pk, tr, cp)
+        for (pk, tr, res, cp), (tr_eff, res_eff) in cycle:
+            b.configure(pk, tr, cp, res)
             got = b.config
-            assert (got["packing"], got["tile_rows"], got["chains_per_cu"]) == (pk, tr or 1536, cp or prev)
+            assert (got["packing"], got["tile_rows"], got["resident"], got["chains_per_cu"]) == \
+                (pk, tr_eff, res_eff, cp or prev)
             prev = got["chains_per_cu"]
             b.run(3)
         b.sync()
@@ -106,21 +112,24 @@ def test_gpu_autotune_runs_real_sweeps_and_keeps_results(oracle):
     _say("autotune")
     with sampler.BatchSampler(probs, m, nskip, nd, seeds, sweeps_per_launch=5, graph=1) as b:
         t = b.autotune(1)
-        # stage 1: two tile sizes at one chain per CU (six workgroups, fewer than the CUs); stage 2: three packings
-        c1, c2 = t["candidates"][:2], t["candidates"][2:]
-        assert len(c1) == 2 and len(c2) == 3
-        assert {c["tile_rows"] for c in c1} == {1024, 1536} and all(c["chains_per_cu"] == 1 for c in c1)
+        # stage 1: the three tile kinds at one chain per CU (six workgroups, fewer than the CUs); stage 2: packings
+        c1, c2 = t["candidates"][:3], t["candidates"][3:]
+        assert len(c1) == 3 and len(c2) == 3
+        assert {(c["tile_rows"], c["resident"]) for c in c1} == {(1024, 1), (1024, 2), (1536, 2)}
+        assert all(c["chains_per_cu"] == 1 for c in c1)
         assert [c["packing"] for c in c2] == [0, 1, 2]
-        assert t["sweeps"] == 11 and b.it == 11
+        assert t["sweeps"] == 13 and b.it == 13
         assert all(c["ms_per_sweep"] > 0 for c in t["candidates"])
         w1 = min(c1, key=lambda c: c["ms_per_sweep"])
-        assert all(c["tile_rows"] == w1["tile_rows"] for c in c2)
+        assert all((c["tile_rows"], c["resident"]) == (w1["tile_rows"], w1["resident"]) for c in c2)
         best = min(c2, key=lambda c: c["ms_per_sweep"])
-        assert (t["packing"], t["tile_rows"], t["chains_per_cu"]) == (best["packing"], best["tile_rows"], 1)
+        assert (t["packing"], t["tile_rows"], t["resident"], t["chains_per_cu"]) == \
+            (best["packing"], best["tile_rows"], best["resident"], 1)
         cfg = b.config
-        assert (cfg["packing"], cfg["tile_rows"], cfg["chains_per_cu"]) == (t["packing"], t["tile_rows"], 1)
+        assert (cfg["packing"], cfg["tile_rows"], cfg["resident"], cfg["chains_per_cu"]) == \
+            (t["packing"], t["tile_rows"], t["resident"], 1)
         assert cfg["graph"] == 1  # graph capture is left as it was
-        b.run(nskip + nd - 11)
+        b.run(nskip + nd - 13)
         b.sync()
         _check_oracle(oracle, b, probs, m, nskip, nd, seeds)
 
@@ -135,16 +144,22 @@ def test_gpu_launch_config_rejections():
         with pytest.raises(FbhError):
             b.configure(0, 0, 3)
         with pytest.raises(FbhError):
-            b.autotune(1)  # 11 sweeps needed, 6 left
+            b.configure(0, 1536, 0, 1)  # 1536-row tiles with the fit streamed
+        with pytest.raises(FbhError):
+            b.autotune(1)  # 13 sweeps needed, 6 left
         assert b.it == 0
     big = [W.small_problem(12000, 2, 10, 7, "uniform", ntree=4)]
     with sampler.BatchSampler(big, 4, 2, 1, [1]) as b:
-        assert b.config["tile_rows"] == 1024  # the running fit does not fit in LDS
+        assert (b.config["tile_rows"], b.config["resident"]) == (1024, 1)  # the running fit does not fit in LDS
         with pytest.raises(FbhError):
             b.configure(0, 1536)
+        with pytest.raises(FbhError):
+            b.configure(0, 1024, 0, 2)
         with pytest.raises(FbhError):
             b.autotune(1)  # one stage-1 candidate: 9 sweeps needed, 3 left
     with pytest.raises(FbhError):
         sampler.BatchSampler(big, 4, 2, 1, [1], packing=5)
     with pytest.raises(FbhError):
         sampler.BatchSampler(big, 4, 2, 1, [1], chains_per_cu=3)
+    with pytest.raises(FbhError):
+        sampler.BatchSampler(big, 4, 2, 1, [1], resident=2)

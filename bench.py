@@ -162,6 +162,11 @@ def min_ess_chain(sigma_kept, yhat: np.ndarray, max_rows: int = 0) -> float:
 
 
 # ------------------------------------------------------------------ measured HBM traffic
+# sweeps the autotuner takes at one sweep per trial launch: a warm-up launch, two rounds of at most 2 builds x 3 tile
+# kinds, two rounds of 3 packings
+TUNE_SWEEPS_MAX = 1 + 2 * 6 + 2 * 3
+
+
 def measured_traffic(workload: str, sweeps_per_launch: float):
     """HBM bytes per launch of the sweep kernel from the committed counter summary (FETCH_SIZE + WRITE_SIZE per sweep,
     each scaled by the ratio the same counter reads for a plain copy of known size), or None."""
@@ -460,6 +465,9 @@ def main() -> int:
             cpu = cpu_baseline("batch512", cores)
         return learned_vs_plain(args, chains, NSKIP, args.ndpost or NDPOST, cpu)
     W_ = max(0, args.warmup if args.warmup is not None else w_def)
+    if args.packing == "auto":
+        # the autotuner's trials are warmup sweeps: the warmup is at least as long as the longest tuning
+        W_ = max(W_, TUNE_SWEEPS_MAX)
     K = args.steps if args.steps is not None else (k_def if k_def is not None else NSKIP + NDPOST - W_)
     total = W_ + K
     nskip = max(0, total - NDPOST) if total >= NDPOST else 0
@@ -514,6 +522,8 @@ def main() -> int:
     b.run(K)
     b.sync()
     T = time.perf_counter() - t1
+    if b.it != W_ + K:
+        raise RuntimeError(f"ran {b.it} sweeps, expected {W_} warmup + {K} timed")
     avg_ms, nl, _ = b.timing()
     value = chains * M * K / T
     log(f"timed {K} sweeps: {T:.3f} s -> {value / 1e6:.3f} M tree-updates/s; kernel avg {avg_ms:.3f} ms over {nl} "

@@ -2565,7 +2565,9 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
         per_cu = std::min(per_cu, d ? 2 : 1);
         b->fits[d][k] = (long)per_cu * b->ncu >= (long)nchains;
       }
-    bool dense = o->chains_per_cu == 2 || (o->chains_per_cu == 0 && nchains > b->ncu);
+    // automatic: the dense build when there are more workgroups than CUs -- except for probit batches, whose dense
+    // variants spill heavily (the autotuner still tries them)
+    bool dense = o->chains_per_cu == 2 || (o->chains_per_cu == 0 && nchains > b->ncu && !o->probit);
     int kind = pick_kind(b, o->tile_rows, o->resident, dense);
     if (kind < 0) {
       set_error(FBH_EINVAL, "fbh_batch_create: tile_rows / resident ask for the running fit in LDS, and the longest "

@@ -95,8 +95,8 @@ typedef struct fbh_opts {
   int graph;                /* 1: fbh_batch_run captures each call's launches into a hipGraph and launches the graph
                                (0: plain stream launches). Results do not depend on it. */
   int chains_per_cu;        /* build of the sweep kernel: 1 = 168 VGPRs, one workgroup per CU; 2 = 128 VGPRs, two
-                               workgroups share a CU; 0 = automatic (2 when the batch has more workgroups than the
-                               device has CUs). Results do not depend on it. */
+                               workgroups share a CU; 0 = automatic (2 when a wbart batch has more workgroups than
+                               the device has CUs). Results do not depend on it. */
   int resident;             /* the running fit during a launch: 1 = streamed from HBM, 2 = resident in LDS (needs the
                                longest chain's 8 n bytes beside the tile within 80 KiB), 0 = automatic (in LDS when
                                it fits: 1024-row tiles for the two-per-CU build, 1536 otherwise) */

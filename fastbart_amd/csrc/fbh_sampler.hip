@@ -669,9 +669,12 @@ __device__ __forceinline__ uint32_t last_quad(int n, int dw) {
   return n > k0 ? (uint32_t)(((n - 1 - k0) & ~(kLanes - 1)) + k0) : (uint32_t)k0;
 }
 
-template <int R, bool RES, bool FX, bool D = false, bool YF, bool AFT>
+// D: the dense build (two chains per CU, 128 VGPRs), which gathers one row at a time in phase A. RSS: the closing pass
+// (residual sum of squares and the posterior-mean sum instead of leaf sums).
+template <int R, bool RES, bool FX, bool D = false, bool RSS = false, bool YF, bool AFT>
 __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int dw, int lane, int jp, int j, int cb,
-                          bool apply, bool stats, bool rss, bool keep, TileIn<R, YF, AFT>& T, bool first_loaded) {
+                          bool apply, bool stats, bool keep, TileIn<R, YF, AFT>& T, bool first_loaded) {
+  constexpr bool rss = RSS;
   static_assert(AFT || RES, "a running fit outside LDS is part of the tile");
   using G = Geo<R>;
   const int n = P.n;
@@ -738,12 +741,10 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
 #pragma unroll
           for (int k = 0; k < RC; k++) {
             const int u = u0 + k;
+            // tree j-1's change as selects (no branch): a birth splits leaf pnx on the rank, a death merges pl, pr
             const int lf = (int)((T.la[s] >> (8 * u)) & 255u);
-            int lf2 = lf;
-            if (pmode == 1)
-              lf2 = lf == pnx ? ((int)((T.qa[s] >> (8 * u)) & 255u) <= pc ? pL : pR) : lf;
-            else if (pmode == 2)
-              lf2 = (lf == pl || lf == pr) ? pnx : lf;
+            const bool born = pmode == 1 && lf == pnx, merged = pmode == 2 && (lf == pl || lf == pr);
+            const int lf2 = born ? ((int)((T.qa[s] >> (8 * u)) & 255u) <= pc ? pL : pR) : merged ? pnx : lf;
             lfn[k] = lf2;
             if (apply) {
               tho_r[k] = c.tho[lf];
@@ -760,16 +761,17 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
 #pragma unroll
           for (int k = 0; k < RC; k++) {
             const int u = u0 + k;
+            // Rows of the quad at or past n store too, unconditionally, into padding nobody reads as a row: the
+            // global pools' padding rows, or in LDS the pad slot after the row's 16-row group (the LDS running fit
+            // is sized for n rounded up to 16 rows)
             const uint32_t i = r0 + (uint32_t)u;
             const bool valid = i < (uint32_t)n;
             double af = af_r[k];
             if (apply) {
               af = af - tho_r[k] + thn_r[k];
               lnew |= (uint32_t)lfn[k] << (8 * u);
-              if (valid) {
-                if (RES) P.AFl[afl(i)] = af;
-                else sto(P.AF, i, af);
-              }
+              if (RES) P.AFl[valid ? afl(i) : afl(i | 15u) + 1u] = af;
+              else sto(P.AF, i, af);
             }
             double y;
             if constexpr (YF) y = (double)T.y[s][u];
@@ -781,33 +783,24 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
               v0a[u * G::kStride + idx] = 0.0;
               v1a[u * G::kStride + idx] = FX ? __longlong_as_double(fx_round(c2, 16777216.0)) : c2;
               tga[u * G::kTagStride + idx] = 0;
-              if (keep && valid) sto(P.TM, i, ldo((const double*)P.TM, i) + af);
+              if (keep) sto(P.TM, i, ldo((const double*)P.TM, i) + af);
             } else {
               const int lf = (int)((T.lb[s] >> (8 * u)) & 255u);
               const double rr = y - af + thc_r[k];
               const double wr = w * rr;
-              int side = 0;
-              if (cmode == 1 && lf == cnx && valid) {
-                side = (int)((T.qb[s] >> (8 * u)) & 255u) <= cc ? 1 : 2;
-                my_nl += side == 1;
-                my_nr += side == 2;
-              }
+              const bool split = cmode == 1 && lf == cnx && valid;
+              const int side = split ? ((int)((T.qb[s] >> (8 * u)) & 255u) <= cc ? 1 : 2) : 0;
+              my_nl += side == 1;
+              my_nr += side == 2;
               v0a[u * G::kStride + idx] = FX ? __longlong_as_double(fx_round(w, 4294967296.0)) : w;
               v1a[u * G::kStride + idx] = FX ? __longlong_as_double(fx_round(wr, 4294967296.0)) : wr;
               tga[u * G::kTagStride + idx] = (uint8_t)(rank_r[k] | (side << 6));
             }
           }
         }
-        // the quad's new leaf ids: one dword store when the whole quad is below n, else a byte per row
-        if (apply) {
-          if (r0 + 3u < (uint32_t)n) {
-            sto(reinterpret_cast<uint32_t*>(lprev), r0 >> 2, lnew);
-          } else {
-#pragma unroll
-            for (int u = 0; u < kLanesPerWave; u++)
-              if (r0 + (uint32_t)u < (uint32_t)n) sto(lprev, r0 + (uint32_t)u, (uint8_t)(lnew >> (8 * u)));
-          }
-        }
+        // the quad's new leaf ids in one dword (bytes past n land in the column's padding, and stay node ids: they
+        // start at 0 and only ever receive a leaf id of their own row's old value)
+        if (apply) sto(reinterpret_cast<uint32_t*>(lprev), r0 >> 2, lnew);
       }
     }
     wave_sync();
@@ -1432,7 +1425,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       int cb = 0;
       bool pre = false;
       for (int j = 0; j < m; j++) {
-        data_pass<R, RES, FX, D>(c, dyn, P, dw, lane, j - 1, j, cb, j > 0, true, false, false, T, pre);
+        data_pass<R, RES, FX, D, false>(c, dyn, P, dw, lane, j - 1, j, cb, j > 0, true, false, T, pre);
         __syncthreads();
         // while the control wave decides tree j, the next pass's first tile is loaded
         load_next_first_tile<R, RES>(T, c, P, dw, lane, j, m, cb);
@@ -1440,7 +1433,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         __syncthreads();
         cb ^= 1;
       }
-      data_pass<R, RES, FX, D>(c, dyn, P, dw, lane, m - 1, m, cb, true, false, true, keep, T, pre);
+      data_pass<R, RES, FX, D, true>(c, dyn, P, dw, lane, m - 1, m, cb, true, false, keep, T, pre);
       __syncthreads();
       if (PB) {
         __syncthreads();
@@ -1710,7 +1703,7 @@ __global__ __launch_bounds__(kBlock) void k_lp_stage1(LpArgs A) {
   if (wave > 0) {
     if (j > 0) lp_afx_apply(c, P, AFX, j - 1, dw, lane);
     TileIn<kRowsStreamL> T;
-    data_pass<kRowsStreamL, false, FX>(c, dyn, P, dw, lane, j - 1, j, 0, j > 0, true, false, false, T, false);
+    data_pass<kRowsStreamL, false, FX, false, false>(c, dyn, P, dw, lane, j - 1, j, 0, j > 0, true, false, T, false);
   }
   __syncthreads();
   if (wave == 0) ctl_combine<FX>(c, 0, false, 0, lane);
@@ -1955,7 +1948,7 @@ __global__ __launch_bounds__(kBlock) void k_lp_stage2(LpArgs A) {
   const int cmode = c.cmode[0];
   if (cmode == 1 && wave > 0) {
     TileIn<kRowsStreamL> T;
-    data_pass<kRowsStreamL, false, FX>(c, dyn, P, dw, lane, j - 1, j, 0, false, true, false, false, T, false);
+    data_pass<kRowsStreamL, false, FX, false, false>(c, dyn, P, dw, lane, j - 1, j, 0, false, true, false, T, false);
   }
   __syncthreads();
   if (wave == 0) {
@@ -2101,7 +2094,7 @@ __global__ __launch_bounds__(kBlock) void k_lp_close(LpArgs A) {
   if (wave > 0) {
     lp_afx_apply(c, P, A.AFX + c.cd.row_off, m - 1, dw, lane);
     TileIn<kRowsStreamL> T;
-    data_pass<kRowsStreamL, false, FX>(c, dyn, P, dw, lane, m - 1, m, 0, true, false, true, keep, T, false);
+    data_pass<kRowsStreamL, false, FX, false, true>(c, dyn, P, dw, lane, m - 1, m, 0, true, false, keep, T, false);
   }
   __syncthreads();
   if (wave == 0) {
@@ -2353,7 +2346,7 @@ static void packing_order(const fbh_batch* b, int mode, std::vector<int>& ord) {
 }
 
 static size_t dyn_of(const fbh_batch* b, int kind) {
-  return align16(kind_scratch(kind)) + (kind_resident(kind) ? 8 * afl_doubles((size_t)b->nmax) : 0);
+  return align16(kind_scratch(kind)) + (kind_resident(kind) ? 8 * afl_doubles(((size_t)b->nmax + 15) / 16 * 16) : 0);
 }
 
 static void use_variant(fbh_batch* b, int kind, bool dense) {
@@ -2536,7 +2529,7 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     const size_t budget = std::min<size_t>((size_t)lds_max, kLdsBudgetPerWG);
     for (int k = 0; k < kKinds; k++)
       b->kind_ok[k] = !kind_resident(k) ||
-                      kCtlBytes + align16(kind_scratch(k)) + 8 * afl_doubles((size_t)b->nmax) <= budget;
+                      kCtlBytes + dyn_of(b, k) <= budget;
     if (hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess) b->ncu = 0;
     const bool fx = b->stats_mode == FBH_STATS_FX64;
     for (int d = 0; d < 2; d++)

@@ -1240,6 +1240,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         if (want) c.z[lane] = __builtin_sqrtf(-2.f * lf) * cf;
         if (lane == 0) c.lu = luv;
         __syncthreads();
+        __builtin_amdgcn_s_setprio(3);
         // ---- lane sums, Metropolis-Hastings (:469-518), leaf values (:520-528)
         if (split) row_split_exchange(c, a, cb, false, xseq, xerr, lane);
         const int cmode = c.cmode[cb];
@@ -1369,6 +1370,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
           }
         }
         __syncthreads();
+        __builtin_amdgcn_s_setprio(0);
         cb ^= 1;
       }
       // closing pass (:536-541) on the data waves; meanwhile the last tree (buffer cb ^ 1) is stored
@@ -1419,6 +1421,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
     }
   } else {
     TileIn<R, !PB, !RES> T;
+    // Wave priorities (they only order issue among the waves sharing a SIMD): the data waves run at 2; the control
+    // wave at 0 while it prepares the next update during the pass (it finishes with thousands of cycles to spare, and
+    // the data wave beside it on its SIMD sets the pass's length) and at 3 from the pass barrier to the end of the
+    // update, where its lane sums, Metropolis-Hastings step and leaf values are the chain's critical path.
+    __builtin_amdgcn_s_setprio(2);
     for (int it = a.it0; it < a.it1; it++) {
       const bool keep = it >= a.nskip;
       __syncthreads();
@@ -2230,7 +2237,7 @@ struct fbh_batch {
   int packing = 0, ncu = 0;
   std::vector<int> order;
   int* d_order = nullptr;
-  // launches captured in a graph per fbh_batch_run call (kept until the batch is destroyed)
+  // launches captured in a graph per fbh_batch_run call (released by the next fbh_batch_sync or by destroy)
   int graph = 0;
   std::vector<hipGraph_t> graphs;
   std::vector<hipGraphExec_t> gexecs;
@@ -2881,6 +2888,11 @@ extern "C" int fbh_batch_sync(fbh_batch* b) {
   if (!b) return FBH_EINVAL;
   FBH_HIP_CHECK(hipSetDevice(b->device));
   FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
+  // every graph launched so far has completed: release them
+  for (auto g : b->gexecs) (void)hipGraphExecDestroy(g);
+  for (auto g : b->graphs) (void)hipGraphDestroy(g);
+  b->gexecs.clear();
+  b->graphs.clear();
   return check_split(b);
 }
 

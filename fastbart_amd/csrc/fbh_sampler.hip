@@ -1189,6 +1189,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       wave_sync();
       ctl_propose(c, 0, g, cap, lane);
       __syncthreads();
+      __builtin_amdgcn_s_setprio(0);
       int cb = 0;
       for (int j = 0; j < m; j++) {
         // the previous update's tree (buffer cb ^ 1) is stored before that buffer receives tree j+1
@@ -1379,6 +1380,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         a.trees_out[c.cd.tout_off + ((size_t)(it - a.nskip) * m + (m - 1)) * kNodes + lane] = pack_word(tp, lane);
       copy_tree(&TR[m - 1], &tp, lane);
       __syncthreads();
+      // from here to the first pass of the next sweep (sigma, the first proposal) the control wave is on the critical
+      // path again; in the probit redraw the data waves' decisions are, and it stays below them (measured faster)
+      if (!PB) __builtin_amdgcn_s_setprio(3);
       if (PB) {
         // the latents are redrawn by every wave (:543-547); sigma stays at 1
         if (lane == 0) {

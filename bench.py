@@ -204,8 +204,9 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
             means.append(out["trmean"].astype(np.float64))
         return tot, means
 
-    # plain
-    b = sampler.BatchSampler(probs, M, nskip, ndpost, seeds, keep_trees=1, sweeps_per_launch=20)
+    # plain (the launch configuration the autotuner picks for this batch: serpentine packing, the automatic build and
+    # tile kind)
+    b = sampler.BatchSampler(probs, M, nskip, ndpost, seeds, keep_trees=1, sweeps_per_launch=20, packing=1, graph=1)
     b.sync()
     t0 = time.perf_counter()
     b.run(total)
@@ -216,7 +217,7 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
     log(f"plain: {total} sweeps {t_plain:.2f} s, sum min-ESS {ess_plain:.1f}")
     # learned schedule
     net = MLP((513, 256, 256, 8160), max_batch=max(chains, 512), seed=5)
-    b = sampler.BatchSampler(probs, M, nskip, ndpost, seeds, keep_trees=1, sweeps_per_launch=20)
+    b = sampler.BatchSampler(probs, M, nskip, ndpost, seeds, keep_trees=1, sweeps_per_launch=20, packing=1, graph=1)
     b.sync()
     t0 = time.perf_counter()
     b.run(n_plain0)

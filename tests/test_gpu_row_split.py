@@ -99,3 +99,27 @@ def test_gpu_row_split_automatic():
             assert same_bits(outs[-1][c][k], outs[0][c][k]), (c, k)
     with sampler.BatchSampler(probs, m, nskip, nd, [5, 6], stats_mode=0, row_split=-1) as b:
         assert b.row_split == 1
+
+
+def test_gpu_row_split_with_launch_configuration(oracle):
+    """A row-split batch under the launch configuration: graph capture, the autotuner's trial launches (only
+    configurations whose workgroups are all resident at once), and a caller's packing mid-run, against the oracle's
+    FX64 twin."""
+    probs = [W.small_problem(n, 4, 25, 77 + n, "uniform", ntree=12) for n in (900, 1500, 2100)]
+    m, nskip, nd = 12, 18, 6
+    seeds = [11, 12, 13]
+    _say("split with autotune, graph and a caller's order")
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds, stats_mode=1, row_split=3, graph=1,
+                              sweeps_per_launch=4) as b:
+        t = b.autotune(1)
+        assert b.it == t["sweeps"]
+        b.set_order([2, 0, 1])
+        b.run(5)
+        b.configure(1, 0, 1, 0)
+        b.run(nskip + nd - b.it)
+        b.sync()
+        for c, pb in enumerate(probs):
+            e = oracle.fit(pb, m, nskip, nd, seeds[c], stats_mode=1)
+            out = b.fetch(c)
+            for k in ("sigma", "trmean", "trees", "stats"):
+                assert same_bits(out[k], e[k]), (c, k)

@@ -144,8 +144,9 @@ int fbh_batch_set_order(fbh_batch* b, const int* order);
 int fbh_batch_config(const fbh_batch* b, int* packing, int* tile_rows, int* resident, int* chains_per_cu, int* graph);
 /* Time launch configurations on the batch's next sweeps and keep the fastest: first build x tile kind (1024-row tiles
  * streamed or in LDS, 1536-row tiles in LDS) at largest-first packing, then the three packings with the winner; one
- * warm-up launch, then two launches of trial_sweeps sweeps per candidate. The sweeps are real sweeps of every chain and count toward nskip + ndpost (FBH_EINVAL if too few are
- * left); no candidate changes any result. out may be null. */
+ * warm-up launch, then two launches of trial_sweeps sweeps per candidate. The sweeps are real sweeps of every chain
+ * and count toward nskip + ndpost (FBH_EINVAL if too few are left); no candidate changes any result. out may be
+ * null. */
 int fbh_batch_autotune(fbh_batch* b, int trial_sweeps, fbh_tune* out);
 /* Copy chain c's outputs to host buffers (synchronises). */
 int fbh_batch_fetch(fbh_batch* b, int chain, const fbh_chain_out* out);

@@ -190,6 +190,20 @@ def test_batch_create_rejects_bad_inputs_before_any_device_work():
         sampler.BatchSampler([pb], 0, 1, 1, [1])        # m = 0
     with pytest.raises(_lib.FbhError):
         sampler.BatchSampler([pb], 2, 1, 1, [1], node_cap=2)
+    for kw in (dict(packing=3), dict(chains_per_cu=3), dict(resident=3), dict(tile_rows=512)):
+        with pytest.raises(_lib.FbhError):
+            sampler.BatchSampler([pb], 2, 1, 1, [1], **kw)
+
+
+def test_launch_configuration_entry_points_reject_a_null_batch():
+    L = _lib.lib()
+    einval = -1  # FBH_EINVAL
+    assert L.fbh_batch_configure(None, 0, 0, 0, 0) == einval
+    assert L.fbh_batch_config(None, *[C.pointer(C.c_int()) for _ in range(5)]) == einval
+    assert L.fbh_batch_autotune(None, 1, None) == einval
+    order = (C.c_int * 1)(0)
+    assert L.fbh_batch_set_order(None, C.cast(order, C.c_void_p)) == einval
+    assert b"null" in L.fbh_last_error()
 
 
 def test_external_generator_is_refused():

@@ -397,7 +397,14 @@ struct fbh_trees {
   uint16_t* d_rank = nullptr;
   size_t rank_cap = 0;
   uint64_t rank_rows_uid = 0;  // the rows the ranks in d_rank belong to (0 = none)
-  hipStream_t rank_stream = nullptr;
+  // Ordering of d_rank across streams: rank_ready is recorded after the k_rank launch that filled it (on rank_stream),
+  // last_use after every predict kernel that read it (on last_stream). A launch on another stream waits for
+  // rank_ready before reading the ranks; a refill waits for last_use before overwriting them.
+  hipStream_t rank_stream = nullptr, last_stream = nullptr;
+  hipEvent_t rank_ready = nullptr, last_use = nullptr;
+  bool rank_recorded = false, use_recorded = false;
+  // the kernel the last launch took: 1 rank form, 0 general form (-1: none yet); rows per thread of the rank form
+  int last_form = -1, last_rows_per_thread = 0;
   // launch timing: a ring of event pairs, folded into the totals when a pair is reused
   static constexpr int kEv = 128;
   hipEvent_t ev0[kEv] = {}, ev1[kEv] = {};
@@ -426,6 +433,8 @@ static void free_trees(fbh_trees* t) {
     if (t->ev0[k]) (void)hipEventDestroy(t->ev0[k]);
     if (t->ev1[k]) (void)hipEventDestroy(t->ev1[k]);
   }
+  if (t->rank_ready) (void)hipEventDestroy(t->rank_ready);
+  if (t->last_use) (void)hipEventDestroy(t->last_use);
   delete t;
 }
 
@@ -779,8 +788,15 @@ static int predict_launch(fbh_trees* t, const fbh_rows* r, const int* d_draws, i
   const size_t lds = rank_lds_bytes(t->max_draw_nodes, t->ntree, r->p, kRankThreads * rows_per_thread);
   const size_t lds_max = rank_lds_limit(t->device);
   const bool rank_form = t->rank_ok && lds <= lds_max;
+  if (rank_form && !t->rank_ready) {
+    FBH_HIP_CHECK(hipEventCreateWithFlags(&t->rank_ready, hipEventDisableTiming));
+    FBH_HIP_CHECK(hipEventCreateWithFlags(&t->last_use, hipEventDisableTiming));
+  }
   if (rank_form && t->rank_rows_uid != r->uid) {
-    // the rows' ranks against this table; rows are immutable, so the ranks stay valid while the same rows are used
+    // the rows' ranks against this table; rows are immutable, so the ranks stay valid while the same rows are used.
+    // A launch still reading the old ranks on another stream finishes first (a larger buffer is a new allocation,
+    // and hipFree of the old one waits for the device).
+    if (t->use_recorded && t->last_stream != s) FBH_HIP_CHECK(hipStreamWaitEvent(s, t->last_use, 0));
     if (!grow((void**)&t->d_rank, &t->rank_cap, sizeof(uint16_t) * (size_t)r->n * (r->p ? r->p : 1))) {
       set_error(FBH_ENOMEM, "fbh_predict: hipMalloc of the rank buffer failed");
       return FBH_ENOMEM;
@@ -790,8 +806,16 @@ static int predict_launch(fbh_trees* t, const fbh_rows* r, const int* d_draws, i
                          r->p, t->pcut, t->d_rank);
       FBH_HIP_CHECK(hipGetLastError());
     }
+    FBH_HIP_CHECK(hipEventRecord(t->rank_ready, s));
+    t->rank_recorded = true;
+    t->rank_stream = s;
     t->rank_rows_uid = r->uid;
+  } else if (rank_form && t->rank_recorded && t->rank_stream != s) {
+    // ranks computed on another stream: this launch reads them only once they are complete
+    FBH_HIP_CHECK(hipStreamWaitEvent(s, t->rank_ready, 0));
   }
+  t->last_form = rank_form ? 1 : 0;
+  t->last_rows_per_thread = rank_form ? rows_per_thread : 1;
   FBH_HIP_CHECK(hipEventRecord(t->ev0[slot], s));
   if (rank_form) {
     const int rows = kRankThreads * rows_per_thread;
@@ -810,6 +834,23 @@ static int predict_launch(fbh_trees* t, const fbh_rows* r, const int* d_draws, i
   FBH_HIP_CHECK(hipGetLastError());
   FBH_HIP_CHECK(hipEventRecord(t->ev1[slot], s));
   t->ev_live[slot] = true;
+  if (rank_form) {
+    FBH_HIP_CHECK(hipEventRecord(t->last_use, s));
+    t->use_recorded = true;
+    t->last_stream = s;
+  }
+  return FBH_OK;
+}
+
+extern "C" int fbh_predict_last_launch(const fbh_trees* tc, int* form, int* rows_per_thread) {
+  fbh_trees* t = const_cast<fbh_trees*>(tc);
+  if (!t) {
+    set_error(FBH_EINVAL, "fbh_predict_last_launch: null trees");
+    return FBH_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(t->mu);
+  if (form) *form = t->last_form;
+  if (rows_per_thread) *rows_per_thread = t->last_rows_per_thread;
   return FBH_OK;
 }
 

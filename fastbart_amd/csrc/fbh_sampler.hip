@@ -2450,9 +2450,25 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
         set_error(FBH_EINVAL, "fbh_batch_create: numcut outside 0..255");
         return FBH_EINVAL;
       }
+    double ymax = 0.0, wmax = 0.0;
     for (int i = 0; i < ci.n; i++) {
       if (!std::isfinite(ci.wt[i]) || (!o->probit && !std::isfinite(ci.y[i]))) {
         set_error(FBH_EINVAL, "fbh_batch_create: non-finite y or weight in chain " + std::to_string(c));
+        return FBH_EINVAL;
+      }
+      wmax = std::max(wmax, std::fabs((double)ci.wt[i]));
+      if (!o->probit) ymax = std::max(ymax, std::fabs((double)ci.y[i]));
+    }
+    if (o->stats_mode == FBH_STATS_FX64) {
+      // Fixed-point headroom: per-row images of w and w r at 2^32 and of w e^2 at 2^24 are summed in int64, so
+      // |sum w r| < 2^31 and sum w e^2 < 2^39 must hold at every pass. With residuals bounded by 4 max|y| (probit
+      // latents: by 64), these bounds keep a factor of 2 of margin; past them the sums could wrap.
+      const double rb = o->probit ? 64.0 : 4.0 * ymax;
+      const double s1 = (double)ci.n * wmax * rb, s2 = (double)ci.n * wmax * rb * rb;
+      if (!(s1 < 1073741824.0) || !(s2 < 274877906944.0)) {
+        set_error(FBH_EINVAL, "fbh_batch_create: chain " + std::to_string(c) +
+                                  " is outside the FX64 range (n max(wt) (4 max|y|)^2 must stay below 2^38 and "
+                                  "n max(wt) 4 max|y| below 2^30): rescale y or use REF16 statistics");
         return FBH_EINVAL;
       }
     }

@@ -45,6 +45,12 @@ class CompiledTrees:
         check(lib().fbh_predict_timing(self.h, C.byref(ms), C.byref(k)), "fbh_predict_timing")
         return ms.value, k.value
 
+    def last_launch(self):
+        """(form, rows_per_thread) of the last prediction: form 1 = rank-form kernel, 0 = general, -1 = none yet."""
+        f, r = C.c_int(), C.c_int()
+        check(lib().fbh_predict_last_launch(self.h, C.byref(f), C.byref(r)), "fbh_predict_last_launch")
+        return f.value, r.value
+
     def arrays(self):
         k, r = self.nnodes, self.ndpost * self.ntree
         var, child, root = np.zeros(k, np.int32), np.zeros(k, np.int32), np.zeros(r, np.int32)

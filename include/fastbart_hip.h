@@ -36,7 +36,10 @@ extern "C" {
 
 /* statistics modes */
 #define FBH_STATS_REF16 0  /* double sums in 16 row lanes, bit-identical to the reference CPU sampler */
-#define FBH_STATS_FX64 1   /* int64 fixed point (w, w*r at 2^32; rss at 2^24): order independent */
+#define FBH_STATS_FX64 1   /* int64 fixed point (w, w*r at 2^32; rss at 2^24): order independent. Range: every chain
+                              needs n max(wt) 4 max|y| < 2^30 and n max(wt) (4 max|y|)^2 < 2^38 (probit: 64 in place
+                              of 4 max|y|), so that the int64 sums cannot wrap; fbh_batch_create returns FBH_EINVAL
+                              otherwise (rescale y, or use REF16) */
 
 /* ---------------------------------------------------------------- drop-in single chain */
 /* Same arguments and semantics as src/bart_kernel.h:63-66 / src/bart_kernel.cpp:304-307, run on the GPU with the
@@ -201,12 +204,15 @@ void fbh_rows_free(fbh_rows* r);
 int fbh_predict_rows(const fbh_trees* t, const fbh_rows* r, const int* draws, int ndraws, double mu, double* out_host);
 /* The same with the draw list and the output in device memory (draws_dev [ndraws], out_dev [ndraws][n]), enqueued on
  * stream (a hipStream_t; null = the default stream) without waiting. A draw outside 1..ndpost yields NaN for its row of
- * out_dev. Calls on one trees handle must use one stream (or be ordered by the caller): the handle keeps the ranks of
- * the last rows it predicted in a buffer of its own. */
+ * out_dev. The handle keeps the ranks of the last rows it predicted in a buffer of its own; calls on different
+ * streams are ordered against each other's use of that buffer with events (the output buffers are the caller's). */
 int fbh_predict_rows_device(const fbh_trees* t, const fbh_rows* r, const int* draws_dev, int ndraws, double mu,
                             double* out_dev, void* stream);
 /* average predict-kernel duration (ms) and launch count since the trees were compiled */
 int fbh_predict_timing(const fbh_trees* t, double* avg_ms, int* nlaunch);
+/* the kernel the last prediction on t launched: form 1 = rank form (k_predict_rank), 0 = general form (k_predict),
+ * -1 = none yet; rows_per_thread = rows each thread of the rank-form kernel evaluated (1 or 2) */
+int fbh_predict_last_launch(const fbh_trees* t, int* form, int* rows_per_thread);
 /* 1 when the trees and their cutpoint table take the rank-form kernel (cutpoints of every variable nondecreasing and
  * free of NaN, at most 65,535 per variable, trees at most 255 levels deep), 0 when the general kernel is used */
 int fbh_trees_rank_form(const fbh_trees* t);

@@ -224,3 +224,36 @@ def test_external_generator_is_refused():
         assert list(st) == [-1, -1, -1]
     finally:
         L.bart_set_rng(None, None, None, None)
+
+
+def test_fx64_range_is_checked_before_any_device_work():
+    """FX64 sums int64 images (w r at 2^32, w e^2 at 2^24); fbh_batch_create refuses a chain whose outcome scale could
+    wrap them (n max(wt) (4 max|y|)^2 >= 2^38), and does not refuse the same chain under REF16 for that reason."""
+    pb = W.small_problem(4000, 3, 10, 5)
+    big = dict(pb)
+    big["y"] = (pb["y"] * np.float32(3000.0 / np.max(np.abs(pb["y"])))).astype(np.float32)  # max|y| = 3000
+    with pytest.raises(_lib.FbhError, match="FX64 range"):
+        sampler.BatchSampler([big], 2, 1, 1, [1], stats_mode=1)
+    for stats_mode, prob in ((0, big), (1, pb)):
+        try:
+            sampler.BatchSampler([prob], 2, 1, 1, [1], stats_mode=stats_mode).close()
+        except _lib.FbhError as e:       # no device here: a later failure, not the range check
+            assert "FX64 range" not in str(e)
+
+
+def test_fx64_oracle_holds_at_a_large_outcome_scale_inside_the_range(oracle):
+    """Inside the range (max|y| = 500, n = 4000: n (4 max|y|)^2 = 1.6e10 < 2^38) the FX64 twin's sigma trace stays
+    within fixed-point rounding of the REF16 trace: no sum wraps."""
+    pb = W.small_problem(4000, 3, 10, 5, weights="ones")
+    s = np.float32(500.0 / np.max(np.abs(pb["y"])))
+    prob = dict(pb)
+    prob["y"] = (pb["y"] * s).astype(np.float32)
+    prob["sigest"] = float(np.float32(pb["sigest"] * s))
+    prob["lambda"] = float(np.float32(pb["lambda"] * s * s))
+    prob["tau"] = float(np.float32(pb["tau"] * s))
+    a = oracle.fit(prob, 5, 3, 2, 7, stats_mode=0)
+    b = oracle.fit(prob, 5, 3, 2, 7, stats_mode=1)
+    sa, sb = np.asarray(a["sigma"], np.float64), np.asarray(b["sigma"], np.float64)
+    assert np.all(np.isfinite(sb)) and np.all(sb > 0)
+    # the two modes take the same moves while their sums agree to rounding: the traces agree closely
+    assert np.max(np.abs(sa - sb) / sa) < 1e-3

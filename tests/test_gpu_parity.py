@@ -183,3 +183,21 @@ def test_gpu_probit_workload_chain_matches_oracle(oracle):
     assert same_bits(s["latent"], e["latent"])
     assert same_bits(out["trees"].reshape(-1), ch.trees_out)
     assert same_bits(out["trmean"], ch.trmean())
+
+
+@pytest.mark.parametrize("name", ["fwbart_n4096", "fpbart_n24576"])
+def test_gpu_workload_chain_full_length_matches_oracle(oracle, name):
+    """The single-chain workloads as configured: all 1,100 sweeps (nskip 1000, ndpost 100), every output bit for bit
+    against the oracle (sigma trace, posterior-mean fit, the 100 kept draws' tree words, move counts)."""
+    if name == "fwbart_n4096":
+        pb, m, probit = W.workload_fwbart_n4096(), 200, 0
+    else:
+        pb, m, probit = W.workload_fpbart_n24576(), 50, 1
+    nskip, nd = 1000, 100
+    _say("full-length chain", name)
+    with sampler.BatchSampler([pb], m, nskip, nd, [1], probit=probit) as b:
+        b.run(nskip + nd)
+        out = b.fetch(0)
+    e = oracle.fit(pb, m, nskip, nd, 1, probit=probit)
+    for k in ("sigma", "trmean", "trees", "stats"):
+        assert same_bits(out[k], e[k]), k

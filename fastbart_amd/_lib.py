@@ -64,7 +64,7 @@ EXPORTS = ["bart_fit_cpu", "bart_wfit", "bart_set_rng", "fbh_last_status", "fbh_
            "fbh_batch_yhat", "fbh_batch_timing", "fbh_batch_destroy", "fbh_fit_batch", "fbh_tree_text", "fbh_yhat",
            "fbh_compile_trees", "fbh_trees_info", "fbh_trees_arrays", "fbh_trees_free", "fbh_predict_draw",
            "fbh_predict_draws", "fbh_rows_upload", "fbh_rows_free", "fbh_predict_rows", "fbh_predict_timing",
-           "fbh_predict_rows_device", "fbh_predict_last_launch", "fbh_trees_rank_form", "fbh_device_alloc", "fbh_device_free",
+           "fbh_predict_rows_device", "fbh_predict_rows_device_noalloc", "fbh_predict_last_launch", "fbh_trees_rank_form", "fbh_device_alloc", "fbh_device_free",
            "fbh_device_copy", "fbh_device_sync", "fbh_batch_row_split", "fbh_batch_configure", "fbh_batch_config",
            "fbh_batch_set_order",
            "fbh_batch_autotune",
@@ -116,6 +116,7 @@ def _declare(L):
     L.fbh_predict_rows.argtypes = [vp, vp, vp, i, d, vp]
     L.fbh_predict_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i)]
     L.fbh_predict_rows_device.argtypes = [vp, vp, vp, i, d, vp, vp]
+    L.fbh_predict_rows_device_noalloc.argtypes = [vp, vp, vp, i, d, vp, vp]
     L.fbh_predict_last_launch.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
     L.fbh_trees_rank_form.argtypes = [vp]
     L.fbh_device_alloc.argtypes = [C.c_size_t, i, C.POINTER(vp)]

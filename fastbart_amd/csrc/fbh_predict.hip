@@ -513,6 +513,8 @@ static void build_rank_form(fbh_trees* t, const double* cutpoints, const int* cu
   t->rank_ok = true;
 }
 
+static int upload_trees(fbh_trees* t);
+
 extern "C" int fbh_compile_trees(const char* text, size_t len, const double* cutpoints, const int* cut_off, int pcut,
                                  int device, fbh_trees** out) {
   if (!text || !out || (pcut > 0 && (!cutpoints || !cut_off))) {
@@ -612,6 +614,17 @@ extern "C" int fbh_compile_trees(const char* text, size_t len, const double* cut
   }
   t->draw_lo[t->ndpost] = (int)t->var.size();
   build_rank_form(t, cutpoints, cut_off, pcut, cidx, tdepth);
+  // The device copies are made now when the device is visible, so that every prediction entry point (the
+  // allocation-free one included) finds them in place; on a host without the device they wait for the first
+  // prediction.
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) == hipSuccess && device >= 0 && device < ndev) {
+    const int rc = upload_trees(t);
+    if (rc) {
+      free_trees(t);
+      return rc;
+    }
+  }
   *out = t;
   return FBH_OK;
 }
@@ -758,6 +771,31 @@ static size_t rank_lds_limit(int device) {
   return lim[device];
 }
 
+// The predict kernel of ndraws draws (device array d_draws, 1-based) for the rows r into d_out [ndraws][n] on stream
+// s: the rank form reads the rows' ranks from rank (filled by k_rank), the general form the rows themselves. Caller
+// holds t->mu.
+static int predict_kernels(fbh_trees* t, const fbh_rows* r, const int* d_draws, int ndraws, double mu, double* d_out,
+                           hipStream_t s, bool rank_form, int rows_per_thread, size_t lds, const uint16_t* rank) {
+  t->last_form = rank_form ? 1 : 0;
+  t->last_rows_per_thread = rank_form ? rows_per_thread : 1;
+  if (rank_form) {
+    const int rows = kRankThreads * rows_per_thread;
+    dim3 grid((r->n + rows - 1) / rows, ndraws);
+    if (rows_per_thread == 2)
+      hipLaunchKernelGGL(k_predict_rank<2>, grid, dim3(kRankThreads), lds, s, t->d_node, t->d_theta, t->d_tinfo,
+                         t->d_lo, d_draws, rank, r->n, r->p, t->ntree, t->ndpost, mu, d_out);
+    else
+      hipLaunchKernelGGL(k_predict_rank<1>, grid, dim3(kRankThreads), lds, s, t->d_node, t->d_theta, t->d_tinfo,
+                         t->d_lo, d_draws, rank, r->n, r->p, t->ntree, t->ndpost, mu, d_out);
+  } else {
+    dim3 grid((r->n + kPredThreads - 1) / kPredThreads, ndraws);
+    hipLaunchKernelGGL(k_predict, grid, dim3(kPredThreads), 0, s, t->d_var, t->d_child, t->d_cut, t->d_theta,
+                       t->d_root, d_draws, t->d_lo, r->d_x, r->n, t->ntree, t->ndpost, mu, d_out);
+  }
+  FBH_HIP_CHECK(hipGetLastError());
+  return FBH_OK;
+}
+
 // Launch the prediction of ndraws draws (device array d_draws, 1-based, checked by the caller) for the rows r into
 // d_out [ndraws][n] on stream s. Caller holds t->mu.
 static int predict_launch(fbh_trees* t, const fbh_rows* r, const int* d_draws, int ndraws, double mu, double* d_out,
@@ -814,24 +852,9 @@ static int predict_launch(fbh_trees* t, const fbh_rows* r, const int* d_draws, i
     // ranks computed on another stream: this launch reads them only once they are complete
     FBH_HIP_CHECK(hipStreamWaitEvent(s, t->rank_ready, 0));
   }
-  t->last_form = rank_form ? 1 : 0;
-  t->last_rows_per_thread = rank_form ? rows_per_thread : 1;
   FBH_HIP_CHECK(hipEventRecord(t->ev0[slot], s));
-  if (rank_form) {
-    const int rows = kRankThreads * rows_per_thread;
-    dim3 grid((r->n + rows - 1) / rows, ndraws);
-    if (rows_per_thread == 2)
-      hipLaunchKernelGGL(k_predict_rank<2>, grid, dim3(kRankThreads), lds, s, t->d_node, t->d_theta, t->d_tinfo,
-                         t->d_lo, d_draws, t->d_rank, r->n, r->p, t->ntree, t->ndpost, mu, d_out);
-    else
-      hipLaunchKernelGGL(k_predict_rank<1>, grid, dim3(kRankThreads), lds, s, t->d_node, t->d_theta, t->d_tinfo,
-                         t->d_lo, d_draws, t->d_rank, r->n, r->p, t->ntree, t->ndpost, mu, d_out);
-  } else {
-    dim3 grid((r->n + kPredThreads - 1) / kPredThreads, ndraws);
-    hipLaunchKernelGGL(k_predict, grid, dim3(kPredThreads), 0, s, t->d_var, t->d_child, t->d_cut, t->d_theta,
-                       t->d_root, d_draws, t->d_lo, r->d_x, r->n, t->ntree, t->ndpost, mu, d_out);
-  }
-  FBH_HIP_CHECK(hipGetLastError());
+  rc = predict_kernels(t, r, d_draws, ndraws, mu, d_out, s, rank_form, rows_per_thread, lds, t->d_rank);
+  if (rc) return rc;
   FBH_HIP_CHECK(hipEventRecord(t->ev1[slot], s));
   t->ev_live[slot] = true;
   if (rank_form) {
@@ -898,6 +921,75 @@ extern "C" int fbh_predict_rows_device(const fbh_trees* tc, const fbh_rows* r, c
   }
   std::lock_guard<std::mutex> lk(t->mu);
   return predict_launch(t, r, draws_dev, ndraws, mu, out_dev, reinterpret_cast<hipStream_t>(stream));
+}
+
+// ------------------------------------------------------------------ allocation-free prediction
+// The rows' ranks of fbh_predict_rows_device_noalloc live in `scratch`, a file-scope device array: every GPU the library
+// runs on holds its own copy (at its own address) until the process ends. Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in symbol): the device address comes from hipGetSymbolAddress, taken once per
+// device with that device current, and only that address is used.
+constexpr size_t kNoAllocRanks = size_t(8) << 20;  // 8 Mi ranks = 16 MiB: rows x predictors of one call at most
+static __device__ __align__(256) uint16_t scratch[kNoAllocRanks];
+static std::mutex g_scratch_mu;
+static uint16_t* g_scratch_addr[64];  // per device ordinal: the device address of its copy of scratch
+
+static int scratch_address(int device, uint16_t** out) {
+  if (device < 0 || device >= 64) {
+    set_error(FBH_EINVAL, "fbh_predict_rows_device_noalloc: device ordinal out of range");
+    return FBH_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    FBH_HIP_CHECK(hipSetDevice(device));
+    void* p = nullptr;
+    FBH_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      set_error(FBH_EHIP, "fbh_predict_rows_device_noalloc: no device address for the rank scratch");
+      return FBH_EHIP;
+    }
+    g_scratch_addr[device] = static_cast<uint16_t*>(p);
+  }
+  *out = g_scratch_addr[device];
+  return FBH_OK;
+}
+
+extern "C" int fbh_predict_rows_device_noalloc(const fbh_trees* tc, const fbh_rows* r, const int* draws_dev,
+                                               int ndraws, double mu, double* out_dev, void* stream) {
+  fbh_trees* t = const_cast<fbh_trees*>(tc);
+  int rc = check_predict(t, r, ndraws);
+  if (rc) return rc;
+  if (ndraws > 0 && (!draws_dev || !out_dev)) {
+    set_error(FBH_EINVAL, "fbh_predict_rows_device_noalloc: null draws or output");
+    return FBH_EINVAL;
+  }
+  if ((size_t)r->n * (size_t)r->p > kNoAllocRanks) {
+    set_error(FBH_EINVAL, "fbh_predict_rows_device_noalloc: rows x predictors = " +
+                              std::to_string((size_t)r->n * (size_t)r->p) + " exceeds the scratch of " +
+                              std::to_string(kNoAllocRanks) + " ranks");
+    return FBH_EINVAL;
+  }
+  if (ndraws == 0 || r->n == 0) return FBH_OK;
+  std::lock_guard<std::mutex> lk(t->mu);
+  if (!t->d_var) {
+    set_error(FBH_EINVAL, "fbh_predict_rows_device_noalloc: the trees are not on the device (compiled while no "
+                          "device was visible)");
+    return FBH_EINVAL;
+  }
+  uint16_t* ranks = nullptr;
+  if ((rc = scratch_address(t->device, &ranks))) return rc;
+  FBH_HIP_CHECK(hipSetDevice(t->device));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const long blocks2 = (long)((r->n + kRankRowsMax - 1) / kRankRowsMax) * ndraws;
+  const int rows_per_thread = blocks2 >= 512 ? 2 : 1;
+  const size_t lds = rank_lds_bytes(t->max_draw_nodes, t->ntree, r->p, kRankThreads * rows_per_thread);
+  const bool rank_form = t->rank_ok && lds <= rank_lds_limit(t->device);
+  if (rank_form && r->p > 0) {
+    // every rank the predict kernel reads is written here first: nothing carries over from an earlier call
+    hipLaunchKernelGGL(k_rank, dim3((r->n + 255) / 256, r->p), dim3(256), 0, s, r->d_x, t->d_xi, t->d_xoff, r->n, r->p,
+                       t->pcut, ranks);
+    FBH_HIP_CHECK(hipGetLastError());
+  }
+  return predict_kernels(t, r, draws_dev, ndraws, mu, out_dev, s, rank_form, rows_per_thread, lds, ranks);
 }
 
 extern "C" int fbh_predict_timing(const fbh_trees* tc, double* avg_ms, int* nlaunch) {

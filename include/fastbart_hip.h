@@ -208,6 +208,14 @@ int fbh_predict_rows(const fbh_trees* t, const fbh_rows* r, const int* draws, in
  * streams are ordered against each other's use of that buffer with events (the output buffers are the caller's). */
 int fbh_predict_rows_device(const fbh_trees* t, const fbh_rows* r, const int* draws_dev, int ndraws, double mu,
                             double* out_dev, void* stream);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): fbh_predict_rows_device without any allocation. The rows' cutpoint
+ * ranks go to a fixed per-GPU scratch array of 8 Mi ranks (16 MiB, held by every GPU the library runs on until the
+ * process ends) instead of the handle's buffer; r->n x r->p above 8,388,608 is refused (FBH_EINVAL). The trees must be
+ * on the device (fbh_compile_trees uploads them when the device is visible). Calls that overlap on one GPU share that
+ * scratch and race: order them on one stream. Same outputs as fbh_predict_rows_device, bit for bit. */
+int fbh_predict_rows_device_noalloc(const fbh_trees* t, const fbh_rows* r, const int* draws_dev, int ndraws,
+                                    double mu, double* out_dev, void* stream);
 /* average predict-kernel duration (ms) and launch count since the trees were compiled */
 int fbh_predict_timing(const fbh_trees* t, double* avg_ms, int* nlaunch);
 /* the kernel the last prediction on t launched: form 1 = rank form (k_predict_rank), 0 = general form (k_predict),

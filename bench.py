@@ -99,6 +99,29 @@ def _cpu_worker(args):
     return m * sweeps, time.perf_counter() - t0, os.getpid()
 
 
+def host_cpu_info() -> dict:
+    """The host the CPU baseline ran on: logical CPUs, the process's affinity, the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)), cpu_model=model)
+
+
+# The CPU baseline uses at most this many processes: the GPU host's CPU share for one GPU (the rest of the machine's
+# cores serve other GPUs' jobs), whatever the machine's core count.
+CPU_SHARE = 16
+
+
+def baseline_cores() -> int:
+    return min(CPU_SHARE, len(os.sched_getaffinity(0)))
+
+
 def cpu_baseline(workload: str, cores: int):
     from oracle import pyoracle
     kind = "reference" if pyoracle.RefCore.available() else "port"
@@ -127,13 +150,16 @@ def cpu_baseline(workload: str, cores: int):
     m = WORKLOADS[workload][1]
     note = " (REF16: the reference has no FX64 mode; same rows and trees)" if workload == "maxshape64" else ""
     return dict(value=updates / cpu_s if nch == 1 else updates / makespan, unit="tree-updates/s", cores=cores,
-                kind=kind,
+                kind=kind, host=host_cpu_info(), core_cap=f"at most {CPU_SHARE} processes: the host's CPU share "
+                                                         f"for one GPU",
                 sample=f"{len(jobs)} chain(s) of the workload x {jobs[0][2]} sweeps x {m} trees, one chain at a time "
                        f"per process on {cores} core(s) ({cpu_s:.1f} CPU-s of fits; busiest process {makespan:.2f} s; "
                        f"{wall:.1f} s wall with start-up; {updates / cpu_s:.0f} tree-updates/s per core){note}")
 
 
 # ------------------------------------------------------------------ ESS (Geyer initial monotone sequence)
+ESS_MIN_DRAWS = 100  # fewer kept draws leave Geyer's estimator at its floor for these strongly autocorrelated chains
+
 def ess_geyer_cols(x: np.ndarray) -> np.ndarray:
     """ESS of every column of x [draws][series]: autocovariances by FFT, Geyer's initial monotone sequence of pair
     sums (Geyer 1992), ESS = n / (-1 + 2 sum of the monotone pair sums)."""
@@ -171,6 +197,11 @@ def min_ess_chain(sigma_kept, yhat: np.ndarray, max_rows: int = 0) -> float:
 # sweeps the autotuner takes at one sweep per trial launch: a warm-up launch, two rounds of at most 2 builds x 3 tile
 # kinds, two rounds of 3 packings
 TUNE_SWEEPS_MAX = 1 + 2 * 6 + 2 * 3
+
+
+TRAFFIC_SOURCE = ("profiles/k_sweeps_{}_traffic.txt: FETCH_SIZE + WRITE_SIZE per sweep from two rocprofv3 --pmc "
+                  "runs of this workload, calibrated against a plain copy kernel, scaled to this run's sweeps per "
+                  "launch (not measured in this run)")
 
 
 def measured_traffic(workload: str, sweeps_per_launch: float):
@@ -468,8 +499,7 @@ def main() -> int:
     if wl == "learned512":
         cpu = None
         if not args.no_cpu:
-            cores = min(16, len(os.sched_getaffinity(0)))
-            cpu = cpu_baseline("batch512", cores)
+            cpu = cpu_baseline("batch512", baseline_cores())
         return learned_vs_plain(args, chains, NSKIP, args.ndpost or NDPOST, cpu)
     W_ = max(0, args.warmup if args.warmup is not None else w_def)
     if args.packing == "auto":
@@ -483,7 +513,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        cores = min(16, len(os.sched_getaffinity(0)))
+        cores = baseline_cores()
         log(f"cpu baseline on up to {cores} cores ...")
         cpu = cpu_baseline(wl, cores)
         log("cpu baseline:", cpu)
@@ -545,8 +575,11 @@ def main() -> int:
     achieved = bytes_per_sweep * sweeps_launch / (ms_launch / 1e3) / 1e9 if nlt else 0.0
     traffic = measured_traffic(wl, sweeps_launch)
 
-    ess = None
-    if ndpost >= 10:
+    ess, ess_note = None, None
+    if ndpost < ESS_MIN_DRAWS:
+        ess_note = (f"not computed: {ndpost} kept draws per chain; Geyer's estimator needs at least {ESS_MIN_DRAWS} "
+                    f"for a figure that is not at its floor (run the whole fit: no --steps)")
+    else:
         t2 = time.perf_counter()
         tot = 0.0
         for c in range(chains):
@@ -554,6 +587,8 @@ def main() -> int:
             yh = b.yhat(c)
             tot += min_ess_chain(None if probit else out["sigma"][nskip:], yh, args.ess_rows)
         ess = tot / (t_w + T)
+        ess_note = (f"sum over chains of min(ESS(sigma), min_i ESS(f(x_i))) over {ndpost} kept draws, Geyer's initial "
+                    f"monotone sequence estimator, / (warmup + timed) wall time")
         log(f"min-ESS: sum over chains {tot:.1f} ({time.perf_counter() - t2:.1f} s to compute)")
     b.close()
     if args.calib_copy:
@@ -572,9 +607,14 @@ def main() -> int:
                    "fit_in_lds": launch_cfg["resident"] == 2,
                    "chains_per_cu": launch_cfg["chains_per_cu"], "graph": bool(launch_cfg["graph"]),
                    "autotuned": tuned is not None},
+        "warmup_requested": args.warmup,
+        "warmup_note": (f"{W_} warmup sweeps run: the autotuner's trials take up to {TUNE_SWEEPS_MAX} of them"
+                        if W_ != (args.warmup if args.warmup is not None else w_def) else None),
         "min_ess_per_sec": ess,
+        "min_ess_note": ess_note,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": TRAFFIC_SOURCE.format(wl) if traffic is not None else None,
                      "kernel": "k_sweeps", "kernel_avg_ms": ms_launch, "launches": nlt,
                      "sweeps_per_launch": sweeps_launch, "bytes_per_tree_update_row": 37,
                      "algorithmic_bytes_per_launch": bytes_per_sweep * sweeps_launch},

@@ -1651,16 +1651,14 @@ __device__ void lp_choose(Ctl& c, Rng& g, int lane, int* mode_out, int* nx_out) 
   }
   const int ngood = __popcll(goodm), nnogs = __popcll(nogm);
   const double PBx = ngood == 0 ? 0 : (used == 1ull ? 1 : 0.5);
+  // the same draws as the plain proposal (:431, :433, :455): a death is drawn whenever a birth is not
   int mode, nx;
   if (unif_d(g) < PBx) {
     mode = 1;
     nx = select_kth(goodm, (int)(unif_d(g) * ngood));
-  } else if (nnogs > 0) {
+  } else {
     mode = 2;
     nx = select_kth(nogm, (int)(unif_d(g) * nnogs));
-  } else {
-    mode = 0;
-    nx = 0;
   }
   if (lane == 0) {
     c.nleaf[0] = __popcll(lm);
@@ -1837,15 +1835,17 @@ __global__ __launch_bounds__(kBlock) void k_lp_stage2(LpArgs A) {
     double lpr = 0;
     uint64_t cand = 0;
     if (mode == 1) {
-      // the rule: uniform with probability 1 - alpha (:435-437), else from the network
+      // The rule: uniform with probability 1 - alpha (:435-437), else from the network. The mixture uniform is drawn
+      // only when alpha > 0, so that with alpha = 0 the generator stream is the plain proposal's, draw for draw
+      // (the cut is drawn, as in :436-437, only when the range is not empty and the node is above the depth limit).
       int v = -1, cut = -1;
-      const double um = unif_d(g);
-      if (!(alpha > 0.f) || um >= (double)alpha) {
+      const bool deep_ok = (int)t.d[nx] + 1 < kMaxDepth;
+      if (!(alpha > 0.f) || unif_d(g) >= (double)alpha) {
         v = (int)(unif_d(g) * p);
         if (v >= p) v = p - 1;
         const uint16_t r = rg[v];
         const int L = r & 255, U = r >> 8;
-        if (U >= L) {
+        if (U >= L && deep_ok) {
           cut = L + (int)(unif_d(g) * (U - L + 1));
           if (cut > U) cut = U;
         }
@@ -1889,7 +1889,7 @@ __global__ __launch_bounds__(kBlock) void k_lp_stage2(LpArgs A) {
           acc += (double)__shfl(pre, 63);
         }
       }
-      if (v >= 0 && cut >= 0 && (int)t.d[nx] + 1 < kMaxDepth) {
+      if (v >= 0 && cut >= 0 && deep_ok) {
         const uint16_t r = rg[v];
         const int bL = r & 255, bU = r >> 8;
         const int dnx = t.d[nx];

@@ -1,0 +1,119 @@
+"""The learned-proposal pipeline on chains of the 512-chain workload (n_i 3,091..4,573, p_i 15..19, 255 cutpoints,
+200 trees).
+
+With alpha = 0 the lock-step pipeline draws exactly the plain proposal's generator outputs (src/bart_kernel.cpp:431-437,
+:455, :482, :521-523), so its outputs equal the oracle's bit for bit, also when a fit switches between the persistent
+kernel and the pipeline. With alpha = 0.8 (the network in the mixture) the chain is a different Markov chain with the
+same posterior: checked here by full-size properties -- every kept tree is a valid tree of the chain's variables and
+cutpoints, sigma stays finite, births are accepted, and the posterior-mean fit agrees with the plain fit's as closely
+as a plain fit with other seeds does."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import same_bits
+from fastbart_amd import sampler
+from fastbart_amd import workloads as W
+from fastbart_amd.mlp import MLP
+
+pytestmark = pytest.mark.gpu
+
+NCH = 64
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _check_tree_words(words, numcut):
+    """One tree's 64 output words (v | c << 8 | l << 16 | r << 24 | theta << 32; v = 255 marks an unused slot) form a
+    binary tree from slot 0: internal nodes split a variable of the chain on one of its cutpoints, every used slot is
+    reached exactly once, leaves carry finite values."""
+    w = np.asarray(words, np.uint64)
+    v = (w & 0xFF).astype(int)
+    c = ((w >> 8) & 0xFF).astype(int)
+    l = ((w >> 16) & 0xFF).astype(int)
+    r = ((w >> 24) & 0xFF).astype(int)
+    th = (w >> 32).astype(np.uint32).view(np.float32)
+    used = v != 255
+    assert used[0]
+    seen = np.zeros(64, bool)
+    stack = [0]
+    while stack:
+        s = stack.pop()
+        assert used[s] and not seen[s]
+        seen[s] = True
+        if l[s]:
+            assert r[s] and v[s] < len(numcut) and c[s] < numcut[v[s]]
+            stack += [l[s], r[s]]
+        else:
+            assert np.isfinite(th[s])
+    assert np.array_equal(seen, used)
+
+
+def test_gpu_learned_alpha0_equals_the_oracle_bit_for_bit(oracle):
+    """64 chains, 10 sweeps: 3 in the persistent kernel, then 7 in the lock-step pipeline with alpha = 0 (BART's
+    uniform rule), against the oracle's plain fit; the pipeline alone from the start as well."""
+    probs = [W.batch512_chain(c) for c in range(NCH)]
+    seeds = [1 + c for c in range(NCH)]
+    m, nskip, nd = 200, 6, 4
+    for first_plain in (3, 0):
+        _say("learned alpha 0 after", first_plain, "plain sweeps")
+        with sampler.BatchSampler(probs, m, nskip, nd, seeds) as b:
+            if first_plain:
+                b.run(first_plain)
+            b.learned_setup(None, 0.0)
+            b.run_learned(nskip + nd - first_plain)
+            outs = [b.fetch(c) for c in range(NCH)]
+        for c in (0, 1, 7, 30, 63) if first_plain == 0 else range(NCH):
+            e = oracle.fit(probs[c], m, nskip, nd, seeds[c])
+            for k in ("sigma", "trmean", "trees", "stats"):
+                assert same_bits(outs[c][k], e[k]), (first_plain, c, k)
+
+
+def test_gpu_learned_alpha08_properties_at_full_size():
+    """64 chains: a plain fit, and the learned schedule on the same chains and seeds (plain burn-in, births collected
+    under BART's proposal, the network trained on them, then alpha = 0.8 through the kept draws)."""
+    probs = [W.batch512_chain(c) for c in range(NCH)]
+    seeds = [1 + c for c in range(NCH)]
+    m, nskip, nd = 200, 200, 100
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds, sweeps_per_launch=20) as b:
+        b.run(nskip + nd)
+        plain = [b.fetch(c) for c in range(NCH)]
+    # the seed-to-seed agreement of two plain fits sets the scale of the posterior-mean comparison
+    with sampler.BatchSampler(probs, m, nskip, nd, [1000 + s for s in seeds], sweeps_per_launch=20) as b:
+        b.run(nskip + nd)
+        plain2 = [b.fetch(c)["trmean"].astype(np.float64) for c in range(NCH)]
+    net = MLP((513, 256, 256, 8160), max_batch=512, seed=5)
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds, sweeps_per_launch=20) as b:
+        b.run(100)
+        b.learned_setup(net, 0.0, pair_capacity=200000)
+        b.run_learned(20)
+        npairs = b.learned_pairs()
+        losses = b.learned_train(300, batch=512, lr=1e-3, seed=3)
+        b.learned_setup(net, 0.8)
+        b.run_learned(nskip + nd - 120)
+        learned = [b.fetch(c) for c in range(NCH)]
+    _say("pairs", npairs, "loss", float(losses[:10].mean()), "->", float(losses[-10:].mean()))
+    assert npairs > 1000 and np.isfinite(losses).all()
+    births = sum(int(o["stats"][0]) for o in learned) - sum(int(o["stats"][0]) for o in plain)
+    _say("births learned - plain", births)
+    corr = []
+    for c in range(NCH):
+        o = learned[c]
+        assert np.all(np.isfinite(o["sigma"])) and np.all(o["sigma"] > 0)
+        assert o["stats"][0] > 0 and o["stats"][1] > 0
+        numcut = probs[c]["numcut"]
+        for d in (0, nd // 2, nd - 1):
+            for j in range(0, m, 7):
+                _check_tree_words(o["trees"][d, j], numcut)
+        corr.append(np.corrcoef(o["trmean"].astype(np.float64), plain[c]["trmean"].astype(np.float64))[0, 1])
+    corr = np.array(corr)
+    ref = np.array([np.corrcoef(plain2[c], plain[c]["trmean"].astype(np.float64))[0, 1] for c in range(NCH)])
+    _say("posterior-mean correlation with the plain fit: mean", corr.mean(), "min", corr.min(),
+         "| two plain fits, other seeds: mean", ref.mean(), "min", ref.min())
+    # as close to the plain fit as a plain fit with other seeds is
+    assert corr.mean() >= ref.mean() - 0.004 and corr.min() >= ref.min() - 0.01
+    assert corr.mean() >= 0.985

@@ -716,15 +716,19 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
 #pragma unroll
   for (int g = 0; g < kGroups; g++) acc[g] = 0;
   const int nper_kb = kb < n ? (n - kb + kLanes - 1) / kLanes : 0;
+  // rows of the birth proposal's halves in this wave's lanes (wave-uniform: counted with ballots; lane 0 takes part in
+  // every ballot, as a quad of a higher lane starts below n only if lane 0's does)
   int my_nl = 0, my_nr = 0;
 
   if (!first_loaded) tile_load<R, RES>(T, P, la, lb, qa, qb, 0, tq, qlast);
   for (int tb = 0; tb < n; tb += G::kTile) {
     const int ii0 = tb / kLanes;
-    // ---- phase A: the arithmetic on the tile's loads (issued before the previous tile's phase B). Per batch of RC rows
-    // of a quad, first the leaf ids after tree j-1's change and every LDS gather (leaf values, ranks, the resident
-    // fit), all in flight together; then the arithmetic and the stores. Rows past n compute on the loaded values and
-    // store nothing global; a quad past n, or a slot past the tile, does nothing.
+    // ---- phase A: the arithmetic on the tile's loads (issued before the previous tile's phase B). First the quad's
+    // leaf ids after tree j-1's change (the move type is the same for the whole workgroup), then per batch of RC rows
+    // every LDS gather (leaf values, ranks, the resident fit), all in flight together, then the arithmetic and the
+    // stores. Rows of the quad at or past n compute on whatever they load and store into padding that nothing reads
+    // as a row: the global pools' padding rows, and in LDS their own slots (the running fit in LDS has a slot for
+    // every row of the chain's last 16-row group); a quad past n, or a slot past the tile, does nothing.
     constexpr int S = TileIn<R, YF, AFT>::S;
     constexpr int RC = D ? 1 : 2;  // rows per gather batch (batches keep the registers in budget)
 #pragma unroll
@@ -732,23 +736,38 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
       const int idx = lane + 64 * s;  // the slot within the tile
       const uint32_t r0 = (uint32_t)((ii0 + 64 * s) * kLanes) + tq;
       if (idx < G::kTW4 && r0 < (uint32_t)n) {
-        uint32_t lnew = 0;  // the quad's leaf ids after the change, a byte a row
+        // the quad's rows r0 .. r0 + 3 lie in one 16-row group: in the LDS running fit they are afl(r0) + u
+        const uint32_t a0 = afl(r0);
+        // the quad's leaf ids after the change, a byte a row: a birth splits leaf pnx on the rank, a death merges
+        // pl and pr into pnx
+        uint32_t lnew = T.la[s];
+        if (apply && pmode == 1) {
+          lnew = 0;
+#pragma unroll
+          for (int u = 0; u < kLanesPerWave; u++) {
+            const uint32_t lf = (T.la[s] >> (8 * u)) & 255u, qv = (T.qa[s] >> (8 * u)) & 255u;
+            const uint32_t lf2 = lf == (uint32_t)pnx ? (qv <= (uint32_t)pc ? (uint32_t)pL : (uint32_t)pR) : lf;
+            lnew |= lf2 << (8 * u);
+          }
+        } else if (apply && pmode == 2) {
+          lnew = 0;
+#pragma unroll
+          for (int u = 0; u < kLanesPerWave; u++) {
+            const uint32_t lf = (T.la[s] >> (8 * u)) & 255u;
+            const uint32_t lf2 = (lf == (uint32_t)pl || lf == (uint32_t)pr) ? (uint32_t)pnx : lf;
+            lnew |= lf2 << (8 * u);
+          }
+        }
 #pragma unroll
         for (int u0 = 0; u0 < kLanesPerWave; u0 += RC) {
-          int lfn[RC];
           double tho_r[RC], thn_r[RC], thc_r[RC], af_r[RC];
           uint32_t rank_r[RC];
 #pragma unroll
           for (int k = 0; k < RC; k++) {
             const int u = u0 + k;
-            // tree j-1's change as selects (no branch): a birth splits leaf pnx on the rank, a death merges pl, pr
-            const int lf = (int)((T.la[s] >> (8 * u)) & 255u);
-            const bool born = pmode == 1 && lf == pnx, merged = pmode == 2 && (lf == pl || lf == pr);
-            const int lf2 = born ? ((int)((T.qa[s] >> (8 * u)) & 255u) <= pc ? pL : pR) : merged ? pnx : lf;
-            lfn[k] = lf2;
             if (apply) {
-              tho_r[k] = c.tho[lf];
-              thn_r[k] = c.thn[lf2];
+              tho_r[k] = c.tho[(T.la[s] >> (8 * u)) & 255u];
+              thn_r[k] = c.thn[(lnew >> (8 * u)) & 255u];
             }
             if (!rss) {
               const uint32_t lb_u = (T.lb[s] >> (8 * u)) & 255u;
@@ -756,21 +775,16 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
               rank_r[k] = rank_of[lb_u];
             }
             if constexpr (AFT) af_r[k] = d4_at(T.af[s], u);
-            else af_r[k] = P.AFl[afl(min(r0 + (uint32_t)u, (uint32_t)(n - 1)))];
+            else af_r[k] = P.AFl[a0 + (uint32_t)u];
           }
 #pragma unroll
           for (int k = 0; k < RC; k++) {
             const int u = u0 + k;
-            // Rows of the quad at or past n store too, unconditionally, into padding nobody reads as a row: the
-            // global pools' padding rows, or in LDS the pad slot after the row's 16-row group (the LDS running fit
-            // is sized for n rounded up to 16 rows)
             const uint32_t i = r0 + (uint32_t)u;
-            const bool valid = i < (uint32_t)n;
             double af = af_r[k];
             if (apply) {
               af = af - tho_r[k] + thn_r[k];
-              lnew |= (uint32_t)lfn[k] << (8 * u);
-              if (RES) P.AFl[valid ? afl(i) : afl(i | 15u) + 1u] = af;
+              if (RES) P.AFl[a0 + (uint32_t)u] = af;
               else sto(P.AF, i, af);
             }
             double y;
@@ -785,13 +799,16 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
               tga[u * G::kTagStride + idx] = 0;
               if (keep) sto(P.TM, i, ldo((const double*)P.TM, i) + af);
             } else {
-              const int lf = (int)((T.lb[s] >> (8 * u)) & 255u);
               const double rr = y - af + thc_r[k];
               const double wr = w * rr;
-              const bool split = cmode == 1 && lf == cnx && valid;
-              const int side = split ? ((int)((T.qb[s] >> (8 * u)) & 255u) <= cc ? 1 : 2) : 0;
-              my_nl += side == 1;
-              my_nr += side == 2;
+              uint32_t side = 0;
+              if (cmode == 1) {
+                const uint32_t lf = (T.lb[s] >> (8 * u)) & 255u;
+                const bool split = lf == (uint32_t)cnx && i < (uint32_t)n;
+                side = split ? (((T.qb[s] >> (8 * u)) & 255u) <= (uint32_t)cc ? 1u : 2u) : 0u;
+                my_nl += __popcll(__ballot(side == 1u));
+                my_nr += __popcll(__ballot(side == 2u));
+              }
               v0a[u * G::kStride + idx] = FX ? __longlong_as_double(fx_round(w, 4294967296.0)) : w;
               v1a[u * G::kStride + idx] = FX ? __longlong_as_double(fx_round(wr, 4294967296.0)) : wr;
               tga[u * G::kTagStride + idx] = (uint8_t)(rank_r[k] | (side << 6));
@@ -823,8 +840,8 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
     }
     wave_sync();
   }
-  if (my_nl) atomicAdd(&c.nL, my_nl);
-  if (my_nr) atomicAdd(&c.nR, my_nr);
+  if (lane == 0 && my_nl) atomicAdd(&c.nL, my_nl);
+  if (lane == 0 && my_nr) atomicAdd(&c.nR, my_nr);
 #pragma unroll
   for (int g = 0; g < kGroups; g++) {
     const int task = slot + kTaskSlots * g;

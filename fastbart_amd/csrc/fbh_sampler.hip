@@ -801,17 +801,19 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
             } else {
               const double rr = y - af + thc_r[k];
               const double wr = w * rr;
-              uint32_t side = 0;
+              // tag: the leaf rank, bit 6 / 7 for the left / right half of the birth proposal's node
+              uint32_t tag = rank_r[k];
               if (cmode == 1) {
-                const uint32_t lf = (T.lb[s] >> (8 * u)) & 255u;
-                const bool split = lf == (uint32_t)cnx && i < (uint32_t)n;
-                side = split ? (((T.qb[s] >> (8 * u)) & 255u) <= (uint32_t)cc ? 1u : 2u) : 0u;
-                my_nl += __popcll(__ballot(side == 1u));
-                my_nr += __popcll(__ballot(side == 2u));
+                const bool split = ((T.lb[s] >> (8 * u)) & 255u) == (uint32_t)cnx && i < (uint32_t)n;
+                const bool left = ((T.qb[s] >> (8 * u)) & 255u) <= (uint32_t)cc;
+                const bool isl = split && left, isr = split && !left;
+                my_nl += __popcll(__ballot(isl));
+                my_nr += __popcll(__ballot(isr));
+                tag |= isl ? 64u : (isr ? 128u : 0u);
               }
               v0a[u * G::kStride + idx] = FX ? __longlong_as_double(fx_round(w, 4294967296.0)) : w;
               v1a[u * G::kStride + idx] = FX ? __longlong_as_double(fx_round(wr, 4294967296.0)) : wr;
-              tga[u * G::kTagStride + idx] = (uint8_t)(rank_r[k] | (side << 6));
+              tga[u * G::kTagStride + idx] = (uint8_t)tag;
             }
           }
         }

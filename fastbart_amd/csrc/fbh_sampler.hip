@@ -304,18 +304,22 @@ __device__ __forceinline__ double latent_value(const LatentRow& r, uint32_t x1, 
 }
 
 // ------------------------------------------------------------------ control state of one chain's workgroup
+struct ThR {  // a slot's leaf value and, for a used leaf, its rank among the tree's leaves (a row's task tag)
+  double th;
+  uint32_t rank, pad_;
+};
 struct Ctl {
   TreeDev tree[2];               // tree j (being updated) and tree j+1 (proposal prepared during pass j)
   ChainDesc cd;
   double PG[kNodes];
-  double thc[2][kNodes];         // leaf values of the tree whose statistics the pass computes
+  ThR thr[2][kNodes];            // leaf value and leaf rank of every slot of the tree whose statistics the pass computes
   double tho[kNodes], thn[kNodes];  // previous tree's values before / after its update (applied by the next pass)
   double S0[kNodes], S1[kNodes];
   double part[kMaxTasks][kLanes];   // per-lane sums (double bits, or int64 in FX64 mode)
   uint64_t cand[2];              // slots that can be leaves after the MH step
   double prr[2], lpr[2];           // proposal ratio and its log (taken while the data waves run)
   int cmode[2], cnx[2], cv[2], cc[2], nleaf[2], ngood[2], nnogs[2];
-  uint8_t rank_of[2][kNodes], slot_of[2][kNodes];
+  uint8_t slot_of[2][kNodes];
   uint32_t rnd[2 * kNodes + 4];  // generator outputs of the MH uniform and the leaf normals
   uint32_t rng_mark[4];          // generator state right after them (trace records)
   float z[kNodes];
@@ -413,13 +417,13 @@ __device__ void ctl_propose(Ctl& c, int b, Rng& g, int cap, int lane) {
   const bool nog = isused && !isleaf && BITM(leafm, t.l[s] & 63) && BITM(leafm, t.r[s] & 63);
   const uint64_t goodm = __ballot(good), nogm = __ballot(nog);
   const uint64_t lm = used & leafm;
-  c.thc[b][s] = (double)t.th[s];
+  c.thr[b][s].th = (double)t.th[s];
   if (BITM(lm, s)) {
     const int rk = __popcll(lm & ((1ull << s) - 1ull));
-    c.rank_of[b][s] = (uint8_t)rk;
+    c.thr[b][s].rank = (uint32_t)rk;
     c.slot_of[b][rk] = (uint8_t)s;
   } else {
-    c.rank_of[b][s] = 0;
+    c.thr[b][s].rank = 0;
   }
   const int ngood = __popcll(goodm), nnogs = __popcll(nogm);
   const bool single = used == 1ull;
@@ -694,8 +698,7 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
   const uint8_t* __restrict__ lb = lcur ? lcur : P.LEAF;
   const uint8_t* __restrict__ qa = qp ? qp : P.Q;
   const uint8_t* __restrict__ qb = qc ? qc : P.Q;
-  const double* __restrict__ thc = c.thc[cb];
-  const uint8_t* __restrict__ rank_of = c.rank_of[cb];
+  const ThR* __restrict__ thr = c.thr[cb];
 
   // phase-A coordinates: the thread's row quad (16 lane + 4 dw: the wave's four lanes in a row slot); phase-B
   // coordinates: local lane bl, task slot
@@ -771,8 +774,9 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
             }
             if (!rss) {
               const uint32_t lb_u = (T.lb[s] >> (8 * u)) & 255u;
-              thc_r[k] = thc[lb_u];
-              rank_r[k] = rank_of[lb_u];
+              const ThR e = thr[lb_u];  // one 16-byte gather
+              thc_r[k] = e.th;
+              rank_r[k] = e.rank;
             }
             if constexpr (AFT) af_r[k] = d4_at(T.af[s], u);
             else af_r[k] = P.AFl[a0 + (uint32_t)u];
@@ -1344,7 +1348,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         wave_sync();
         // leaf values: the normals of the used leaves (gathered during the pass)
         const int s = lane;
-        c.tho[s] = c.thc[cb][s];
+        c.tho[s] = c.thr[cb][s].th;
         if (s < cap && BITM(t.leaf, s) && BITM(t.used, s)) {
           const double b = c.S0[s] * inv_s2, M = c.S1[s] * inv_s2;
           t.th[s] = (float)(M / (a_prior + b) + (double)c.z[s] / __builtin_sqrt(a_prior + b));
@@ -1660,13 +1664,13 @@ __device__ void lp_choose(Ctl& c, Rng& g, int lane, int* mode_out, int* nx_out) 
   const bool nog = isused && !isleaf && BITM(leafm, t.l[s] & 63) && BITM(leafm, t.r[s] & 63);
   const uint64_t goodm = __ballot(good), nogm = __ballot(nog);
   const uint64_t lm = used & leafm;
-  c.thc[0][s] = (double)t.th[s];
+  c.thr[0][s].th = (double)t.th[s];
   if (BITM(lm, s)) {
     const int rk = __popcll(lm & ((1ull << s) - 1ull));
-    c.rank_of[0][s] = (uint8_t)rk;
+    c.thr[0][s].rank = (uint32_t)rk;
     c.slot_of[0][rk] = (uint8_t)s;
   } else {
-    c.rank_of[0][s] = 0;
+    c.thr[0][s].rank = 0;
   }
   const int ngood = __popcll(goodm), nnogs = __popcll(nogm);
   const double PBx = ngood == 0 ? 0 : (used == 1ull ? 1 : 0.5);
@@ -1744,7 +1748,7 @@ __global__ __launch_bounds__(kBlock) void k_lp_stage1(LpArgs A) {
     for (int i = tid; i < n; i += kBlock) {
       const int lf = lc[i];
       if (lf != l && lf != r) continue;
-      const double res = (double)P.Y[i] - (double)(AFX[i] - fit_fx(c.thc[0][lf])) * (1.0 / kFxFit);
+      const double res = (double)P.Y[i] - (double)(AFX[i] - fit_fx(c.thr[0][lf].th)) * (1.0 / kFxFit);
       const double w = (double)P.W[i];
       const unsigned long long wr = (unsigned long long)(long long)rint(w * res * kFxFeat);
       for (int v = 0; v < p; v++) atomicAdd(&H[v * kFeatBins + (P.Q[(size_t)v * P.ns + i] >> 4)], wr);
@@ -2061,7 +2065,7 @@ __global__ __launch_bounds__(kBlock) void k_lp_stage2(LpArgs A) {
       if (lane == s) zn = normal_f(x1, x2, tb);
     }
     const int s = lane;
-    c.tho[s] = c.thc[0][s];
+    c.tho[s] = c.thr[0][s].th;
     if (s < cap && BITM(t.leaf, s) && BITM(t.used, s)) {
       const double b = c.S0[s] * inv_s2, M = c.S1[s] * inv_s2;
       t.th[s] = (float)(M / (a_prior + b) + (double)zn / __builtin_sqrt(a_prior + b));

@@ -47,6 +47,23 @@ class Tune(C.Structure):
                 ("ms_per_sweep", C.c_double * TUNE_MAX), ("sweeps", C.c_int)]
 
 
+class PrepOpts(C.Structure):
+    _fields_ = [("ntree", C.c_int), ("numcut", C.c_int), ("rm_const", C.c_int), ("k", C.c_double),
+                ("power", C.c_double), ("base", C.c_double), ("sigdf", C.c_double), ("sigquant", C.c_double),
+                ("sigest", C.c_double), ("sigmaf", C.c_double), ("lambda_", C.c_double), ("fmean", C.c_double),
+                ("binary_offset", C.c_double)]
+
+
+class Inputs(C.Structure):
+    _fields_ = [("n", C.c_int), ("p", C.c_int), ("q", C.POINTER(C.c_uint8)), ("numcut", C.POINTER(C.c_int)),
+                ("rm_const", C.POINTER(C.c_int)), ("cutpoints", C.POINTER(C.c_double)),
+                ("cut_off", C.POINTER(C.c_int)), ("y", C.POINTER(C.c_float)), ("wt", C.POINTER(C.c_float)),
+                ("ybin", C.POINTER(C.c_int)), ("pgtab", C.c_float * 64), ("tau", C.c_float), ("lambda_", C.c_float),
+                ("nu", C.c_float), ("sigest", C.c_float), ("offset", C.c_float), ("tau_d", C.c_double),
+                ("lambda_d", C.c_double), ("nu_d", C.c_double), ("sigest_d", C.c_double), ("offset_d", C.c_double),
+                ("fmean", C.c_double), ("lm_rank", C.c_int), ("priv_", C.c_void_p)]
+
+
 class ChainOut(C.Structure):
     _fields_ = [("sigma", C.c_void_p), ("trmean", C.c_void_p), ("trees", C.c_void_p), ("stats", C.c_void_p)]
 
@@ -71,7 +88,8 @@ EXPORTS = ["bart_fit_cpu", "bart_wfit", "bart_set_rng", "fbh_last_status", "fbh_
            "fbh_debug_math", "fbh_debug_libm_host", "fbh_debug_copy", "fbh_batch_trace", "fbh_version",
            "fbh_mlp_create", "fbh_mlp_nparams", "fbh_mlp_params", "fbh_mlp_forward", "fbh_mlp_grad", "fbh_mlp_train",
            "fbh_mlp_timing", "fbh_mlp_destroy", "fbh_batch_learned_setup", "fbh_batch_run_learned",
-           "fbh_batch_learned_pairs", "fbh_batch_learned_train", "fbh_batch_learned_timing"]
+           "fbh_batch_learned_pairs", "fbh_batch_learned_train", "fbh_batch_learned_timing", "fbh_prep_defaults",
+           "fbh_wbart_inputs", "fbh_pbart_inputs", "fbh_inputs_free", "fbh_qnorm", "fbh_qchisq"]
 
 _lock = threading.Lock()
 _lib = None
@@ -148,6 +166,16 @@ def _declare(L):
     L.fbh_batch_learned_pairs.restype = C.c_long
     L.fbh_batch_learned_train.argtypes = [vp, i, i, C.c_float, C.c_uint, vp]
     L.fbh_batch_learned_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i)]
+    L.fbh_prep_defaults.argtypes = [i, C.POINTER(PrepOpts)]
+    L.fbh_prep_defaults.restype = None
+    L.fbh_wbart_inputs.argtypes = [vp, i, i, vp, vp, vp, vp, C.POINTER(PrepOpts), C.POINTER(C.POINTER(Inputs))]
+    L.fbh_pbart_inputs.argtypes = [vp, i, i, vp, vp, vp, C.POINTER(PrepOpts), C.POINTER(C.POINTER(Inputs))]
+    L.fbh_inputs_free.argtypes = [C.POINTER(Inputs)]
+    L.fbh_inputs_free.restype = None
+    L.fbh_qnorm.argtypes = [d]
+    L.fbh_qnorm.restype = d
+    L.fbh_qchisq.argtypes = [d, d]
+    L.fbh_qchisq.restype = d
     return L
 
 

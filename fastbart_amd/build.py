@@ -16,7 +16,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfastbart_hip.so")
-SOURCES = ["fbh_common.hip", "fbh_sampler.hip", "fbh_predict.hip", "fbh_mlp.hip"]
+SOURCES = ["fbh_common.hip", "fbh_sampler.hip", "fbh_predict.hip", "fbh_mlp.hip", "fbh_prep.hip"]
 ARCH = os.environ.get("FBH_ARCH", "gfx950")
 
 

@@ -168,6 +168,56 @@ void fbh_batch_destroy(fbh_batch* b);
 /* create + run all sweeps + fetch every chain + destroy */
 int fbh_fit_batch(const fbh_chain_in* chains, int nchains, const fbh_opts* opts, const fbh_chain_out* outs);
 
+/* ---------------------------------------------------------------- preprocessing of the R wrappers
+ * fwbart's R/fwbart.R:8-27 and fpbart's R/fpbart.R:7-21 in C++, so that callers without R can pass raw x, y, w:
+ * BART::bartModelMatrix's default model matrix (constant columns dropped; per column the midpoints of consecutive
+ * distinct values when it has at most numcut + 1 of them, else numcut equally spaced points strictly inside
+ * [min, max]; xinfo replaces them), q = findInterval(x, xi), fmean, sigest (residual standard error of lm(yc ~ X) when
+ * p < n, else sd(yc)), lambda (qchisq), tau, the prior table, the probit offset (qnorm(mean(y))). The results fill an
+ * fbh_chain_in (q, y, wt, numcut, ybin, tau, lambda, nu, sigest, offset) and fbh_opts.pgtab. Settings for which the
+ * R wrappers fall back to BART (n > 65536, more than 32 predictors after rm.const, ntree > 200, more than 255
+ * cutpoints in a column) return FBH_EINVAL. */
+typedef struct fbh_prep_opts {
+  int ntree, numcut;        /* R's defaults: 200 (fwbart) or 50 (fpbart), 100 */
+  int rm_const;             /* 1: drop constant columns */
+  double k, power, base;    /* 2, 2, 0.95 */
+  double sigdf, sigquant;   /* fwbart: 3, 0.90 */
+  double sigest, sigmaf, lambda, fmean;  /* fwbart: NaN = R's NA (derived as R/fwbart.R:21-28 does) */
+  double binary_offset;     /* fpbart: NaN = qnorm(mean(y)) */
+} fbh_prep_opts;
+
+typedef struct fbh_inputs {  /* owned by the library: fbh_inputs_free */
+  int n, p;                 /* rows, predictors kept */
+  uint8_t* q;               /* [p][n] cutpoint ranks */
+  int* numcut;              /* [p] */
+  int* rm_const;            /* [p] kept columns, 1-based (R's rm.const) */
+  double* cutpoints;        /* kept columns' cutpoints, concatenated; column v's at cut_off[v] .. cut_off[v+1]-1 */
+  int* cut_off;             /* [p+1] */
+  float* y;                 /* [n] (float)(y - fmean) for fwbart, zeros for fpbart */
+  float* wt;                /* [n] (float)(1 / w^2) for fwbart, ones for fpbart */
+  int* ybin;                /* [n] fpbart outcomes (null for fwbart) */
+  float pgtab[64];          /* (float)(base / (1 + d)^power) */
+  float tau, lambda, nu, sigest, offset;            /* as the sampler takes them */
+  double tau_d, lambda_d, nu_d, sigest_d, offset_d; /* the same before the float conversion */
+  double fmean;             /* fwbart's mu (0 for fpbart) */
+  int lm_rank;              /* rank of [1, X] in the lm fit (0 if sigest was not estimated by lm) */
+  void* priv_;
+} fbh_inputs;
+
+/* R's defaults for fwbart (probit = 0) or fpbart (probit = 1) */
+void fbh_prep_defaults(int probit, fbh_prep_opts* o);
+/* x: ncol host columns of n rows; w may be null (all 1); xinfo (may be null): cutpoints of every input column,
+ * concatenated, column v's at xinfo_off[v] .. xinfo_off[v+1]-1, each nondecreasing */
+int fbh_wbart_inputs(const double* const* x, int n, int ncol, const double* y, const double* w, const double* xinfo,
+                     const int* xinfo_off, const fbh_prep_opts* o, fbh_inputs** out);
+/* y01: 0/1 outcomes */
+int fbh_pbart_inputs(const double* const* x, int n, int ncol, const int* y01, const double* xinfo,
+                     const int* xinfo_off, const fbh_prep_opts* o, fbh_inputs** out);
+void fbh_inputs_free(fbh_inputs* in);
+/* the quantile functions the preprocessing uses: qnorm (AS 241) and qchisq (inverse regularised incomplete gamma) */
+double fbh_qnorm(double p);
+double fbh_qchisq(double p, double df);
+
 /* ---------------------------------------------------------------- output formats (src/glue.cpp:29-57) */
 /* treedraws text of ndpost x m packed trees: "ndpost m p" then per tree the node count and pre-order lines
  * "id v c theta" (%.10g). Returns the full length; writes at most cap bytes (NUL-terminated if it fits).

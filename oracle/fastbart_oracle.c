@@ -46,6 +46,45 @@ float orc_rnorm(orc_rng* g) {
   return sqrtf(-2.f * logf(u1)) * cosf(6.28318531f * u2);
 }
 
+/* ------------------------------------------------------------------ log of the libm comparisons
+ * For measuring how often a last-bit difference between two libm implementations could change a decision: every
+ * accept test of rtnorm's exponential branch (u > exp(arg), src/bart_kernel.cpp:289) and of the gamma draw
+ * (log u < A + d log v, :80) is seen; the first `sample` of them, and every one whose two sides are within 1e-13
+ * (relative) of each other, are recorded as 7 doubles: kind (0 rtnorm, 1 gamma), the libm argument (arg or v), the
+ * comparison's left side (u, or log u), A, d (gamma), the host libm's value (exp(arg) or log v), near (1 if the sides
+ * are that close). Not thread-safe; test infrastructure. */
+static double* g_cmp = NULL;
+static long g_cmp_cap = 0, g_cmp_n = 0, g_cmp_seen = 0, g_cmp_sample = 0;
+
+void orc_cmp_log(double* buf, long cap, long sample) {
+  g_cmp = buf;
+  g_cmp_cap = cap;
+  g_cmp_sample = sample;
+  g_cmp_n = 0;
+  g_cmp_seen = 0;
+}
+
+long orc_cmp_count(long* seen) {
+  if (seen) *seen = g_cmp_seen;
+  return g_cmp_n;
+}
+
+static void cmp_record(int kind, double arg, double lhs, double A, double d, double hostval, double rhs) {
+  if (!g_cmp) return;
+  g_cmp_seen++;
+  const int near = fabs(lhs - rhs) <= 1e-13 * (fabs(rhs) > 1.0 ? fabs(rhs) : 1.0);
+  if ((g_cmp_seen <= g_cmp_sample || near) && g_cmp_n < g_cmp_cap) {
+    double* r = g_cmp + 7 * g_cmp_n++;
+    r[0] = kind;
+    r[1] = arg;
+    r[2] = lhs;
+    r[3] = A;
+    r[4] = d;
+    r[5] = hostval;
+    r[6] = near;
+  }
+}
+
 /* Marsaglia-Tsang (2000) for shape a >= 1, at most 32 tries, src/bart_kernel.cpp:73-83 */
 double orc_rgamma(orc_rng* g, double a) {
   const double d = a - 1.0 / 3.0;
@@ -57,6 +96,7 @@ double orc_rgamma(orc_rng* g, double a) {
     v = v * v * v;
     const double u = (double)orc_unif(g);
     const double rhs = 0.5 * x * x + d - d * v + d * log(v);
+    if (g_cmp) cmp_record(1, v, log(u), 0.5 * x * x + d - d * v, d, log(v), rhs);
     if (log(u) < rhs) return d * v;
   }
   return d;
@@ -77,6 +117,10 @@ double orc_rtnorm(orc_rng* g, double mean, double tau) {
       const double e = -log((double)orc_unif(g));
       z = e / lam + tau;
       const double u = (double)orc_unif(g);
+      if (g_cmp) {
+        const double arg = -0.5 * (z - lam) * (z - lam);
+        cmp_record(0, arg, u, 0.0, 0.0, exp(arg), exp(arg));
+      }
       if (!(u > exp(-0.5 * (z - lam) * (z - lam)))) break;
     }
   }

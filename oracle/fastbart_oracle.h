@@ -42,6 +42,9 @@ float orc_unif(orc_rng* r);                        /* :59 */
 float orc_rnorm(orc_rng* r);                       /* :60 */
 double orc_rgamma(orc_rng* r, double a);           /* :73-83 */
 double orc_rtnorm(orc_rng* r, double mean, double tau); /* :282-292 */
+/* log of the libm comparisons of rtnorm and rgamma (see fastbart_oracle.c); buf holds cap records of 7 doubles */
+void orc_cmp_log(double* buf, long cap, long sample);
+long orc_cmp_count(long* seen);
 
 /* one record per tree update, filled when a trace buffer is given (debugging / kernel-level parity) */
 typedef struct orc_trace {

@@ -100,6 +100,10 @@ class Oracle:
         L.orc_rgamma.restype = C.c_double
         L.orc_rtnorm.argtypes = [_u32p, C.c_double, C.c_double]
         L.orc_rtnorm.restype = C.c_double
+        L.orc_cmp_log.argtypes = [C.c_void_p, C.c_long, C.c_long]
+        L.orc_cmp_log.restype = None
+        L.orc_cmp_count.argtypes = [C.POINTER(C.c_long)]
+        L.orc_cmp_count.restype = C.c_long
         L.orc_chain_new.argtypes = [_u8p, _f32p, _f32p, _i32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint, C.c_int, C.c_void_p,
                                     C.c_float, C.POINTER(OrcOpts)]
@@ -174,6 +178,19 @@ class Oracle:
         if tr is not None:
             out["trace"] = tr
         return out
+
+    def fit_logging_comparisons(self, prob, m, nskip, ndpost, seed, probit=0, cap=1_000_000, sample=200_000):
+        """fit() with the libm comparisons of rtnorm and rgamma logged (orc_cmp_log): (records [k][7], comparisons
+        seen, fit outputs). The first `sample` comparisons are all recorded; after them only the near ones."""
+        buf = np.zeros((cap, 7), np.float64)
+        self.L.orc_cmp_log(buf.ctypes.data, cap, sample)
+        try:
+            out = self.fit(prob, m, nskip, ndpost, seed, keep_trees=0, probit=probit)
+            seen = C.c_long()
+            k = self.L.orc_cmp_count(C.byref(seen))
+        finally:
+            self.L.orc_cmp_log(None, 0, 0)
+        return buf[:k].copy(), seen.value, out
 
     # -- chains advanced sweep by sweep (state snapshots for parity)
     def chain(self, prob, m, nskip, ndpost, seed, probit=0, stats_mode=0, node_cap=0, trace_cap=0):

@@ -63,3 +63,45 @@ def test_gpu_double_log_exp_agreement_with_host_libm():
     assert np.max(np.abs(gl - dl) / np.abs(gl + 1e-300)) < 4e-16
     assert np.max(np.abs(ge - de) / ge) < 4e-16
     assert mis_log < 0.02 and mis_exp < 0.2
+
+
+@pytest.mark.parametrize("case", ["fpbart_n24576", "batch512_chain0"])
+def test_gpu_libm_decision_risk_on_full_workload_fits(oracle, case):
+    """How often the device's double exp / log could change a decision of the reference's sampler, measured on every
+    acceptance test of a whole fit as the oracle makes it (1,100 sweeps): rtnorm's exponential branch (u > exp(arg),
+    src/bart_kernel.cpp:289) on the probit workload, the gamma draw (log u < A + d log v, :80) on a chain of the
+    512-chain workload. Each logged argument is evaluated with the device's exp / log; a decision flips when the
+    comparison's left side falls between the host and the device value. With the drawn uniforms no decision flips
+    (and the comparisons that come within 1e-13 of a tie are all logged and checked); over random uniforms the
+    expected number of flips per fit is the sum of the probabilities that a uniform lands between the two values,
+    estimated on the first 200,000 comparisons and scaled to all of them."""
+    from fastbart_amd import workloads as W
+    if case == "fpbart_n24576":
+        pb, m, probit = W.workload_fpbart_n24576(), 50, 1
+    else:
+        pb, m, probit = W.batch512_chain(0), 200, 0
+    sample = 200_000
+    recs, seen, _ = oracle.fit_logging_comparisons(pb, m, 1000, 100, 1, probit=probit, cap=2 * sample, sample=sample)
+    kind, arg, lhs, A, d, host, near = (recs[:, k] for k in range(7))
+    dev = np.empty(len(recs))
+    e0, e1 = kind == 0, kind == 1
+    if e0.any():
+        dev[e0] = sampler.debug_math(3, d=np.ascontiguousarray(arg[e0]))
+    if e1.any():
+        dev[e1] = sampler.debug_math(2, d=np.ascontiguousarray(arg[e1]))
+    # the decisions with the drawn uniforms: host value against device value
+    flips0 = np.count_nonzero(e0 & ((lhs > host) != (lhs > dev)))
+    rh, rd = A + d * host, A + d * dev
+    flips1 = np.count_nonzero(e1 & ((lhs < rh) != (lhs < rd)))
+    # expected flips for a uniform u on (0, 1): P(u between exp(arg)_host and exp(arg)_dev) = |difference|; for the
+    # gamma test P(log u between the two right sides) = |exp(rh) - exp(rd)|
+    ns = min(sample, len(recs))
+    p0 = np.where(e0[:ns], np.abs(host[:ns] - dev[:ns]), 0.0)
+    p1 = np.where(e1[:ns], np.abs(np.exp(rh[:ns]) - np.exp(rd[:ns])), 0.0)
+    differ = np.count_nonzero(host[:ns] != dev[:ns]) / ns
+    expected = (p0.sum() + p1.sum()) * seen / ns
+    print(f"{case}: {seen} comparisons per fit, {int(near.sum())} within 1e-13 of a tie, device value differs from "
+          f"the host's in {differ:.2%} of them; flips with the drawn uniforms {flips0 + flips1}; expected flips per "
+          f"fit over random uniforms {expected:.3e}")
+    assert flips0 + flips1 == 0
+    assert expected < 1e-6

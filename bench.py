@@ -232,14 +232,25 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
     M = 200
 
     def ess_of(b):
+        """(sum over chains of the min-ESS, posterior means, ESS distribution: per chain ESS(sigma) and the 5th / 50th
+        percentile and minimum of ESS(f(x_i)) over its rows)"""
         tot = 0.0
-        means = []
+        means, es, f5, f50, fmin = [], [], [], [], []
         for c in range(chains):
             out = b.fetch(c)
             yh = b.yhat(c)
-            tot += min_ess_chain(out["sigma"][nskip:], yh, args.ess_rows)
+            cols = yh if not args.ess_rows else yh[:, np.linspace(0, yh.shape[1] - 1, args.ess_rows).astype(int)]
+            ef = ess_geyer_cols(cols)
+            esig = ess_geyer(out["sigma"][nskip:])
+            tot += min(esig, float(ef.min()))
             means.append(out["trmean"].astype(np.float64))
-        return tot, means
+            es.append(esig)
+            f5.append(float(np.percentile(ef, 5)))
+            f50.append(float(np.median(ef)))
+            fmin.append(float(ef.min()))
+        dist = {"ess_sigma_median": float(np.median(es)), "ess_f_p5_median": float(np.median(f5)),
+                "ess_f_median_median": float(np.median(f50)), "ess_f_min_median": float(np.median(fmin))}
+        return tot, means, dist
 
     # plain (the launch configuration the autotuner picks for this batch: serpentine packing, the automatic build and
     # tile kind)
@@ -249,7 +260,7 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
     b.run(total)
     b.sync()
     t_plain = time.perf_counter() - t0
-    ess_plain, mean_plain = ess_of(b)
+    ess_plain, mean_plain, dist_plain = ess_of(b)
     b.close()
     log(f"plain: {total} sweeps {t_plain:.2f} s, sum min-ESS {ess_plain:.1f}")
     # learned schedule
@@ -273,7 +284,7 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
     t_b = time.perf_counter() - t2
     t_learned = t_a + t_b
     ms_sweep, nls = b.learned_timing()
-    ess_learned, mean_learned = ess_of(b)
+    ess_learned, mean_learned, dist_learned = ess_of(b)
     st = np.array([b.fetch(c)["stats"] for c in range(chains)])
     b.close()
     fwd_ms, _ = net.timing() if hasattr(net, "timing") else (0.0, 0)
@@ -293,9 +304,10 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
                                f"{args.train_steps} Adam steps, {total - n_plain0 - args.collect} sweeps at alpha "
                                f"{args.alpha}"},
         "min_ess_per_sec": ess_ps,
-        "plain": {"min_ess_per_sec": ess_pp, "seconds": t_plain, "sum_min_ess": ess_plain},
+        "ndpost": ndpost,
+        "plain": {"min_ess_per_sec": ess_pp, "seconds": t_plain, "sum_min_ess": ess_plain, **dist_plain},
         "learned": {"min_ess_per_sec": ess_ps, "seconds_sampling": t_learned, "seconds_training": t_train,
-                    "sum_min_ess": ess_learned, "pairs": npairs, "ms_per_learned_sweep": ms_sweep,
+                    "sum_min_ess": ess_learned, **dist_learned, "pairs": npairs, "ms_per_learned_sweep": ms_sweep,
                     "learned_sweeps": nls, "births": int(st[:, 0].sum()), "deaths": int(st[:, 1].sum())},
         "ess_per_sec_ratio_learned_over_plain": ess_ps / ess_pp if ess_pp else None,
         "posterior_mean_corr": corr,

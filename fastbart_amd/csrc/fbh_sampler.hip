@@ -315,7 +315,9 @@ struct Ctl {
   ThR thr[2][kNodes];            // leaf value and leaf rank of every slot of the tree whose statistics the pass computes
   double tho[kNodes], thn[kNodes];  // previous tree's values before / after its update (applied by the next pass)
   double S0[kNodes], S1[kNodes];
-  double part[kMaxTasks][kLanes];   // per-lane sums (double bits, or int64 in FX64 mode)
+  // per-lane sums (double bits, or int64 in FX64 mode); a pad column so that the combine's lanes (one task each,
+  // reading its 16 lanes in order) fall in different LDS banks
+  double part[kMaxTasks][kLanes + 1];
   uint64_t cand[2];              // slots that can be leaves after the MH step
   double prr[2], lpr[2];           // proposal ratio and its log (taken while the data waves run)
   int cmode[2], cnx[2], cv[2], cc[2], nleaf[2], ngood[2], nnogs[2];

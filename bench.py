@@ -483,6 +483,8 @@ def main() -> int:
     ap.add_argument("--packing", default="auto",
                     help="workgroup -> chain packing of the sweep kernel: 0 batch order, 1 serpentine, 2 largest "
                          "first, auto = the autotuner picks packing, tile rows and build during the warmup")
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuner cache file: a batch shape tuned before takes the stored choice without trials")
     ap.add_argument("--tile-rows", type=int, default=0, help="rows per LDS tile (0 automatic, 1024, 1536)")
     ap.add_argument("--resident", type=int, default=0,
                     help="running fit during a launch: 1 streamed, 2 in LDS, 0 automatic")
@@ -553,11 +555,12 @@ def main() -> int:
         # the autotuner's trials are real warmup sweeps (one sweep per trial launch); what is left of the warmup runs
         # in the chosen configuration
         try:
-            tuned = b.autotune(1)
+            tuned = b.autotune(1, cache=args.tune_cache)
         except FbhError as e:
             log("autotune skipped:", e)
         if tuned:
-            log("autotune:", {k: tuned[k] for k in ("packing", "tile_rows", "resident", "chains_per_cu", "sweeps")},
+            log("autotune:", {k: tuned[k] for k in ("packing", "tile_rows", "resident", "chains_per_cu", "sweeps",
+                                                    "from_cache")},
                 [(c["packing"], c["tile_rows"], c["resident"], c["chains_per_cu"], round(c["ms_per_sweep"], 3))
                  for c in tuned["candidates"]])
     if W_ > b.it:
@@ -618,7 +621,8 @@ def main() -> int:
                    "packing": launch_cfg["packing"], "tile_rows": launch_cfg["tile_rows"],
                    "fit_in_lds": launch_cfg["resident"] == 2,
                    "chains_per_cu": launch_cfg["chains_per_cu"], "graph": bool(launch_cfg["graph"]),
-                   "autotuned": tuned is not None},
+                   "autotuned": tuned is not None,
+                   "tune_from_cache": bool(tuned and tuned["from_cache"])},
         "warmup_requested": args.warmup,
         "warmup_note": (f"{W_} warmup sweeps run: the autotuner's trials take up to {TUNE_SWEEPS_MAX} of them"
                         if W_ != (args.warmup if args.warmup is not None else w_def) else None),

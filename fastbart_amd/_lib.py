@@ -44,7 +44,7 @@ class Tune(C.Structure):
     _fields_ = [("packing", C.c_int), ("tile_rows", C.c_int), ("resident", C.c_int), ("chains_per_cu", C.c_int),
                 ("ncand", C.c_int), ("cand_packing", C.c_int * TUNE_MAX), ("cand_tile_rows", C.c_int * TUNE_MAX),
                 ("cand_resident", C.c_int * TUNE_MAX), ("cand_chains_per_cu", C.c_int * TUNE_MAX),
-                ("ms_per_sweep", C.c_double * TUNE_MAX), ("sweeps", C.c_int)]
+                ("ms_per_sweep", C.c_double * TUNE_MAX), ("sweeps", C.c_int), ("from_cache", C.c_int)]
 
 
 class PrepOpts(C.Structure):
@@ -84,7 +84,7 @@ EXPORTS = ["bart_fit_cpu", "bart_wfit", "bart_set_rng", "fbh_last_status", "fbh_
            "fbh_predict_rows_device", "fbh_predict_rows_device_noalloc", "fbh_predict_last_launch", "fbh_trees_rank_form", "fbh_device_alloc", "fbh_device_free",
            "fbh_device_copy", "fbh_device_sync", "fbh_batch_row_split", "fbh_batch_configure", "fbh_batch_config",
            "fbh_batch_set_order",
-           "fbh_batch_autotune",
+           "fbh_batch_autotune", "fbh_batch_autotune_cached",
            "fbh_debug_math", "fbh_debug_libm_host", "fbh_debug_copy", "fbh_batch_trace", "fbh_version",
            "fbh_mlp_create", "fbh_mlp_nparams", "fbh_mlp_params", "fbh_mlp_forward", "fbh_mlp_grad", "fbh_mlp_train",
            "fbh_mlp_timing", "fbh_mlp_destroy", "fbh_batch_learned_setup", "fbh_batch_run_learned",
@@ -147,6 +147,7 @@ def _declare(L):
     L.fbh_batch_set_order.argtypes = [vp, vp]
     L.fbh_batch_config.argtypes = [vp] + [C.POINTER(i)] * 5
     L.fbh_batch_autotune.argtypes = [vp, i, C.POINTER(Tune)]
+    L.fbh_batch_autotune_cached.argtypes = [vp, i, C.c_char_p, C.POINTER(Tune)]
     L.fbh_debug_math.argtypes = [i, vp, vp, C.c_long, vp, vp, i]
     L.fbh_debug_libm_host.argtypes = [i, C.c_long, C.c_long, vp]
     L.fbh_debug_copy.argtypes = [C.c_long, i, C.POINTER(d)]

@@ -2914,6 +2914,136 @@ extern "C" int fbh_batch_autotune(fbh_batch* b, int trial_sweeps, fbh_tune* out)
   return FBH_OK;
 }
 
+// ------------------------------------------------------------------ autotuner cache
+// A text file of one line per tuned batch shape, "<key>\t<packing> <tile_rows> <resident> <chains_per_cu> <ms>",
+// where ms is the chosen configuration's time per sweep. The key names what the time of a sweep depends on: the
+// device (architecture string and CU count), the library build, statistics mode, probit, row split, chain count, trees,
+// node cap, the rows per chain (minimum, quartiles and maximum, rounded up to 256-row steps) and the predictors per
+// chain (range). The last line for a key wins, so a re-tune appended later replaces an earlier entry.
+static std::string tune_key(const fbh_batch* b) {
+  std::string arch = "unknown";
+  hipDeviceProp_t pr;
+  if (hipGetDeviceProperties(&pr, b->device) == hipSuccess) arch = pr.gcnArchName;
+  else (void)hipGetLastError();
+  std::vector<int> ns(b->CL), ps(b->CL);
+  for (int c = 0; c < b->CL; c++) {
+    const ChainDesc& d = b->hdesc[(size_t)c * b->W];
+    ns[c] = (d.n_total + 255) / 256 * 256;
+    ps[c] = d.p;
+  }
+  std::sort(ns.begin(), ns.end());
+  std::sort(ps.begin(), ps.end());
+  auto q = [&](int k) { return ns[(size_t)k * (ns.size() - 1) / 4]; };
+  std::string build = __DATE__ "_" __TIME__;
+  for (char& ch : build)
+    if (ch == ' ') ch = '_';
+  char buf[512];
+  std::snprintf(buf, sizeof buf,
+                "fbh1 arch=%s cu=%d build=%s stats=%s probit=%d split=%d chains=%d ntree=%d cap=%d rows=%d,%d,%d,%d,%d "
+                "p=%d-%d",
+                arch.c_str(), b->ncu, build.c_str(), b->stats_mode == FBH_STATS_FX64 ? "fx64" : "ref16", b->probit,
+                b->W, b->CL, b->m, b->node_cap, q(0), q(1), q(2), q(3), q(4), ps.front(), ps.back());
+  std::string key(buf);
+  for (char& ch : key)
+    if (ch == '\t' || ch == '\n' || ch == '\r') ch = '_';
+  return key;
+}
+
+struct TuneEntry {
+  int packing, tile_rows, resident, chains_per_cu;
+  double ms;
+};
+
+// the last entry for key in the cache file; false if the file or the key is absent (malformed lines are skipped)
+static bool tune_lookup(const char* path, const std::string& key, TuneEntry* e) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) return false;
+  bool found = false;
+  char* line = nullptr;
+  size_t cap = 0;
+  ssize_t len;
+  while ((len = getline(&line, &cap, f)) >= 0) {
+    const char* tab = std::strchr(line, '\t');
+    if (!tab || (size_t)(tab - line) != key.size() || std::memcmp(line, key.data(), key.size()) != 0) continue;
+    TuneEntry t{};
+    if (std::sscanf(tab + 1, "%d %d %d %d %lf", &t.packing, &t.tile_rows, &t.resident, &t.chains_per_cu, &t.ms) ==
+        5) {
+      *e = t;
+      found = true;
+    }
+  }
+  std::free(line);
+  std::fclose(f);
+  return found;
+}
+
+extern "C" int fbh_batch_autotune_cached(fbh_batch* b, int trial_sweeps, const char* cache_path, fbh_tune* out) {
+  if (!b || trial_sweeps < 1) {
+    set_error(FBH_EINVAL, "fbh_batch_autotune_cached: bad argument");
+    return FBH_EINVAL;
+  }
+  if (!cache_path || !*cache_path) return fbh_batch_autotune(b, trial_sweeps, out);
+  if (b->net) {
+    set_error(FBH_EINVAL, "fbh_batch_autotune_cached: not for a batch set up for the learned proposal");
+    return FBH_EINVAL;
+  }
+  const std::string key = tune_key(b);
+  TuneEntry e{};
+  if (tune_lookup(cache_path, key, &e)) {
+    // an entry this batch cannot run (edited by hand, or from a build with other limits) is tuned afresh
+    const bool dense = e.chains_per_cu == 2;
+    const int kind = (e.chains_per_cu == 1 || e.chains_per_cu == 2) && (e.resident == 1 || e.resident == 2) &&
+                             (e.tile_rows == 1024 || e.tile_rows == 1536)
+                         ? pick_kind(b, e.tile_rows, e.resident, dense)
+                         : -1;
+    if (kind >= 0 && e.packing >= 0 && e.packing <= 2 && (b->W <= 1 || b->fits[dense][kind])) {
+      const int rc = configure(b, e.packing, kind, dense);
+      if (rc) return rc;
+      if (out) {
+        std::memset(out, 0, sizeof *out);
+        out->packing = e.packing;
+        out->tile_rows = 256 * kind_rows(kind);
+        out->resident = kind_resident(kind) ? 2 : 1;
+        out->chains_per_cu = dense ? 2 : 1;
+        out->from_cache = 1;
+      }
+      return FBH_OK;
+    }
+  }
+  // the file must be writable before any sweep is spent on tuning
+  FILE* f = std::fopen(cache_path, "a");
+  if (!f) {
+    set_error(FBH_EINVAL, std::string("fbh_batch_autotune_cached: cannot open the cache file ") + cache_path);
+    return FBH_EINVAL;
+  }
+  fbh_tune t{};
+  int rc = fbh_batch_autotune(b, trial_sweeps, &t);
+  if (rc) {
+    std::fclose(f);
+    return rc;
+  }
+  // the chosen configuration's time: its candidate in the packing stage (the last candidates)
+  double ms = 0;
+  for (int i = t.ncand - 1; i >= 0; i--)
+    if (t.cand_packing[i] == t.packing && t.cand_tile_rows[i] == t.tile_rows && t.cand_resident[i] == t.resident &&
+        t.cand_chains_per_cu[i] == t.chains_per_cu) {
+      ms = t.ms_per_sweep[i];
+      break;
+    }
+  char vals[128];
+  std::snprintf(vals, sizeof vals, "\t%d %d %d %d %.6f\n", t.packing, t.tile_rows, t.resident, t.chains_per_cu, ms);
+  const std::string line = key + vals;
+  // one write of the whole line: appends from other processes do not interleave within it
+  const bool ok = std::fwrite(line.data(), 1, line.size(), f) == line.size();
+  if (std::fclose(f) != 0 || !ok) {
+    set_error(FBH_EINVAL, std::string("fbh_batch_autotune_cached: writing the cache file ") + cache_path +
+                              " failed (the tuned configuration is in effect)");
+    return FBH_EINVAL;
+  }
+  if (out) *out = t;
+  return FBH_OK;
+}
+
 
 // a row split whose exchange timed out (members not resident together) has invalid results: report it
 static int check_split(fbh_batch* b) {

@@ -6,6 +6,7 @@ The names and argument meanings follow the reference's C-ABI (``bart_fit_cpu``, 
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -84,15 +85,20 @@ class BatchSampler:
         check(lib().fbh_batch_config(self.h, *[C.byref(x) for x in v]), "fbh_batch_config")
         return dict(zip(("packing", "tile_rows", "resident", "chains_per_cu", "graph"), (x.value for x in v)))
 
-    def autotune(self, trial_sweeps=1):
+    def autotune(self, trial_sweeps=1, cache=None):
         """Time every (packing, tile rows) candidate on the next sweeps (real sweeps of every chain) and keep the
-        fastest. Returns the choice and each candidate's ms per sweep."""
+        fastest. Returns the choice and each candidate's ms per sweep. With cache (a file path) a batch of a shape
+        tuned before takes the stored choice without running a sweep, and a new shape's choice is appended."""
         t = Tune()
-        check(lib().fbh_batch_autotune(self.h, int(trial_sweeps), C.byref(t)), "fbh_batch_autotune")
+        if cache is None:
+            check(lib().fbh_batch_autotune(self.h, int(trial_sweeps), C.byref(t)), "fbh_batch_autotune")
+        else:
+            check(lib().fbh_batch_autotune_cached(self.h, int(trial_sweeps), os.fsencode(cache), C.byref(t)),
+                  "fbh_batch_autotune_cached")
         cands = [dict(packing=t.cand_packing[k], tile_rows=t.cand_tile_rows[k], resident=t.cand_resident[k],
                       chains_per_cu=t.cand_chains_per_cu[k], ms_per_sweep=t.ms_per_sweep[k]) for k in range(t.ncand)]
         return dict(packing=t.packing, tile_rows=t.tile_rows, resident=t.resident, chains_per_cu=t.chains_per_cu,
-                    sweeps=t.sweeps, candidates=cands)
+                    sweeps=t.sweeps, candidates=cands, from_cache=bool(t.from_cache))
 
     def run(self, nsweeps):
         check(lib().fbh_batch_run(self.h, int(nsweeps)), "fbh_batch_run")

@@ -135,6 +135,7 @@ typedef struct fbh_tune {
       cand_chains_per_cu[FBH_TUNE_MAX];
   double ms_per_sweep[FBH_TUNE_MAX];
   int sweeps;                                       /* sweeps the tuning ran: real sweeps, results unchanged */
+  int from_cache;                                   /* 1: taken from the autotuner cache, nothing timed */
 } fbh_tune;
 /* Switch packing (0, 1, 2 as fbh_opts.packing), tile rows and resident (as in fbh_opts; 0 = the automatic choice)
  * and the build (chains_per_cu 1 or 2, 0 = keep) between runs; waits for the batch's queued work. Results do not
@@ -151,6 +152,15 @@ int fbh_batch_config(const fbh_batch* b, int* packing, int* tile_rows, int* resi
  * and count toward nskip + ndpost (FBH_EINVAL if too few are left); no candidate changes any result. out may be
  * null. */
 int fbh_batch_autotune(fbh_batch* b, int trial_sweeps, fbh_tune* out);
+/* fbh_batch_autotune with a cache file (a tile schedule per batch shape): if cache_path holds an entry for this
+ * batch's shape -- device architecture and CU count, library build, statistics mode, probit, row split, chain count,
+ * trees, node cap, rows per chain (minimum, quartiles, maximum in 256-row steps), predictors per chain (range) -- its
+ * configuration is applied at once (no sweep run, out->from_cache = 1, out->ncand = 0); otherwise the batch is tuned
+ * as by fbh_batch_autotune and one line is appended to the file (created if absent). Lines are
+ * "<key>\t<packing> <tile_rows> <resident> <chains_per_cu> <ms per sweep>"; the last line for a key wins; an entry
+ * this batch cannot run is tuned afresh. A null or empty cache_path is fbh_batch_autotune. FBH_EINVAL if the file
+ * cannot be opened for appending (before any sweep) or written (the tuned configuration is then in effect). */
+int fbh_batch_autotune_cached(fbh_batch* b, int trial_sweeps, const char* cache_path, fbh_tune* out);
 /* Copy chain c's outputs to host buffers (synchronises). */
 int fbh_batch_fetch(fbh_batch* b, int chain, const fbh_chain_out* out);
 /* Copy chain c's sampler state (parity checks, checkpoints): rng[4], sigma, af[n], latent[n], leaf[m][n],

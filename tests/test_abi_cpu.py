@@ -201,6 +201,7 @@ def test_launch_configuration_entry_points_reject_a_null_batch():
     assert L.fbh_batch_configure(None, 0, 0, 0, 0) == einval
     assert L.fbh_batch_config(None, *[C.pointer(C.c_int()) for _ in range(5)]) == einval
     assert L.fbh_batch_autotune(None, 1, None) == einval
+    assert L.fbh_batch_autotune_cached(None, 1, b"tune.txt", None) == einval
     order = (C.c_int * 1)(0)
     assert L.fbh_batch_set_order(None, C.cast(order, C.c_void_p)) == einval
     assert b"null" in L.fbh_last_error()

@@ -134,6 +134,69 @@ def test_gpu_autotune_runs_real_sweeps_and_keeps_results(oracle):
         _check_oracle(oracle, b, probs, m, nskip, nd, seeds)
 
 
+def test_gpu_autotune_cache(oracle, tmp_path):
+    """The autotuner cache: a new batch shape is tuned and its choice appended; a batch of the same shape (other
+    seeds) takes the stored choice without a sweep; the last entry for a shape wins; an entry the batch cannot run is
+    tuned afresh; another shape is a miss. Every run still matches the oracle."""
+    probs = _probs()
+    m, nskip, nd = 12, 16, 6
+    cache = tmp_path / "tune.txt"
+    _say("autotune cache: miss")
+    seeds = [71 + c for c in range(len(probs))]
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds, sweeps_per_launch=5) as b:
+        t = b.autotune(1, cache=str(cache))
+        assert not t["from_cache"] and t["sweeps"] == 13 and b.it == 13
+        lines = cache.read_text().splitlines()
+        assert len(lines) == 1
+        key, vals = lines[0].split("\t")
+        f = vals.split()
+        assert [int(x) for x in f[:4]] == [t["packing"], t["tile_rows"], t["resident"], t["chains_per_cu"]]
+        assert float(f[4]) == pytest.approx(min(c["ms_per_sweep"] for c in t["candidates"][3:]), rel=1e-5)
+        assert "chains=6" in key and "ntree=12" in key and "stats=ref16" in key
+        b.run(nskip + nd - 13)
+        b.sync()
+        _check_oracle(oracle, b, probs, m, nskip, nd, seeds)
+    _say("autotune cache: hit")
+    seeds = [81 + c for c in range(len(probs))]
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds, sweeps_per_launch=5) as b:
+        t2 = b.autotune(1, cache=str(cache))
+        assert t2["from_cache"] and t2["sweeps"] == 0 and b.it == 0 and t2["candidates"] == []
+        assert all(t2[k] == t[k] for k in ("packing", "tile_rows", "resident", "chains_per_cu"))
+        cfg = b.config
+        assert all(cfg[k] == t[k] for k in ("packing", "tile_rows", "resident", "chains_per_cu"))
+        b.run(nskip + nd)
+        b.sync()
+        _check_oracle(oracle, b, probs, m, nskip, nd, seeds)
+    _say("autotune cache: last entry wins, malformed lines skipped")
+    other = (t["packing"] + 1) % 3
+    with open(cache, "a") as fh:
+        fh.write("no tab on this line\n")
+        fh.write(f"{key}\tnot numbers\n")
+        fh.write(f"{key}\t{other} 1024 1 1 9.5\n")
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds, sweeps_per_launch=5) as b:
+        t3 = b.autotune(1, cache=str(cache))
+        assert t3["from_cache"] and b.it == 0
+        assert (t3["packing"], t3["tile_rows"], t3["resident"], t3["chains_per_cu"]) == (other, 1024, 1, 1)
+    _say("autotune cache: an entry the batch cannot run")
+    with open(cache, "a") as fh:
+        fh.write(f"{key}\t0 1536 1 1 1.0\n")  # 1536-row tiles need the running fit in LDS
+    n_before = len(cache.read_text().splitlines())
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds, sweeps_per_launch=5) as b:
+        t4 = b.autotune(1, cache=str(cache))
+        assert not t4["from_cache"] and t4["sweeps"] == 13
+    assert len(cache.read_text().splitlines()) == n_before + 1
+    _say("autotune cache: another shape")
+    with sampler.BatchSampler(_probs(ntree=8), 8, nskip, nd, seeds, sweeps_per_launch=5) as b:
+        t5 = b.autotune(1, cache=str(cache))
+        assert not t5["from_cache"] and t5["sweeps"] == 13
+    last = cache.read_text().splitlines()[-1]
+    assert "ntree=8" in last and last.split("\t")[0] != key
+    with sampler.BatchSampler(probs, m, nskip, nd, seeds) as b:
+        with pytest.raises(FbhError):
+            b.autotune(1, cache=str(tmp_path / "no_such_dir" / "tune.txt"))
+        assert b.it == 0
+
+
 def test_gpu_launch_config_rejections():
     probs = _probs()[:2]
     with sampler.BatchSampler(probs, 8, 4, 2, [1, 2]) as b:

@@ -1,6 +1,8 @@
 /* fastbart_oracle.c -- TEST INFRASTRUCTURE ONLY: CPU restatement of the reference sampler, glue and predictor.
  * Only tests/, the smoke test and the benchmark's CPU leg load this file's library; the product never does.
- * Each function names the reference lines it restates (paths relative to the reference root).
+ * Each function names the reference lines it restates (paths relative to the reference root). One part has no
+ * reference counterpart: the informed birth rule of the proposal study (off unless orc_chain_set_informed is called),
+ * pinned by the enumerated single-tree posterior (tests/test_tree_posterior.py).
  * Build: oracle/Makefile (-O2 -ffp-contract=off, no -march: no fused multiply-add anywhere).
  */
 #include "fastbart_oracle.h"
@@ -243,6 +245,8 @@ struct orc_chain {
   int n_birth, n_death, n_cap;
   orc_trace* trace;
   long trace_cap, trace_len;
+  double informed_alpha; /* proposal study (no reference counterpart): 0 = the reference's proposal */
+  long informed_births;  /* births proposed from the informed rule */
 };
 
 static inline int64_t fx_round(double x, int shift) { return (int64_t)llrint(ldexp(x, shift)); }
@@ -442,6 +446,111 @@ void orc_chain_free(orc_chain* ch) {
   free(ch);
 }
 
+/* ------------------------------------------------------------------ informed birth rule (no reference counterpart)
+ * A proposal study, off unless orc_chain_set_informed sets alpha > 0. A birth at node nx draws its rule (v, c) with
+ * probability alpha from the rule's conditional given the tree, sigma and the partial residuals of the tree's rows,
+ * and otherwise as the reference does; the mixture's probability of the rule replaces the uniform one in the
+ * Metropolis-Hastings ratio of births and of deaths (the reverse of a death is a birth at the merged node with its old
+ * rule), so the chain's target is the reference's. The conditional is
+ *   q(v, c) proportional to (1 - PGl)(1 - PGr) PD / (U_v - L_v + 1) * exp(lh(left) + lh(right))
+ * over the rules with a non-empty range whose children hold at least 5 rows each (the ratio refuses smaller ones);
+ * the other factors of the birth ratio do not depend on the rule. It is the best a birth proposal learned from node
+ * features can approach. Sums are plain sequential doubles; the forward and the reverse evaluation see the same rows
+ * and residuals, equal up to the rounding of the running fit. */
+typedef struct orule {
+  int v, c;
+  double lw;
+} orule;
+
+/* the conditional at node nx over the rows whose leaf in tree j is nx (or l or r: the merged node of a death);
+ * draw = 1: draw a rule into *v, *c and return its probability (0 if no rule qualifies); draw = 0: return the
+ * probability of rule (*v, *c) */
+static double informed_rule(orc_chain* ch, const otree* t, int j, int nx, int l, int r, const double* th, double inv_s2,
+                            int ngood_pre, int draw, int* v, int* c) {
+  static __thread double H0[ORC_PMAX][256], H1[ORC_PMAX][256];
+  static __thread int Hn[ORC_PMAX][256];
+  static __thread orule rules[ORC_PMAX * 256];
+  const int n = ch->n, p = ch->p;
+  const double pb = 0.5, t2 = ch->t2;
+  if ((int)t->d[nx] + 1 >= ORC_MAXDEPTH) return 0;
+  for (int u = 0; u < p; u++) {
+    memset(H0[u], 0, sizeof(double) * (ch->nc[u] + 1));
+    memset(H1[u], 0, sizeof(double) * (ch->nc[u] + 1));
+    memset(Hn[u], 0, sizeof(int) * (ch->nc[u] + 1));
+  }
+  const uint8_t* lf = ch->leaf + (size_t)j * n;
+  double B0 = 0, B1 = 0;
+  int N = 0;
+  for (int i = 0; i < n; i++) {
+    const int s = lf[i];
+    if (l < 0 ? s != nx : (s != l && s != r)) continue;
+    const double w = ch->W[i], wr = w * (ch->Y[i] - ch->AF[i] + th[s]);
+    B0 += w;
+    B1 += wr;
+    N++;
+    for (int u = 0; u < p; u++) {
+      const int k = ch->q[(size_t)u * n + i];
+      H0[u][k] += w;
+      H1[u][k] += wr;
+      Hn[u][k]++;
+    }
+  }
+  const int ngv = count_goodvars(t, nx, ch->nc, p), d = t->d[nx];
+  int nr = 0;
+  double lmax = -INFINITY;
+  for (int u = 0; u < p; u++) {
+    int L, U;
+    node_range(t, nx, u, ch->nc[u], &L, &U);
+    if (U < L) continue;
+    double c0 = 0, c1 = 0;
+    int cn = 0;
+    for (int k = 0; k <= U; k++) {
+      c0 += H0[u][k];
+      c1 += H1[u][k];
+      cn += Hn[u][k];
+      if (k < L || cn < 5 || N - cn < 5) continue;
+      double PGl, PGr;
+      if (ngv > 1) {
+        PGl = PGr = ch->PG[d + 1];
+      } else {
+        PGl = (k - 1 < L) ? 0 : ch->PG[d + 1];
+        PGr = (U < k + 1) ? 0 : ch->PG[d + 1];
+      }
+      const double PD = (ngood_pre > 1) ? 1 - pb : ((PGr == 0 && PGl == 0) ? 1 : 1 - pb);
+      const double lw = log((1 - PGl) * (1 - PGr) * PD / (U - L + 1)) + lh(c0 * inv_s2, c1 * inv_s2, t2) +
+                        lh((B0 - c0) * inv_s2, (B1 - c1) * inv_s2, t2);
+      rules[nr].v = u;
+      rules[nr].c = k;
+      rules[nr].lw = lw;
+      nr++;
+      if (lw > lmax) lmax = lw;
+    }
+  }
+  if (nr == 0) return 0;
+  double Z = 0;
+  for (int k = 0; k < nr; k++) Z += exp(rules[k].lw - lmax);
+  if (draw) {
+    const double u = (double)orc_unif(&ch->rng) * Z;
+    double acc = 0;
+    int k = 0;
+    for (; k < nr - 1; k++) {
+      acc += exp(rules[k].lw - lmax);
+      if (u < acc) break;
+    }
+    *v = rules[k].v;
+    *c = rules[k].c;
+    return exp(rules[k].lw - lmax) / Z;
+  }
+  for (int k = 0; k < nr; k++)
+    if (rules[k].v == *v && rules[k].c == *c) return exp(rules[k].lw - lmax) / Z;
+  return 0;
+}
+
+void orc_chain_set_informed(orc_chain* ch, double alpha) {
+  ch->informed_alpha = alpha > 0 ? (alpha < 1 ? alpha : 1) : 0;
+}
+long orc_chain_informed_births(const orc_chain* ch) { return ch->informed_births; }
+
 /* One sweep: m tree updates then the closing pass and the sigma / latent draw. src/bart_kernel.cpp:396-572 */
 static void one_sweep(orc_chain* ch, float* sigma_out, uint64_t* trees_out, int keep_trees) {
   const int p = ch->p, m = ch->m, it = ch->it;
@@ -450,8 +559,16 @@ static void one_sweep(orc_chain* ch, float* sigma_out, uint64_t* trees_out, int 
   oprev pu;
   memset(&pu, 0, sizeof pu);
   ostats st;
+  const double ia = ch->informed_alpha;
   for (int j = 0; j < m; j++) {
     otree* t = &ch->T[j];
+    /* the informed rule reads tree j's partial residuals before the pass: apply tree j-1's change first */
+    const int pre = ia > 0 && j > 0;
+    if (pre) {
+      ocur none;
+      memset(&none, 0, sizeof none);
+      data_pass(ch, j - 1, j, &pu, 1, &none, 0, 0, 0, &st);
+    }
     /* ---- proposal, :413-464 */
     uint64_t goodm = 0, nogm = 0;
     for (int s = 0; s < ORC_NODES; s++) {
@@ -469,13 +586,22 @@ static void one_sweep(orc_chain* ch, float* sigma_out, uint64_t* trees_out, int 
     double pr = 0;
     if ((double)orc_unif(&ch->rng) < PBx) {
       const int nx = select_kth(goodm, (int)((double)orc_unif(&ch->rng) * ngood));
-      int v = (int)((double)orc_unif(&ch->rng) * p);
-      if (v >= p) v = p - 1;
-      int bL, bU;
-      node_range(t, nx, v, ch->nc[v], &bL, &bU);
+      int v = 0, c = 0, bL = 0, bU = -1, inf = 0;
+      double qx = 0;
+      if (ia > 0 && (double)orc_unif(&ch->rng) < ia) {
+        inf = 1;
+        qx = informed_rule(ch, t, j, nx, -1, -1, cp.th, inv_s2, ngood, 1, &v, &c);
+        if (qx > 0) node_range(t, nx, v, ch->nc[v], &bL, &bU);
+      } else {
+        v = (int)((double)orc_unif(&ch->rng) * p);
+        if (v >= p) v = p - 1;
+        node_range(t, nx, v, ch->nc[v], &bL, &bU);
+      }
       if (bU >= bL && (int)t->d[nx] + 1 < ORC_MAXDEPTH) {
-        int c = bL + (int)((double)orc_unif(&ch->rng) * (bU - bL + 1));
-        if (c > bU) c = bU;
+        if (!inf) {
+          c = bL + (int)((double)orc_unif(&ch->rng) * (bU - bL + 1));
+          if (c > bU) c = bU;
+        }
         const int dnx = t->d[nx];
         const double PGnx = ch->PG[dnx];
         double PGly, PGry;
@@ -496,6 +622,12 @@ static void one_sweep(orc_chain* ch, float* sigma_out, uint64_t* trees_out, int 
         }
         const double Pbotx = 1.0 / ngood;
         pr = (PGnx * (1 - PGly) * (1 - PGry) * PDy * Pnogy) / ((1 - PGnx) * Pbotx * PBx);
+        if (ia > 0) {
+          const double qu = (1.0 / p) / (bU - bL + 1);
+          if (!inf) qx = informed_rule(ch, t, j, nx, -1, -1, cp.th, inv_s2, ngood, 0, &v, &c);
+          pr *= qu / ((1 - ia) * qu + ia * qx);
+          ch->informed_births += inf;
+        }
         cp.mode = 1;
         cp.nx = nx;
         cp.v = v;
@@ -510,12 +642,19 @@ static void one_sweep(orc_chain* ch, float* sigma_out, uint64_t* trees_out, int 
       const int ng2 = ngood - (t->ngv[l] > 0) - (t->ngv[r] > 0) + 1;
       const double Pboty = 1.0 / ng2, PDx = 1 - PBx, Pnogx = 1.0 / nnogs;
       pr = ((1 - PGny) * PBy * Pboty) / (PGny * (1 - PGlx) * (1 - PGrx) * PDx * Pnogx);
+      if (ia > 0) {
+        int v = t->v[nx], c = t->c[nx], bL, bU;
+        node_range(t, nx, v, ch->nc[v], &bL, &bU);
+        const double qu = (1.0 / p) / (bU - bL + 1);
+        const double qx = informed_rule(ch, t, j, nx, l, r, cp.th, inv_s2, ng2, 0, &v, &c);
+        pr *= ((1 - ia) * qu + ia * qx) / qu;
+      }
       cp.mode = 2;
       cp.nx = nx;
     }
 
     /* ---- data pass, :467 */
-    data_pass(ch, j - 1, j, &pu, j > 0, &cp, 1, 0, 0, &st);
+    data_pass(ch, j - 1, j, &pu, j > 0 && !pre, &cp, 1, 0, 0, &st);
 
     /* ---- Metropolis-Hastings, :469-518 */
     oprev nu_;

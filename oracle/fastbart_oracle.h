@@ -83,6 +83,11 @@ void orc_chain_latent(const orc_chain* ch, double* y);         /* [n] current y 
 void orc_chain_leaf(const orc_chain* ch, uint8_t* leaf);       /* [m][n] leaf slot of each row per tree */
 void orc_chain_pack_trees(const orc_chain* ch, uint64_t* words); /* [m][64] current trees as output words */
 long orc_chain_trace_len(const orc_chain* ch);
+/* proposal study (no reference counterpart): births draw their rule with probability alpha from the rule's
+ * conditional given the tree, sigma and the partial residuals, the mixture's probability entering the ratio of births
+ * and deaths; 0 restores the reference's proposal */
+void orc_chain_set_informed(orc_chain* ch, double alpha);
+long orc_chain_informed_births(const orc_chain* ch);
 
 /* whole fit, same contract as bart_fit_cpu with the internal generator; returns 0 or a negative error */
 int orc_fit(const uint8_t* q, const float* y, const float* wt, const int* numcut, const float* pgtab,

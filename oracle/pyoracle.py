@@ -123,6 +123,10 @@ class Oracle:
         L.orc_chain_pack_trees.argtypes = [C.c_void_p, _u64p]
         L.orc_chain_trace_len.argtypes = [C.c_void_p]
         L.orc_chain_trace_len.restype = C.c_long
+        L.orc_chain_set_informed.argtypes = [C.c_void_p, C.c_double]
+        L.orc_chain_set_informed.restype = None
+        L.orc_chain_informed_births.argtypes = [C.c_void_p]
+        L.orc_chain_informed_births.restype = C.c_long
         L.orc_tree_text.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_size_t, C.c_void_p]
         L.orc_tree_text.restype = C.c_size_t
         L.orc_yhat.argtypes = [_u64p, _u8p, C.c_int, C.c_int, C.c_int, _f64p]
@@ -298,6 +302,15 @@ class OracleChain:
         L.orc_chain_stats(self.h, st)
         return dict(rng=rng, sigma=L.orc_chain_sigma(self.h), af=af, latent=y, leaf=leaf.reshape(self.m, self.n),
                     trees=words.reshape(self.m, NODES), stats=st[:3].copy(), it=self.it)
+
+    def set_informed(self, alpha):
+        """Proposal study (no reference counterpart): births draw their rule with probability alpha from the rule's
+        conditional given the tree, sigma and the partial residuals; 0 is the reference's proposal."""
+        self.o.L.orc_chain_set_informed(self.h, float(alpha))
+
+    @property
+    def informed_births(self):
+        return self.o.L.orc_chain_informed_births(self.h)
 
     def trmean(self):
         out = np.zeros(self.n, np.float32)

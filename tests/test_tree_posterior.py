@@ -61,3 +61,26 @@ def test_oracle_plain_sampler_matches_enumerated_posterior(oracle):
     tv, zmax, k = check_against(post, per_chain)
     print(f"trees {len(post)}, checked {k}, TV {tv:.4f}, max |z| {zmax:.2f}")
     assert tv < 0.03 and zmax < 5.0
+
+
+def test_oracle_informed_rule_keeps_the_enumerated_posterior(oracle):
+    """The proposal study's informed birth rule (the rule's exact conditional mixed with the reference's uniform rule,
+    its probability in the births' and deaths' ratios) leaves the target unchanged: the visit frequencies over tree
+    structures match the enumeration, at alpha 0.8 and with the informed rule alone (alpha 1)."""
+    pb = tiny_problem()
+    post = TP.posterior(pb)
+    for alpha in (0.8, 1.0):
+        per_chain, informed = [], 0
+        for c in range(48):
+            ch = oracle.chain(pb, 1, 20, 600, 300 + c)
+            ch.set_informed(alpha)
+            ch.sweeps(620)
+            informed += ch.informed_births
+            t = ch.trees_out.reshape(600, 1, -1)
+            per_chain.append([TP.tree_from_words(t[d, 0]) for d in range(600)])
+            ch.close()
+        tv, zmax, k = check_against(post, per_chain)
+        print(f"alpha {alpha}: informed births {informed}, trees {len(post)}, checked {k}, TV {tv:.4f}, "
+              f"max |z| {zmax:.2f}")
+        assert informed > 1000
+        assert tv < 0.03 and zmax < 5.0

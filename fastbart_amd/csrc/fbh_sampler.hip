@@ -721,6 +721,14 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
 #pragma unroll
   for (int g = 0; g < kGroups; g++) acc[g] = 0;
   const int nper_kb = kb < n ? (n - kb + kLanes - 1) / kLanes : 0;
+  // FX64 sums are order-free: when the tasks take at most half of a lane's 16 slots, the spare slots run replicas of
+  // the tasks -- with tp = ntask rounded up to a power of two, slot = task + tp rep for 16 / tp replicas -- each over a
+  // contiguous share of the lane's rows of every tile, and the replicas' totals are added at the end of the pass.
+  // REF16 keeps one thread per task, walking every row in order.
+  const int lgtp = ntask <= 2 ? 1 : (ntask <= 4 ? 2 : (ntask <= 8 ? 3 : 4));
+  const bool rp = FX && lgtp < 4;
+  const int lgrep = rp ? 4 - lgtp : 0, nrep = 1 << lgrep;
+  const int rtask = rp ? (slot & ((1 << lgtp) - 1)) : slot, rep = rp ? (slot >> lgtp) : 0;
   // rows of the birth proposal's halves in this wave's lanes (wave-uniform: counted with ballots; lane 0 takes part in
   // every ballot, as a quad of a higher lane starts below n only if lane 0's does)
   int my_nl = 0, my_nr = 0;
@@ -835,25 +843,41 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
     // the next tile's loads go out now and arrive during phase B
     if (tb + G::kTile < n) tile_load<R, RES>(T, P, la, lb, qa, qb, ii0 + G::kTW4, tq, qlast);
     const int cnt = min(G::kTW4, max(0, nper_kb - ii0));
+    // this thread's rows of the lane's tile: all of them, or a replica's share (whole 16-row blocks but the last)
+    int r0 = 0, rn = cnt;
+    if (rp) {
+      const int share = (((cnt + nrep - 1) >> lgrep) + 15) & ~15;
+      r0 = min(cnt, rep * share);
+      rn = min(cnt - r0, share);
+    }
 #pragma unroll
     for (int g = 0; g < kGroups; g++) {
-      const int task = slot + kTaskSlots * g;
-      if (task < ntask) {
+      const int task = rtask + kTaskSlots * g;
+      if (task < ntask && (!rp || g == 0)) {
         const int q = task >> 1;
         const uint32_t sh = (!rss && q >= nleaf) ? 6u : 0u;
         const uint32_t wd = rss ? 0u : (q < nleaf ? 5u : 2u);
         const uint32_t pat = rss ? 1u : (q < nleaf ? (1u << q) : (q == nleaf ? 2u : 4u));
-        acc[g] = lane_sum<FX>(tgb, (task & 1) ? v1b : v0b, cnt, sh, wd, pat, acc[g]);
+        acc[g] = lane_sum<FX>(tgb + r0, ((task & 1) ? v1b : v0b) + r0, rn, sh, wd, pat, acc[g]);
       }
     }
     wave_sync();
   }
   if (lane == 0 && my_nl) atomicAdd(&c.nL, my_nl);
   if (lane == 0 && my_nr) atomicAdd(&c.nR, my_nr);
+  if (rp) {
+    // the first replica adds the others' totals (int64 adds: exact in any order); replica k of a task sits tp k slots
+    // above it, in the same 16-slot group of the lane
+    const long long a = __double_as_longlong(acc[0]);
+    long long t = a;
+    for (int k = 1; k < nrep; k++) t += __shfl(a, lane + (k << lgtp));
+    if (rep == 0 && rtask < ntask) c.part[rtask][kb] = __longlong_as_double(t);
+  } else {
 #pragma unroll
-  for (int g = 0; g < kGroups; g++) {
-    const int task = slot + kTaskSlots * g;
-    if (task < ntask) c.part[task][kb] = acc[g];
+    for (int g = 0; g < kGroups; g++) {
+      const int task = slot + kTaskSlots * g;
+      if (task < ntask) c.part[task][kb] = acc[g];
+    }
   }
 }
 

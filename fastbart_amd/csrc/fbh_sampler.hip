@@ -1614,6 +1614,7 @@ constexpr int kClasses = FBH_PMAX * kCuts;             // 8160
 constexpr double kFxFit = 4294967296.0;                // running fit at 2^-32
 constexpr double kFxFeat = 1048576.0;                  // feature sums at 2^-20
 constexpr uint16_t kNoRange = 0x00FF;                  // L = 255 > U = 0
+constexpr int kHistCopies = 4;                         // copies of the node histograms (a power of two)
 
 struct LpState {
   uint32_t rng[4];
@@ -1665,18 +1666,30 @@ __device__ __forceinline__ RowPtrs lp_rows(const KArgs& a, const ChainDesc& cd) 
 }
 
 // the fixed-point fit follows the change of tree jp (c.tho -> c.thn, leaf ids jp before the data pass rewrites
-// them); each data wave takes the rows of its own summation lanes, as the data pass does
+// them). Each data wave takes the row quads of its own summation lanes, as the data pass does (rows 16 s + 4 dw ..
+// 16 s + 4 dw + 3: one dword of each byte column, 16-byte aligned and padded), the loads of a quad issued together.
 __device__ void lp_afx_apply(const Ctl& c, const RowPtrs& P, long long* __restrict__ AFX, int jp, int dw, int lane) {
   const int n = P.n;
+  const int pmode = c.pmode, pnx = c.pnx, pc = c.pc, pL = c.pL, pR = c.pR, pl = c.pl, pr = c.pr;
   const uint8_t* lp = P.LEAF + (size_t)jp * P.ns;
-  for (int i = 16 * (lane >> 2) + kLanesPerWave * dw + (lane & 3); i < n; i += 256) {
-    const int lf = lp[i];
-    int lf2 = lf;
-    if (c.pmode == 1 && lf == c.pnx)
-      lf2 = (int)P.Q[(size_t)c.pv * P.ns + i] <= c.pc ? c.pL : c.pR;
-    else if (c.pmode == 2 && (lf == c.pl || lf == c.pr))
-      lf2 = c.pnx;
-    AFX[i] += fit_fx(c.thn[lf2]) - fit_fx(c.tho[lf]);
+  const uint8_t* qp = P.Q + (size_t)(pmode == 1 ? c.pv : 0) * P.ns;
+  for (int i4 = 16 * lane + kLanesPerWave * dw; i4 < n; i4 += 16 * 64) {
+    const uint32_t lw = *reinterpret_cast<const uint32_t*>(lp + i4);
+    const uint32_t qw = *reinterpret_cast<const uint32_t*>(qp + i4);
+    long long x[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) x[u] = i4 + u < n ? AFX[i4 + u] : 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (i4 + u >= n) break;
+      const int lf = (int)((lw >> (8 * u)) & 255u);
+      int lf2 = lf;
+      if (pmode == 1 && lf == pnx)
+        lf2 = (int)((qw >> (8 * u)) & 255u) <= pc ? pL : pR;
+      else if (pmode == 2 && (lf == pl || lf == pr))
+        lf2 = pnx;
+      AFX[i4 + u] = x[u] + (fit_fx(c.thn[lf2]) - fit_fx(c.tho[lf]));
+    }
   }
 }
 
@@ -1720,15 +1733,17 @@ __device__ void lp_choose(Ctl& c, Rng& g, int lane, int* mode_out, int* nx_out) 
 }
 
 template <bool FX>
-__global__ __launch_bounds__(kBlock) void k_lp_stage1(LpArgs A) {
+__global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(4))) void k_lp_stage1(LpArgs A) {
   __shared__ Ctl c;
-  __shared__ unsigned long long H[kFeat];
+  // the node's histograms in kHistCopies copies, lane l adding into copy l % kHistCopies: a wave's 64 lanes then spread
+  // over 4 x 16 bins of a variable instead of 16 (fewer lanes serialised on one address); the copies are added after
+  __shared__ unsigned long long H[kHistCopies][kFeat];
   unsigned char* const dyn = fbh_dyn;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, dw = wave - 1;
   const int chain = blockIdx.x, j = A.j;
   const KArgs& a = A.a;
   img_copy(&c, A.img + chain, tid);
-  for (int k = tid; k < kFeat; k += kBlock) H[k] = 0;
+  for (int k = tid; k < kHistCopies * kFeat; k += kBlock) (&H[0][0])[k] = 0;
   __syncthreads();
   const RowPtrs P = lp_rows(a, c.cd);
   const int n = c.cd.n, p = c.cd.p;
@@ -1771,23 +1786,60 @@ __global__ __launch_bounds__(kBlock) void k_lp_stage1(LpArgs A) {
     const TreeDev& t = c.tree[0];
     const int l = mode == 2 ? t.l[nx] : nx, r = mode == 2 ? t.r[nx] : nx;
     const uint8_t* lc = P.LEAF + (size_t)j * P.ns;
-    for (int i = tid; i < n; i += kBlock) {
-      const int lf = lc[i];
-      if (lf != l && lf != r) continue;
-      const double res = (double)P.Y[i] - (double)(AFX[i] - fit_fx(c.thr[0][lf].th)) * (1.0 / kFxFit);
-      const double w = (double)P.W[i];
-      const unsigned long long wr = (unsigned long long)(long long)rint(w * res * kFxFeat);
-      for (int v = 0; v < p; v++) atomicAdd(&H[v * kFeatBins + (P.Q[(size_t)v * P.ns + i] >> 4)], wr);
-      atomicAdd(&H[kFeat - 1], (unsigned long long)(long long)rint(w * kFxFeat));
+    unsigned long long* Hc = H[lane & (kHistCopies - 1)];
+    long long wsum = 0;
+    // four consecutive rows per thread and iteration: the byte columns (16-byte aligned, padded to a multiple of 16
+    // rows) as one dword each, and the cutpoint ranks of eight variables loaded together before their adds, so that
+    // an iteration waits for memory three or four times instead of once per variable and row
+    for (int i4 = 4 * tid; i4 < n; i4 += 4 * kBlock) {
+      const uint32_t lw = *reinterpret_cast<const uint32_t*>(lc + i4);
+      bool in[4];
+      unsigned long long wr[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int i = i4 + u;
+        const int lf = (int)((lw >> (8 * u)) & 255u);
+        in[u] = i < n && (lf == l || lf == r);
+        wr[u] = 0;
+        if (in[u]) {
+          const double res = (double)P.Y[i] - (double)(AFX[i] - fit_fx(c.thr[0][lf].th)) * (1.0 / kFxFit);
+          const double w = (double)P.W[i];
+          wr[u] = (unsigned long long)(long long)rint(w * res * kFxFeat);
+          wsum += (long long)rint(w * kFxFeat);
+        }
+      }
+      if (!(in[0] || in[1] || in[2] || in[3])) continue;
+      for (int v0 = 0; v0 < p; v0 += 8) {
+        uint32_t qw[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          qw[k] = v0 + k < p ? *reinterpret_cast<const uint32_t*>(P.Q + (size_t)(v0 + k) * P.ns + i4) : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          if (v0 + k >= p) break;
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            if (in[u]) atomicAdd(&Hc[(v0 + k) * kFeatBins + (((qw[k] >> (8 * u)) & 255u) >> 4)], wr[u]);
+        }
+      }
     }
+    // the weight total: one add per thread
+    if (wsum) atomicAdd(&Hc[kFeat - 1], (unsigned long long)wsum);
   }
+  __syncthreads();
+  if (mode)
+    for (int k = tid; k < kFeat; k += kBlock) {
+      unsigned long long t = H[0][k];
+      for (int h = 1; h < kHistCopies; h++) t += H[h][k];
+      H[0][k] = t;
+    }
   __syncthreads();
   float* X = A.X + (size_t)chain * kFeat;
   if (mode) {
-    const double wt = (double)(long long)H[kFeat - 1];
+    const double wt = (double)(long long)H[0][kFeat - 1];
     const double inv = wt > 0 ? 1.0 / wt : 0.0;
     for (int k = tid; k < kFeat - 1; k += kBlock)
-      X[k] = (k / kFeatBins) < p ? (float)((double)(long long)H[k] * inv) : 0.f;
+      X[k] = (k / kFeatBins) < p ? (float)((double)(long long)H[0][k] * inv) : 0.f;
     if (tid == 0) X[kFeat - 1] = (float)c.tree[0].d[nx] * 0.125f;
     if (wave == 0 && lane < FBH_PMAX) {
       uint16_t rv = kNoRange;
@@ -1814,7 +1866,7 @@ __device__ __forceinline__ bool lp_valid(const uint16_t* rg, int k) {
 }
 
 template <bool FX>
-__global__ __launch_bounds__(kBlock) void k_lp_stage2(LpArgs A) {
+__global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(4))) void k_lp_stage2(LpArgs A) {
   __shared__ Ctl c;
   __shared__ float red[kBlock];
   __shared__ float sv[FBH_PMAX];
@@ -1840,9 +1892,16 @@ __global__ __launch_bounds__(kBlock) void k_lp_stage2(LpArgs A) {
   // masked softmax of the chain's logits over the classes allowed at the node: max, total, per-variable totals
   if (mode && alpha > 0.f) {
     const float* z = A.Z + (size_t)chain * A.ldz;
+    // one wave per variable, lanes over its allowed cuts (at most four per lane, loaded together)
     float mx = -INFINITY;
-    for (int k = tid; k < kClasses; k += kBlock)
-      if (lp_valid(rg, k)) mx = fmaxf(mx, z[k]);
+    for (int v = wave; v < p; v += kBlock / 64) {
+      const int L = rg[v] & 255, U = rg[v] >> 8;
+#pragma unroll
+      for (int k = 0; k < (kCuts + 63) / 64; k++) {
+        const int cc = L + lane + 64 * k;
+        if (cc <= U) mx = fmaxf(mx, z[v * kCuts + cc]);
+      }
+    }
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
     if (lane == 0) red[wave] = mx;
     __syncthreads();
@@ -1855,10 +1914,18 @@ __global__ __launch_bounds__(kBlock) void k_lp_stage2(LpArgs A) {
     mx = zmax;
     // per-variable sums: one wave per variable group, lanes over the cuts (fixed order, so reproducible)
     for (int v = wave; v < p; v += kBlock / 64) {
+      const int L = rg[v] & 255, U = rg[v] >> 8;
+      float zc[(kCuts + 63) / 64];
+#pragma unroll
+      for (int k = 0; k < (kCuts + 63) / 64; k++) {
+        const int cc = lane + 64 * k;
+        zc[k] = (cc >= L && cc <= U) ? z[v * kCuts + cc] : 0.f;
+      }
       float s = 0.f;
-      for (int cc = lane; cc < kCuts; cc += 64) {
-        const int k = v * kCuts + cc;
-        if (lp_valid(rg, k)) s += expf(z[k] - mx);
+#pragma unroll
+      for (int k = 0; k < (kCuts + 63) / 64; k++) {
+        const int cc = lane + 64 * k;
+        if (cc >= L && cc <= U) s += expf(zc[k] - mx);
       }
       for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
       if (lane == 0) sv[v] = s;

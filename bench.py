@@ -52,6 +52,7 @@ WORKLOADS = {
     "learned512": (512, 200, 1000, 100, 0, 0, 0, 20, None),
     "predict600": (6, 200, 1000, 100, 0, 0, 0, 3, 20),
     "predict_probit100": (1, 50, 1000, 100, 1, 0, 0, 3, 20),
+    "text512": (512, 200, 1000, 100, 0, 0, 0, 0, None),
 }
 
 
@@ -465,6 +466,71 @@ def predict_bench(args, wl):
     return 0
 
 
+# ------------------------------------------------------------------ treedraws text of a batched fit
+def text_bench(args):
+    """The output half of fbart_cpp for every chain of the 512-chain workload: the fit runs on the device (not
+    timed), each chain's kept trees are copied to the host, then formatted as treedraws text with varcount by the
+    library's threaded writer (src/glue.cpp:32-51). value = trees formatted per second over all chains (10.24 M
+    trees); the copies to the host are timed apart."""
+    from fastbart_amd import sampler
+    from fastbart_amd import workloads as Wk
+    nch, M, NSKIP, NDPOST = 512, 200, 1000, 100
+    chains = min(nch, args.chains) if args.chains else nch
+    probs = [Wk.batch512_chain(c) for c in range(chains)]
+    t0 = time.perf_counter()
+    with sampler.BatchSampler(probs, M, NSKIP, NDPOST, [1 + c for c in range(chains)], keep_trees=1,
+                              chains_per_cu=2 if chains > 256 else 0, packing=1) as b:
+        b.run(NSKIP + NDPOST)
+        b.sync()
+        log(f"text512: fitted {chains} chains in {time.perf_counter() - t0:.1f} s")
+        t_fetch = t_fmt = 0.0
+        nbytes = 0
+        sample = []
+        for c in range(chains):
+            t1 = time.perf_counter()
+            trees = b.fetch(c)["trees"]
+            t2 = time.perf_counter()
+            text, vc = sampler.tree_text(trees, probs[c]["q"].shape[0])
+            t3 = time.perf_counter()
+            t_fetch += t2 - t1
+            t_fmt += t3 - t2
+            nbytes += len(text)
+            if c < 4:
+                sample.append((trees, probs[c]["q"].shape[0], text))
+    ntrees = chains * NDPOST * M
+    value = ntrees / t_fmt
+    log(f"text512: {ntrees / 1e6:.2f} M trees, {nbytes / 1e6:.0f} MB of text in {t_fmt:.2f} s "
+        f"({value / 1e6:.2f} M trees/s); host copies of the kept trees {t_fetch:.2f} s")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pyoracle
+        o = pyoracle.Oracle()
+        t4 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t4 < 5.0 or k == 0:
+            trees, p, text = sample[k % len(sample)]
+            ref, _ = o.tree_text(trees, p)
+            assert ref == text, "treedraws text differs from the oracle's"
+            k += 1
+        cs = time.perf_counter() - t4
+        cpu = dict(value=k * NDPOST * M / cs, unit="trees/s", cores=1, kind="port",
+                   sample=f"{k} chains' treedraws text (20,000 trees each) by the oracle's restatement of "
+                          f"src/glue.cpp:32-51 (snprintf %.10g), one core, {cs:.1f} s; the texts equal the library's")
+    line = {
+        "metric": "treedraws text trees/sec (output half of fbart_cpp, all chains of the batched fit)",
+        "value": value, "unit": "trees/s", "n_gpus": 1, "steps": chains, "warmup": 0,
+        "ms_per_step": t_fmt / chains * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 leaf values as %.10g", "data": "synthetic",
+        "config": {"workload": describe("batch512", chains, NSKIP, NDPOST) + "; treedraws text of every kept draw",
+                   "chains": chains, "trees": ntrees, "text_bytes": nbytes, "host_threads": os.cpu_count()},
+        "host_copy_s": t_fetch,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 # ------------------------------------------------------------------ main
 def main() -> int:
     ap = argparse.ArgumentParser()
@@ -508,6 +574,8 @@ def main() -> int:
     wl = args.workload
     if wl.startswith("predict"):
         return predict_bench(args, wl)
+    if wl == "text512":
+        return text_bench(args)
     nch, M, NSKIP, NDPOST, probit, stats_mode, node_cap, w_def, k_def = WORKLOADS[wl]
     chains = min(nch, args.chains) if args.chains else nch
     if wl == "learned512":

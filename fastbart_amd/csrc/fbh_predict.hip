@@ -318,8 +318,15 @@ void tree_lines(const uint64_t* T, int p, std::string& s, int* vc) {
     s.push_back(' ');
     put_int((unsigned long long)c);
     s.push_back(' ');
-    const int len = std::snprintf(buf, sizeof buf, "%.10g", th);
-    s.append(buf, buf + len);
+    // std::to_chars with general format and precision 10 is printf's %.10g (the ostream precision(10) of
+    // src/glue.cpp:49), several times faster; snprintf for the non-finite values a corrupt input could hold
+    if (std::isfinite(th)) {
+      auto r = std::to_chars(buf, buf + sizeof buf, th, std::chars_format::general, 10);
+      s.append(buf, r.ptr);
+    } else {
+      const int len = std::snprintf(buf, sizeof buf, "%.10g", th);
+      s.append(buf, buf + len);
+    }
     s.push_back('\n');
   }
 }

@@ -229,10 +229,16 @@ def tree_text(trees, p):
     trees = np.ascontiguousarray(trees, dtype=np.uint64)
     nd, m = trees.shape[0], trees.shape[1]
     L = lib()
-    need = L.fbh_tree_text(ptr(trees), nd, m, p, None, 0, None)
-    buf = C.create_string_buffer(need + 1)
     vc = np.zeros((nd, p), np.int32)
-    L.fbh_tree_text(ptr(trees), nd, m, p, C.cast(buf, C.c_void_p), need + 1, ptr(vc))
+    # one formatting pass in the usual case: a buffer of 128 bytes a tree (the text of a tree of the workloads is about
+    # 80), a second call with the exact length only if the text is longer
+    cap = nd * m * 128 + 64
+    buf = C.create_string_buffer(cap)
+    need = L.fbh_tree_text(ptr(trees), nd, m, p, C.cast(buf, C.c_void_p), cap, ptr(vc))
+    if need + 1 > cap:
+        buf = C.create_string_buffer(need + 1)
+        vc[:] = 0
+        L.fbh_tree_text(ptr(trees), nd, m, p, C.cast(buf, C.c_void_p), need + 1, ptr(vc))
     return buf.raw[:need].decode(), vc
 
 

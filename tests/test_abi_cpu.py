@@ -109,6 +109,27 @@ def test_tree_text_number_format_is_ostream_precision_10(tmp_path):
     assert r.stdout.split("\n")[:-1] == ["%.10g" % v for v in vals]
 
 
+def test_tree_text_leaf_values_are_printf_10g():
+    """The writer's leaf values (std::to_chars, general format, precision 10) are printf's %.10g for every float:
+    random bit patterns over all exponents, values of the samplers' range, signed zeros, subnormals, the extremes."""
+    rng = np.random.default_rng(5)
+    bits = rng.integers(0, 2 ** 32, 60000, dtype=np.uint64).astype(np.uint32)
+    vals = bits.view(np.float32)
+    vals = vals[np.isfinite(vals)]
+    extra = np.array([0.0, -0.0, 1.0, -1.5, 1e-45, -1e-45, 1.1754944e-38, 3.4028235e38, 123456.789, 0.1, 1e10, 1e-5,
+                      9.999999e9], np.float32)
+    small = (rng.standard_normal(20000) * 10.0 ** rng.integers(-6, 3, 20000)).astype(np.float32)
+    vals = np.concatenate([vals, extra, small])
+    # one draw of root-only trees: the word of a leaf is its float's bits above the (zero) rule and child bytes
+    words = np.zeros((1, len(vals), 64), np.uint64)
+    words[0, :, 0] = vals.view(np.uint32).astype(np.uint64) << np.uint64(32)
+    text, _ = sampler.tree_text(words, 1)
+    lines = text.split("\n")
+    assert lines[0] == f"1 {len(vals)} 1"
+    got = [lines[2 + 2 * k].split(" ")[3] for k in range(len(vals))]
+    assert got == ["%.10g" % float(v) for v in vals]
+
+
 def test_compile_trees_parser_matches_oracle(oracle, golden_cases):
     from fastbart_amd.predict import CompiledTrees
     g = golden_cases["w200_p3_extreme"]

@@ -201,3 +201,32 @@ def test_gpu_workload_chain_full_length_matches_oracle(oracle, name):
     e = oracle.fit(pb, m, nskip, nd, 1, probit=probit)
     for k in ("sigma", "trmean", "trees", "stats"):
         assert same_bits(out[k], e[k]), k
+
+
+def test_gpu_headline_batch_full_length_matches_oracle():
+    """The headline configuration end to end: all 512 chains of the batched workload in one persistent launch per 20
+    sweeps (the dense build, two chains per CU, serpentine packing, hipGraph capture, running fits in LDS), all 1,100
+    sweeps; sixteen chains spread over the batch against the oracle, every output bit for bit."""
+    from concurrent.futures import ProcessPoolExecutor
+    import multiprocessing as mp
+    import oracle_pool
+    idx = list(range(0, 480, 32)) + [511]
+    # the oracle's fits run in CPU worker processes (started before the GPU work) while the device samples
+    pool = ProcessPoolExecutor(max_workers=min(len(idx), 8), mp_context=mp.get_context("spawn"))
+    futs = [pool.submit(oracle_pool.fit_batch512_chain, (c, 200, 1000, 100)) for c in idx]
+    probs = [W.batch512_chain(c) for c in range(512)]
+    _say("headline batch: 512 chains, 1,100 sweeps")
+    with sampler.BatchSampler(probs, 200, 1000, 100, [1 + c for c in range(512)], chains_per_cu=2, packing=1,
+                              tile_rows=1024, resident=2, graph=1, sweeps_per_launch=20) as b:
+        cfg = b.config
+        assert (cfg["chains_per_cu"], cfg["packing"], cfg["tile_rows"], cfg["resident"], cfg["graph"]) == (2, 1, 1024,
+                                                                                                           2, 1)
+        b.run(1100)
+        b.sync()
+        outs = {c: b.fetch(c) for c in idx}
+    _say("headline batch: device done; waiting for the oracle")
+    exp = dict(f.result() for f in futs)
+    pool.shutdown()
+    for c in idx:
+        for k in ("sigma", "trmean", "trees", "stats"):
+            assert same_bits(outs[c][k], exp[c][k]), (c, k)

@@ -262,6 +262,7 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
     b.sync()
     t_plain = time.perf_counter() - t0
     ess_plain, mean_plain, dist_plain = ess_of(b)
+    births_plain = int(sum(int(b.fetch(c)["stats"][0]) for c in range(chains)))
     b.close()
     log(f"plain: {total} sweeps {t_plain:.2f} s, sum min-ESS {ess_plain:.1f}")
     # learned schedule
@@ -271,10 +272,12 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
     t0 = time.perf_counter()
     b.run(n_plain0)
     b.learned_setup(net, 0.0, pair_capacity=args.pairs)
+    b.learned_target(1 if args.target == "conditional" else 0, seed=11)
     b.run_learned(args.collect)
     b.sync()
     t_a = time.perf_counter() - t0
     npairs = b.learned_pairs()
+    ent, lall = b.learned_target_stats()
     t1 = time.perf_counter()
     losses = b.learned_train(args.train_steps, batch=512, lr=args.lr, seed=3)
     t_train = time.perf_counter() - t1
@@ -290,8 +293,10 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
     b.close()
     fwd_ms, _ = net.timing() if hasattr(net, "timing") else (0.0, 0)
     corr = float(np.mean([np.corrcoef(a, bb)[0, 1] for a, bb in zip(mean_plain, mean_learned)]))
+    loss0, loss1 = float(losses[:20].mean()), float(losses[-20:].mean())
     log(f"learned: {t_learned:.2f} s sampling + {t_train:.2f} s training on {npairs} pairs "
-        f"(loss {losses[:20].mean():.3f} -> {losses[-20:].mean():.3f}); sum min-ESS {ess_learned:.1f}; "
+        f"(loss {loss0:.3f} -> {loss1:.3f}; target entropy {ent:.3f}, uniform rule {lall:.3f} nats); "
+        f"sum min-ESS {ess_learned:.1f}; "
         f"posterior-mean correlation with plain {corr:.4f}")
     value = chains * M * total / t_learned
     ess_ps, ess_pp = ess_learned / t_learned, ess_plain / t_plain
@@ -309,7 +314,11 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
         "plain": {"min_ess_per_sec": ess_pp, "seconds": t_plain, "sum_min_ess": ess_plain, **dist_plain},
         "learned": {"min_ess_per_sec": ess_ps, "seconds_sampling": t_learned, "seconds_training": t_train,
                     "sum_min_ess": ess_learned, **dist_learned, "pairs": npairs, "ms_per_learned_sweep": ms_sweep,
-                    "learned_sweeps": nls, "births": int(st[:, 0].sum()), "deaths": int(st[:, 1].sum())},
+                    "learned_sweeps": nls, "births": int(st[:, 0].sum()), "deaths": int(st[:, 1].sum()),
+                    "target": args.target, "loss_first20": loss0, "loss_last20": loss1,
+                    "target_entropy_nats": ent if np.isfinite(ent) else None,
+                    "uniform_rule_nats": lall if np.isfinite(lall) else None},
+        "plain_births": births_plain,
         "ess_per_sec_ratio_learned_over_plain": ess_ps / ess_pp if ess_pp else None,
         "posterior_mean_corr": corr,
         "roofline": None,
@@ -566,6 +575,9 @@ def main() -> int:
     ap.add_argument("--pairs", type=int, default=2000000, help="learned512: training-pair capacity")
     ap.add_argument("--alpha", type=float, default=0.8)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--target", choices=("conditional", "accepted"), default="conditional",
+                    help="learned512 training pairs: every proposed birth's node labelled from the exact split "
+                         "conditional, or each accepted birth with BART's rule")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs every chain on a single MI355X "

@@ -88,7 +88,10 @@ EXPORTS = ["bart_fit_cpu", "bart_wfit", "bart_set_rng", "fbh_last_status", "fbh_
            "fbh_debug_math", "fbh_debug_libm_host", "fbh_debug_copy", "fbh_batch_trace", "fbh_version",
            "fbh_mlp_create", "fbh_mlp_nparams", "fbh_mlp_params", "fbh_mlp_forward", "fbh_mlp_grad", "fbh_mlp_train",
            "fbh_mlp_timing", "fbh_mlp_destroy", "fbh_batch_learned_setup", "fbh_batch_run_learned",
-           "fbh_batch_learned_pairs", "fbh_batch_learned_train", "fbh_batch_learned_timing", "fbh_prep_defaults",
+           "fbh_batch_learned_pairs", "fbh_batch_learned_train", "fbh_batch_learned_timing",
+           "fbh_batch_learned_target", "fbh_batch_learned_target_stats", "fbh_debug_split_conditional",
+           "fbh_batch_learned_pairs_copy",
+           "fbh_prep_defaults",
            "fbh_wbart_inputs", "fbh_pbart_inputs", "fbh_inputs_free", "fbh_qnorm", "fbh_qchisq"]
 
 _lock = threading.Lock()
@@ -167,6 +170,11 @@ def _declare(L):
     L.fbh_batch_learned_pairs.restype = C.c_long
     L.fbh_batch_learned_train.argtypes = [vp, i, i, C.c_float, C.c_uint, vp]
     L.fbh_batch_learned_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i)]
+    L.fbh_batch_learned_target.argtypes = [vp, i, C.c_ulonglong]
+    L.fbh_batch_learned_pairs_copy.argtypes = [vp, C.c_long, C.c_long, vp, vp, vp, vp]
+    L.fbh_batch_learned_target_stats.argtypes = [vp, C.POINTER(d), C.POINTER(d)]
+    L.fbh_debug_split_conditional.argtypes = [i, i, vp, vp, vp, vp, vp, i, i, i, vp, d, d, vp, i, vp, vp,
+                                              C.POINTER(d)]
     L.fbh_prep_defaults.argtypes = [i, C.POINTER(PrepOpts)]
     L.fbh_prep_defaults.restype = None
     L.fbh_wbart_inputs.argtypes = [vp, i, i, vp, vp, vp, vp, C.POINTER(PrepOpts), C.POINTER(C.POINTER(Inputs))]

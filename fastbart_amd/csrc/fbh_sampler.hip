@@ -1606,7 +1606,10 @@ constexpr int kRowsStreamL = kRowsStream;  // the learned pipeline streams the r
 // rule (v, c) by q_mix / q_unif, where q_unif = (1/p) / (U - L + 1) is BART's rule draw and q_mix = (1 - alpha) q_unif +
 // alpha q_net(v, c | features of the node). The features are the node's partial residuals (excluding tree j) binned
 // by each variable's cutpoint rank, computed from an int64 fixed-point running fit so that they are bitwise the same
-// in the state before a birth and in the state after it (where the reverse death evaluates them).
+// in the state before a birth and in the state after it (where the reverse death evaluates them). Each bin holds
+// sum(w r) sqrt(1 / (sigma^2 sum(w))) over the node's rows: in these units half the square of a child's partial sum
+// is its log-likelihood gain for large sums, so the scale of the split posterior is visible to the network (sigma is
+// fixed within a sweep, so the forward and the reverse move see the same value).
 constexpr int kFeatBins = 16;                          // cutpoint-rank bins per variable (rank >> 4)
 constexpr int kFeat = FBH_PMAX * kFeatBins + 1;        // 513: histograms + depth
 constexpr int kCuts = 255;                             // classes (v, c) -> v * 255 + c
@@ -1637,6 +1640,9 @@ struct LpArgs {
   int* ptg;
   unsigned long long* pcount;
   long pcap;
+  float* pent;              // [pcap][2]: entropy of the pair's split conditional, log of its allowed class count
+  unsigned long long tseed; // label draws of the conditional target (a stream apart from the chains' generators)
+  int target;               // pairs: 0 = accepted births with BART's rule, 1 = every proposed birth, conditional label
   int j;
 };
 
@@ -1836,10 +1842,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
   __syncthreads();
   float* X = A.X + (size_t)chain * kFeat;
   if (mode) {
-    const double wt = (double)(long long)H[0][kFeat - 1];
-    const double inv = wt > 0 ? 1.0 / wt : 0.0;
+    const double sw = (double)(long long)H[0][kFeat - 1] * (1.0 / kFxFeat);
+    const double scale = sw > 0 ? __builtin_sqrt(c.inv_s2 / sw) * (1.0 / kFxFeat) : 0.0;
     for (int k = tid; k < kFeat - 1; k += kBlock)
-      X[k] = (k / kFeatBins) < p ? (float)((double)(long long)H[0][k] * inv) : 0.f;
+      X[k] = (k / kFeatBins) < p ? (float)((double)(long long)H[0][k] * scale) : 0.f;
     if (tid == 0) X[kFeat - 1] = (float)c.tree[0].d[nx] * 0.125f;
     if (wave == 0 && lane < FBH_PMAX) {
       uint16_t rv = kNoRange;
@@ -1856,6 +1862,323 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
   }
   __syncthreads();
   img_copy(A.img + chain, &c, tid);
+}
+
+// ---- the exact split conditional at a birth node: the training target of the proposal network (no reference
+// counterpart). Given the tree, sigma and the node's partial residuals, a birth's rule (v, c) has the conditional
+//   q(v, c) proportional to (1 - PGl)(1 - PGr) PD / (U_v - L_v + 1) * exp(lh(left) + lh(right))
+// over the rules with L_v <= c <= U_v whose children hold at least 5 rows each: BART's own rule probability times the
+// factors of the birth ratio that depend on the rule (src/bart_kernel.cpp:441-451 and the likelihoods of :486); the
+// remaining factors are the same for every rule. A network trained on labels drawn from q learns, in expectation,
+// cross-entropy against q itself; its floor is the entropy of q, the uniform rule's level the log of the allowed
+// class count. Sums of w and w r per (variable, cutpoint rank) are int64 fixed point at 2^-20 (order-free), and prefix
+// sums along the ranks give every rule's left child at once.
+constexpr int kTgtBlock = 256;
+constexpr int kTgtGroup = 8;  // variables per histogram round
+constexpr double kFxTgt = 1048576.0;
+
+struct SplitNode {
+  int n, p, ns;            // rows, variables, column stride of Q
+  int d, ngv, ngood;       // depth and good-variable count of the node, good leaves of the tree
+  double inv_s2, t2;
+  const double* PG;        // depth prior table (:241-242)
+  const uint8_t* Q;        // [p][ns] cutpoint ranks
+  uint16_t rg[FBH_PMAX];   // L | U << 8 per variable (kNoRange: no cut)
+};
+
+struct SplitLds {
+  unsigned long long h0[kTgtGroup][256], h1[kTgtGroup][256];
+  unsigned hn[kTgtGroup][256];
+  float lw[FBH_PMAX][256];   // log weight of each rule, -inf where the rule does not qualify
+  double sv[FBH_PMAX], se[FBH_PMAX];
+  double red[kTgtBlock / 64];
+  double mx, Z, H;
+};
+
+__device__ __forceinline__ unsigned long long shfl_up_u64(unsigned long long x, int o) {
+  const unsigned lo = __shfl_up((unsigned)x, o), hi = __shfl_up((unsigned)(x >> 32), o);
+  return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ unsigned long long shfl_u64(unsigned long long x, int l) {
+  const unsigned lo = __shfl((unsigned)x, l), hi = __shfl((unsigned)(x >> 32), l);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// q at the node over all threads of a kTgtBlock workgroup. row(i, &wf, &wrf) says whether row i is in the node and
+// gives w and w r at 2^-20. Returns false when no rule qualifies; else S.lw, S.mx, S.sv (per-variable totals of
+// exp(lw - mx)), S.Z (their sum) and S.H (the entropy of q) are set.
+template <class RowFn>
+__device__ bool split_conditional(const SplitNode& N, SplitLds& S, RowFn row) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int k = tid; k < FBH_PMAX * 256; k += kTgtBlock) (&S.lw[0][0])[k] = -INFINITY;
+  const double pb = 0.5;
+  for (int g0 = 0; g0 < N.p; g0 += kTgtGroup) {
+    for (int k = tid; k < kTgtGroup * 256; k += kTgtBlock) {
+      (&S.h0[0][0])[k] = 0;
+      (&S.h1[0][0])[k] = 0;
+      (&S.hn[0][0])[k] = 0;
+    }
+    __syncthreads();
+    const int ng = min(kTgtGroup, N.p - g0);
+    for (int i = tid; i < N.n; i += kTgtBlock) {
+      long long wf, wrf;
+      if (!row(i, wf, wrf)) continue;
+      for (int g = 0; g < ng; g++) {
+        const int k = N.Q[(size_t)(g0 + g) * N.ns + i];
+        atomicAdd(&S.h0[g][k], (unsigned long long)wf);
+        atomicAdd(&S.h1[g][k], (unsigned long long)wrf);
+        atomicAdd(&S.hn[g][k], 1u);
+      }
+    }
+    __syncthreads();
+    // one wave per variable: lane l holds ranks 4 l .. 4 l + 3, an inclusive scan over the lanes gives the left sums
+    for (int g = wave; g < ng; g += kTgtBlock / 64) {
+      const int v = g0 + g;
+      const int L = N.rg[v] & 255, U = N.rg[v] >> 8;
+      unsigned long long a0[4], a1[4];
+      unsigned an[4];
+      unsigned long long t0 = 0, t1 = 0;
+      unsigned tn = 0;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        t0 += S.h0[g][4 * lane + e];
+        t1 += S.h1[g][4 * lane + e];
+        tn += S.hn[g][4 * lane + e];
+        a0[e] = t0;
+        a1[e] = t1;
+        an[e] = tn;
+      }
+      unsigned long long i0 = t0, i1 = t1;
+      unsigned in_ = tn;
+      for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long y0 = shfl_up_u64(i0, o), y1 = shfl_up_u64(i1, o);
+        const unsigned yn = __shfl_up(in_, o);
+        if (lane >= o) {
+          i0 += y0;
+          i1 += y1;
+          in_ += yn;
+        }
+      }
+      const long long B0 = (long long)shfl_u64(i0, 63), B1 = (long long)shfl_u64(i1, 63);
+      const int Nn = (int)__shfl(in_, 63);
+      const unsigned long long x0 = i0 - t0, x1 = i1 - t1;
+      const unsigned xn = in_ - tn;
+      if (U < L) continue;
+      const double PGd = N.PG[N.d + 1];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int k = 4 * lane + e;
+        const int cn = (int)(xn + an[e]);
+        if (k < L || k > U || cn < 5 || Nn - cn < 5) continue;
+        const long long c0 = (long long)(x0 + a0[e]), c1 = (long long)(x1 + a1[e]);
+        double PGl, PGr;
+        if (N.ngv > 1) {
+          PGl = PGr = PGd;
+        } else {
+          PGl = (k - 1 < L) ? 0 : PGd;
+          PGr = (U < k + 1) ? 0 : PGd;
+        }
+        const double PD = (N.ngood > 1) ? 1 - pb : ((PGr == 0 && PGl == 0) ? 1 : 1 - pb);
+        const double s0 = (double)c0 * (1.0 / kFxTgt), s1 = (double)c1 * (1.0 / kFxTgt);
+        const double r0 = (double)(B0 - c0) * (1.0 / kFxTgt), r1 = (double)(B1 - c1) * (1.0 / kFxTgt);
+        const double lw = log((1 - PGl) * (1 - PGr) * PD / (double)(U - L + 1)) +
+                          lh(s0 * N.inv_s2, s1 * N.inv_s2, N.t2) + lh(r0 * N.inv_s2, r1 * N.inv_s2, N.t2);
+        S.lw[v][k] = (float)lw;
+      }
+    }
+    __syncthreads();
+  }
+  // the largest log weight, then per-variable totals of exp(lw - mx) and of exp(lw - mx) (lw - mx) (fixed order)
+  float m = -INFINITY;
+  for (int k = tid; k < N.p * 256; k += kTgtBlock) m = fmaxf(m, (&S.lw[0][0])[k]);
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if (lane == 0) S.red[wave] = (double)m;
+  __syncthreads();
+  if (tid == 0) {
+    double r = S.red[0];
+    for (int w = 1; w < kTgtBlock / 64; w++) r = fmax(r, S.red[w]);
+    S.mx = r;
+  }
+  __syncthreads();
+  const double mx = S.mx;
+  if (!(mx > -INFINITY)) return false;
+  for (int v = wave; v < N.p; v += kTgtBlock / 64) {
+    double s = 0, e = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const float l = S.lw[v][lane + 64 * q];
+      if (l > -INFINITY) {
+        const double x = (double)l - mx, ex = exp(x);
+        s += ex;
+        e += ex * x;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      s += __shfl_xor(s, o);
+      e += __shfl_xor(e, o);
+    }
+    if (lane == 0) {
+      S.sv[v] = s;
+      S.se[v] = e;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double Z = 0, E = 0;
+    for (int v = 0; v < N.p; v++) {
+      Z += S.sv[v];
+      E += S.se[v];
+    }
+    S.Z = Z;
+    S.H = log(Z) - E / Z;
+  }
+  __syncthreads();
+  return true;
+}
+
+// inverse CDF of q at u in (0, 1), classes in the order v * 255 + c (one wave; every lane returns the class)
+__device__ int split_draw(const SplitNode& N, const SplitLds& S, double u, int lane) {
+  const double target = u * S.Z;
+  double acc = 0;
+  int v = -1, vlast = -1;
+  for (int q = 0; q < N.p; q++) {
+    if (!(S.sv[q] > 0)) continue;
+    if (acc + S.sv[q] > target) {
+      v = q;
+      break;
+    }
+    acc += S.sv[q];
+    vlast = q;
+  }
+  if (v < 0) {  // the total fell short of the target by rounding: the last variable with mass
+    v = vlast;
+    acc -= S.sv[vlast];
+  }
+  int cut = -1, clast = -1;
+  for (int c0 = 0; c0 < 256; c0 += 64) {
+    const float l = S.lw[v][c0 + lane];
+    const double e = l > -INFINITY ? exp((double)l - S.mx) : 0.0;
+    double pre = e;
+    for (int o = 1; o < 64; o <<= 1) {
+      const double y = __shfl_up(pre, o);
+      if (lane >= o) pre += y;
+    }
+    const uint64_t hit = __ballot(e > 0 && acc + pre > target);
+    const uint64_t has = __ballot(e > 0);
+    if (has) clast = c0 + 63 - __builtin_clzll(has);
+    if (hit) {
+      cut = c0 + __builtin_ctzll(hit);
+      break;
+    }
+    acc += __shfl(pre, 63);
+  }
+  if (cut < 0) cut = clast;
+  return v * kCuts + cut;
+}
+
+__device__ __forceinline__ double mix_unif(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// after stage 1 of a collecting sweep with the conditional target: each chain whose move is a birth appends (features
+// of the node, its ranges, a rule drawn from q, the entropy of q and the log of its allowed class count) to the pairs
+__global__ __launch_bounds__(kTgtBlock) void k_lp_target(LpArgs A) {
+  __shared__ SplitLds S;
+  __shared__ SplitNode N;
+  __shared__ int go, nx_s;
+  __shared__ long long slot;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int chain = blockIdx.x, j = A.j;
+  const KArgs& a = A.a;
+  const Ctl& c = A.img[chain];
+  if (tid == 0) {
+    const int mode = c.cv[0], nx = c.cnx[0];
+    const unsigned long long used = __atomic_load_n(A.pcount, __ATOMIC_RELAXED);
+    go = mode == 1 && (int)c.tree[0].d[nx] + 1 < kMaxDepth && (long long)used < (long long)A.pcap;
+    nx_s = nx;
+    N.n = c.cd.n;
+    N.p = c.cd.p;
+    N.ns = c.cd.ns;
+    N.d = c.tree[0].d[nx];
+    N.ngv = c.tree[0].ngv[nx];
+    N.ngood = c.ngood[0];
+    N.inv_s2 = c.inv_s2;
+    N.t2 = (double)c.cd.tau * c.cd.tau;
+    N.PG = a.PG;
+    N.Q = a.q + c.cd.q_off;
+  }
+  if (tid < FBH_PMAX) N.rg[tid] = A.rg[(size_t)chain * FBH_PMAX + tid];
+  __syncthreads();
+  if (!go) return;
+  const int nx = nx_s;
+  const uint8_t* lf = a.leaf + c.cd.leaf_off + (size_t)j * c.cd.ns;
+  const double* Y = a.Y + c.cd.row_off;
+  const float* Wt = a.W + c.cd.row_off;
+  const long long* AFX = A.AFX + c.cd.row_off;
+  const long long thfx = fit_fx((double)c.tree[0].th[nx]);
+  auto row = [&](int i, long long& wf, long long& wrf) -> bool {
+    if (lf[i] != nx) return false;
+    const double res = Y[i] - (double)(AFX[i] - thfx) * (1.0 / kFxFit);
+    const double w = (double)Wt[i];
+    wf = (long long)rint(w * kFxTgt);
+    wrf = (long long)rint(w * res * kFxTgt);
+    return true;
+  };
+  if (!split_conditional(N, S, row)) return;
+  if (wave == 0) {
+    const int it = A.st[chain].it;
+    const double u = mix_unif(A.tseed + 0x9E3779B97F4A7C15ull * (((unsigned long long)chain << 40) ^
+                                                                 ((unsigned long long)it * (unsigned)a.m + j) + 1));
+    const int cls = split_draw(N, S, u, lane);
+    if (lane == 0) {
+      const unsigned long long k = atomicAdd(A.pcount, 1ull);
+      slot = (long long)k < (long long)A.pcap ? (long long)k : -1;
+      if (slot >= 0) {
+        int allowed = 0;
+        for (int v = 0; v < N.p; v++) {
+          const int L = N.rg[v] & 255, U = N.rg[v] >> 8;
+          if (U >= L) allowed += U - L + 1;
+        }
+        A.ptg[slot] = cls;
+        A.pent[2 * slot] = (float)S.H;
+        A.pent[2 * slot + 1] = (float)log((double)allowed);
+      }
+    }
+  }
+  __syncthreads();
+  if (slot < 0) return;
+  const float* X = A.X + (size_t)chain * kFeat;
+  for (int q = tid; q < kFeat; q += kTgtBlock) A.px[(size_t)slot * kFeat + q] = X[q];
+  if (tid < FBH_PMAX) A.prg[(size_t)slot * FBH_PMAX + tid] = N.rg[tid];
+}
+
+// the split conditional of one node given on the host (the check of split_conditional / split_draw): q over the
+// classes, the entropy, and the class drawn at each of nu uniforms
+__global__ __launch_bounds__(kTgtBlock) void k_debug_split(SplitNode N0, const uint8_t* member, const double* w,
+                                                          const double* r, const double* u, int nu, float* q_out,
+                                                          int* cls_out, double* h_out) {
+  __shared__ SplitLds S;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  auto row = [&](int i, long long& wf, long long& wrf) -> bool {
+    if (!member[i]) return false;
+    wf = (long long)rint(w[i] * kFxTgt);
+    wrf = (long long)rint(w[i] * r[i] * kFxTgt);
+    return true;
+  };
+  const bool ok = split_conditional(N0, S, row);
+  if (tid == 0) h_out[0] = ok ? S.H : -1.0;
+  if (!ok) return;
+  for (int k = tid; k < N0.p * 256; k += kTgtBlock) {
+    const float l = (&S.lw[0][0])[k];
+    q_out[k] = l > -INFINITY ? (float)(exp((double)l - S.mx) / S.Z) : 0.f;
+  }
+  for (int t = wave; t < nu; t += kTgtBlock / 64) {
+    const int cls = split_draw(N0, S, u[t], lane);
+    if (lane == 0) cls_out[t] = cls;
+  }
 }
 
 // class k = v * 255 + c allowed at the node: v < p and L_v <= c <= U_v
@@ -2169,8 +2492,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
     if (it >= a.nskip && a.keep_trees)
       a.trees_out[c.cd.tout_off + ((size_t)(it - a.nskip) * m + j) * kNodes + lane] = pack_word(t, lane);
     copy_tree(&TR[j], &t, lane);
-    // an accepted birth's (features, ranges, class) becomes a training pair
-    if (acc_birth && A.pcap > 0) {
+    // an accepted birth's (features, ranges, class) becomes a training pair (with the conditional target, k_lp_target
+    // has taken every proposed birth's node already)
+    if (acc_birth && A.pcap > 0 && A.target == 0) {
       unsigned long long k = 0;
       if (lane == 0) k = atomicAdd(A.pcount, 1ull);
       k = __shfl(k, 0);
@@ -2384,7 +2708,10 @@ struct fbh_batch {
   int* d_ptg = nullptr;
   unsigned long long* d_pcount = nullptr;
   int* d_idx = nullptr;
+  float* d_pent = nullptr;
   long pcap = 0;
+  int target = 0;
+  unsigned long long tseed = 0;
   std::vector<hipEvent_t> lev;  // start/stop pairs per learned sweep
 };
 
@@ -2394,7 +2721,7 @@ static void free_batch(fbh_batch* b) {
   void* ptrs[] = {b->d_desc, b->d_state, b->d_trees, b->d_q,   b->d_Y,   b->d_W,   b->d_yin,    b->d_AF,
                   b->d_TM,   b->d_ybin,  b->d_leaf,  b->d_sigma, b->d_tout, b->d_PG, b->d_trm, b->d_trace,
                   b->d_img,  b->d_lps,   b->d_afx,   b->d_X,     b->d_rg,   b->d_px, b->d_prg, b->d_ptg,
-                  b->d_pcount, b->d_idx, b->d_xbuf, b->d_xctr, b->d_order};
+                  b->d_pcount, b->d_idx, b->d_xbuf, b->d_xctr, b->d_order, b->d_pent};
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto g : b->gexecs) (void)hipGraphExecDestroy(g);
   for (auto g : b->graphs) (void)hipGraphDestroy(g);
@@ -3410,6 +3737,9 @@ static LpArgs lp_args(const fbh_batch* b, int j) {
   A.ptg = b->d_ptg;
   A.pcount = b->d_pcount;
   A.pcap = b->pcap;
+  A.pent = b->d_pent;
+  A.tseed = b->tseed;
+  A.target = b->target;
   A.j = j;
   return A;
 }
@@ -3447,19 +3777,20 @@ extern "C" int fbh_batch_learned_setup(fbh_batch* b, fbh_mlp* net, float alpha, 
                                       hipFuncAttributeMaxDynamicSharedMemorySize, Geo<kRowsStreamL>::kScratchBytes));
   }
   if (pair_capacity != b->pcap) {
-    for (void* q : {(void*)b->d_px, (void*)b->d_prg, (void*)b->d_ptg, (void*)b->d_idx})
+    for (void* q : {(void*)b->d_px, (void*)b->d_prg, (void*)b->d_ptg, (void*)b->d_idx, (void*)b->d_pent})
       if (q) (void)hipFree(q);
     b->d_px = nullptr;
     b->d_prg = nullptr;
     b->d_ptg = nullptr;
     b->d_idx = nullptr;
+    b->d_pent = nullptr;
     b->pcap = 0;
     if (pair_capacity > 0) {
       int d[4], mb = 0;
       mlp_dims(net, d, &mb);
       if ((rc = dalloc(&b->d_px, (size_t)pair_capacity * kFeat)) ||
           (rc = dalloc(&b->d_prg, (size_t)pair_capacity * FBH_PMAX)) || (rc = dalloc(&b->d_ptg, pair_capacity)) ||
-          (rc = dalloc(&b->d_idx, mb)))
+          (rc = dalloc(&b->d_idx, mb)) || (rc = dalloc(&b->d_pent, (size_t)pair_capacity * 2)))
         return rc;
       b->pcap = pair_capacity;
     }
@@ -3488,6 +3819,8 @@ static int lp_run(fbh_batch* b, int nsweeps) {
     for (int j = 0; j < b->m; j++) {
       const LpArgs A = lp_args(b, j);
       hipLaunchKernelGGL(k_lp_stage1<FX>, dim3(b->C), dim3(kBlock), dyn, b->stream, A);
+      if (b->target == 1 && b->pcap > 0)
+        hipLaunchKernelGGL(k_lp_target, dim3(b->C), dim3(kTgtBlock), 0, b->stream, A);
       if (b->alpha > 0.f) {
         const int rc = mlp_forward_dev(b->net, b->d_X, b->C, b->stream);
         if (rc) return rc;
@@ -3517,6 +3850,139 @@ extern "C" int fbh_batch_run_learned(fbh_batch* b, int nsweeps) {
   nsweeps = std::min(nsweeps, b->nskip + b->ndpost - b->it);
   if (nsweeps <= 0) return FBH_OK;
   return b->stats_mode == FBH_STATS_FX64 ? lp_run<true>(b, nsweeps) : lp_run<false>(b, nsweeps);
+}
+
+extern "C" int fbh_batch_learned_target(fbh_batch* b, int kind, unsigned long long seed) {
+  if (!b || (kind != 0 && kind != 1)) {
+    set_error(FBH_EINVAL, "fbh_batch_learned_target: kind must be 0 (accepted births, BART's rule) or 1 (every "
+                          "proposed birth, a rule drawn from the exact split conditional)");
+    return FBH_EINVAL;
+  }
+  b->target = kind;
+  b->tseed = seed;
+  return FBH_OK;
+}
+
+extern "C" int fbh_batch_learned_target_stats(fbh_batch* b, double* mean_entropy, double* mean_log_allowed) {
+  if (!b || !mean_entropy || !mean_log_allowed) {
+    set_error(FBH_EINVAL, "fbh_batch_learned_target_stats: bad argument");
+    return FBH_EINVAL;
+  }
+  *mean_entropy = *mean_log_allowed = std::nan("");
+  const long np = fbh_batch_learned_pairs(b);
+  if (np < 0) {
+    set_error(FBH_EHIP, "fbh_batch_learned_target_stats: reading the pair count failed");
+    return FBH_EHIP;
+  }
+  if (b->target != 1 || np == 0) return FBH_OK;
+  std::vector<float> h((size_t)np * 2);
+  FBH_HIP_CHECK(hipMemcpy(h.data(), b->d_pent, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+  double e = 0, l = 0;
+  for (long k = 0; k < np; k++) {
+    e += h[2 * k];
+    l += h[2 * k + 1];
+  }
+  *mean_entropy = e / np;
+  *mean_log_allowed = l / np;
+  return FBH_OK;
+}
+
+extern "C" int fbh_batch_learned_pairs_copy(fbh_batch* b, long k0, long count, float* features, uint16_t* ranges,
+                                            int* cls, float* entropy) {
+  const long np = fbh_batch_learned_pairs(b);
+  if (!b || np < 0 || k0 < 0 || count < 0 || k0 + count > np) {
+    set_error(FBH_EINVAL, "fbh_batch_learned_pairs_copy: range outside the collected pairs");
+    return FBH_EINVAL;
+  }
+  if (count == 0) return FBH_OK;
+  if (features)
+    FBH_HIP_CHECK(hipMemcpy(features, b->d_px + (size_t)k0 * kFeat, sizeof(float) * kFeat * count,
+                            hipMemcpyDeviceToHost));
+  if (ranges)
+    FBH_HIP_CHECK(hipMemcpy(ranges, b->d_prg + (size_t)k0 * FBH_PMAX, sizeof(uint16_t) * FBH_PMAX * count,
+                            hipMemcpyDeviceToHost));
+  if (cls) FBH_HIP_CHECK(hipMemcpy(cls, b->d_ptg + k0, sizeof(int) * count, hipMemcpyDeviceToHost));
+  if (entropy) {
+    if (b->target == 1) {
+      FBH_HIP_CHECK(hipMemcpy(entropy, b->d_pent + 2 * k0, sizeof(float) * 2 * count, hipMemcpyDeviceToHost));
+    } else {
+      for (long k = 0; k < 2 * count; k++) entropy[k] = std::nanf("");
+    }
+  }
+  return FBH_OK;
+}
+
+extern "C" int fbh_debug_split_conditional(int n, int p, const uint8_t* q, const uint8_t* member, const double* w,
+                                           const double* r, const uint16_t* ranges, int depth, int ngv, int ngood,
+                                           const double* pg, double sigma, double tau, const double* u, int nu,
+                                           float* q_out, int* cls_out, double* entropy) {
+  if (n < 1 || p < 1 || p > FBH_PMAX || !q || !member || !w || !r || !ranges || depth < 0 || depth + 1 >= kNodes ||
+      !pg || !(sigma > 0) || !(tau > 0) || nu < 0 || (nu > 0 && (!u || !cls_out)) || !q_out || !entropy) {
+    set_error(FBH_EINVAL, "fbh_debug_split_conditional: bad argument");
+    return FBH_EINVAL;
+  }
+  for (int v = 0; v < p; v++) {
+    const int L = ranges[v] & 255, U = ranges[v] >> 8;
+    if (U >= L && U > 254) {
+      set_error(FBH_EINVAL, "fbh_debug_split_conditional: cut above 254");
+      return FBH_EINVAL;
+    }
+  }
+  int rc = ensure_device(0);
+  if (rc) return rc;
+  uint8_t *dq = nullptr, *dm = nullptr;
+  double *dw = nullptr, *dr = nullptr, *du = nullptr, *dpg = nullptr, *dh = nullptr;
+  float* dqo = nullptr;
+  int* dc = nullptr;
+  auto cleanup = [&]() {
+    for (void* x : {(void*)dq, (void*)dm, (void*)dw, (void*)dr, (void*)du, (void*)dpg, (void*)dh, (void*)dqo,
+                    (void*)dc})
+      if (x) (void)hipFree(x);
+  };
+  if ((rc = dalloc(&dq, (size_t)n * p)) || (rc = dalloc(&dm, n)) || (rc = dalloc(&dw, n)) || (rc = dalloc(&dr, n)) ||
+      (rc = dalloc(&du, std::max(nu, 1))) || (rc = dalloc(&dpg, kNodes)) || (rc = dalloc(&dh, 1)) ||
+      (rc = dalloc(&dqo, (size_t)p * 256)) || (rc = dalloc(&dc, std::max(nu, 1)))) {
+    cleanup();
+    return rc;
+  }
+  hipError_t e = hipSuccess;
+  auto up = [&](void* d, const void* h, size_t bytes) {
+    if (e == hipSuccess && bytes) e = hipMemcpy(d, h, bytes, hipMemcpyHostToDevice);
+  };
+  up(dq, q, (size_t)n * p);
+  up(dm, member, n);
+  up(dw, w, sizeof(double) * n);
+  up(dr, r, sizeof(double) * n);
+  up(dpg, pg, sizeof(double) * kNodes);
+  if (nu) up(du, u, sizeof(double) * nu);
+  SplitNode N{};
+  N.n = n;
+  N.p = p;
+  N.ns = n;
+  N.d = depth;
+  N.ngv = ngv;
+  N.ngood = ngood;
+  N.inv_s2 = 1.0 / (sigma * sigma);
+  N.t2 = tau * tau;
+  N.PG = dpg;
+  N.Q = dq;
+  for (int v = 0; v < FBH_PMAX; v++) N.rg[v] = v < p ? ranges[v] : kNoRange;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_debug_split, dim3(1), dim3(kTgtBlock), 0, nullptr, N, dm, dw, dr, du, nu, dqo, dc, dh);
+    e = hipGetLastError();
+  }
+  std::vector<float> qq((size_t)p * 256);
+  if (e == hipSuccess) e = hipMemcpy(qq.data(), dqo, sizeof(float) * qq.size(), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(entropy, dh, sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && nu) e = hipMemcpy(cls_out, dc, sizeof(int) * nu, hipMemcpyDeviceToHost);
+  cleanup();
+  if (e != hipSuccess) {
+    set_error(FBH_EHIP, std::string("fbh_debug_split_conditional: ") + hipGetErrorString(e));
+    return FBH_EHIP;
+  }
+  for (int v = 0; v < p; v++)
+    for (int k = 0; k < kCuts; k++) q_out[(size_t)v * kCuts + k] = qq[(size_t)v * 256 + k];
+  return FBH_OK;
 }
 
 extern "C" long fbh_batch_learned_pairs(fbh_batch* b) {

@@ -156,6 +156,31 @@ class BatchSampler:
     def run_learned(self, nsweeps):
         check(lib().fbh_batch_run_learned(self.h, int(nsweeps)), "fbh_batch_run_learned")
 
+    def learned_target(self, kind, seed=1):
+        """0: accepted births with BART's rule; 1: every proposed birth's node, labelled by a draw from the exact
+        split conditional there (set before collecting)."""
+        check(lib().fbh_batch_learned_target(self.h, int(kind), int(seed) & 0xFFFFFFFFFFFFFFFF),
+              "fbh_batch_learned_target")
+
+    def learned_target_stats(self):
+        """(mean entropy of the pairs' split conditionals, mean log of their allowed class counts), nats"""
+        e, l = C.c_double(), C.c_double()
+        check(lib().fbh_batch_learned_target_stats(self.h, C.byref(e), C.byref(l)), "fbh_batch_learned_target_stats")
+        return e.value, l.value
+
+    def learned_pairs_copy(self, k0=0, count=None):
+        """Collected pairs [k0, k0 + count) on the host: dict of features [count][513], ranges [count][32],
+        cls [count], entropy [count][2]."""
+        if count is None:
+            count = self.learned_pairs() - k0
+        x = np.zeros((count, 513), np.float32)
+        rg = np.zeros((count, 32), np.uint16)
+        cls = np.zeros(count, np.int32)
+        ent = np.zeros((count, 2), np.float32)
+        check(lib().fbh_batch_learned_pairs_copy(self.h, int(k0), int(count), ptr(x), ptr(rg), ptr(cls), ptr(ent)),
+              "fbh_batch_learned_pairs_copy")
+        return dict(features=x, ranges=rg, cls=cls, entropy=ent)
+
     def learned_pairs(self):
         return int(lib().fbh_batch_learned_pairs(self.h))
 
@@ -269,3 +294,23 @@ def debug_copy(mib, device=0):
     ms = C.c_double()
     check(lib().fbh_debug_copy(int(mib), device, C.byref(ms)), "fbh_debug_copy")
     return ms.value
+
+
+def debug_split_conditional(q, member, w, r, ranges, depth, ngv, ngood, pg, sigma, tau, u=None):
+    """The device's exact split conditional at one node (the target of fbh_batch_learned_target kind 1):
+    (q [p][255], entropy, classes drawn at the uniforms u)."""
+    q = np.ascontiguousarray(q, np.uint8)
+    p, n = q.shape
+    member = np.ascontiguousarray(member, np.uint8)
+    w = np.ascontiguousarray(w, np.float64)
+    r = np.ascontiguousarray(r, np.float64)
+    ranges = np.ascontiguousarray(ranges, np.uint16)
+    pg = np.ascontiguousarray(pg, np.float64)
+    u = np.ascontiguousarray(u if u is not None else [], np.float64)
+    out = np.zeros((p, 255), np.float32)
+    cls = np.zeros(max(len(u), 1), np.int32)
+    h = C.c_double()
+    check(lib().fbh_debug_split_conditional(n, p, ptr(q), ptr(member), ptr(w), ptr(r), ptr(ranges), int(depth),
+                                            int(ngv), int(ngood), ptr(pg), float(sigma), float(tau), ptr(u), len(u),
+                                            ptr(out), ptr(cls), C.byref(h)), "fbh_debug_split_conditional")
+    return out, h.value, cls[:len(u)]

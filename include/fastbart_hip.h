@@ -327,6 +327,19 @@ int fbh_batch_learned_setup(fbh_batch* b, fbh_mlp* net, float alpha, long pair_c
 int fbh_batch_run_learned(fbh_batch* b, int nsweeps);
 /* training pairs collected so far (at most the capacity) */
 long fbh_batch_learned_pairs(fbh_batch* b);
+/* what a collected pair is: kind 0 (the default) = each accepted birth with BART's rule; kind 1 = the node of each
+ * proposed birth labelled by a rule drawn from the exact split conditional there (BART's rule probability times the
+ * rule-dependent factors of the birth ratio, over the rules whose children hold at least 5 rows), drawn from a stream
+ * seeded by seed apart from the chains' generators, so the chains are unchanged. Set before collecting. */
+int fbh_batch_learned_target(fbh_batch* b, int kind, unsigned long long seed);
+/* kind 1 only (else NaN): the mean over the collected pairs of the conditional's entropy (the cross-entropy floor a
+ * network can reach) and of the log of the allowed class count (the uniform rule's cross-entropy), in nats */
+int fbh_batch_learned_target_stats(fbh_batch* b, double* mean_entropy, double* mean_log_allowed);
+/* copy pairs [k0, k0 + count) to the host (any pointer may be null): features [count][513], ranges [count][32]
+ * (L | U << 8), cls [count] (v * 255 + c), entropy [count][2] (kind 1: the conditional's entropy and the log of the
+ * allowed class count; kind 0: NaN) */
+int fbh_batch_learned_pairs_copy(fbh_batch* b, long k0, long count, float* features, uint16_t* ranges, int* cls,
+                                 float* entropy);
 /* steps Adam steps on minibatches of the collected pairs (device-resident), loss_out [steps] may be null */
 int fbh_batch_learned_train(fbh_batch* b, int steps, int batch, float lr, unsigned seed, float* loss_out);
 /* average duration (ms) of the learned sweeps so far, and their count */
@@ -349,6 +362,16 @@ int fbh_batch_trace(fbh_batch* b, int cap, void* host_out, int chain);
 /* one launch of a plain copy kernel over two fresh device buffers of mib MiB (1..16384); its duration in ms goes to
  * *ms. Used to calibrate hardware byte counters (FETCH_SIZE / WRITE_SIZE) against a transfer of known size. */
 int fbh_debug_copy(long mib, int device, double* ms);
+/* the exact split conditional of fbh_batch_learned_target kind 1 at one node given on the host (the device function
+ * the pairs use): n rows with cutpoint ranks q [p][n], membership member[n] (1 = the row is in the node), weights w and
+ * partial residuals r; ranges [p] = L | U << 8 of the node's allowed cuts (U < L: none); the node's depth and
+ * good-variable count, the tree's good-leaf count ngood, the depth prior table pg [64], sigma and tau. q_out [p][255]
+ * = q(v, c); *entropy = its entropy (-1 when no rule qualifies); cls_out[t] = the class v * 255 + c at the inverse CDF
+ * of u[t] (classes in that order). No reference counterpart. */
+int fbh_debug_split_conditional(int n, int p, const uint8_t* q, const uint8_t* member, const double* w,
+                                const double* r, const uint16_t* ranges, int depth, int ngv, int ngood,
+                                const double* pg, double sigma, double tau, const double* u, int nu, float* q_out,
+                                int* cls_out, double* entropy);
 /* library build information */
 const char* fbh_version(void);
 

@@ -462,45 +462,40 @@ typedef struct orule {
   double lw;
 } orule;
 
-/* the conditional at node nx over the rows whose leaf in tree j is nx (or l or r: the merged node of a death);
- * draw = 1: draw a rule into *v, *c and return its probability (0 if no rule qualifies); draw = 0: return the
- * probability of rule (*v, *c) */
-static double informed_rule(orc_chain* ch, const otree* t, int j, int nx, int l, int r, const double* th, double inv_s2,
-                            int ngood_pre, int draw, int* v, int* c) {
+/* The rules of the conditional and their log weights at one node: rows with member[i] != 0, weights w, partial
+ * residuals r (q [p][n] cutpoint ranks; ranges[u] = L | U << 8, U < L: no cut). Sums are plain sequential doubles in
+ * row order. Returns the number of rules written to rules[] (variables in order, cuts ascending) and their largest log
+ * weight in *lmax. */
+static int split_rules(int n, int p, const uint8_t* q, const uint8_t* member, const double* w, const double* r,
+                       const uint16_t* ranges, int d, int ngv, int ngood_pre, const double* PG, double inv_s2,
+                       double t2, orule* rules, double* lmax_out) {
   static __thread double H0[ORC_PMAX][256], H1[ORC_PMAX][256];
   static __thread int Hn[ORC_PMAX][256];
-  static __thread orule rules[ORC_PMAX * 256];
-  const int n = ch->n, p = ch->p;
-  const double pb = 0.5, t2 = ch->t2;
-  if ((int)t->d[nx] + 1 >= ORC_MAXDEPTH) return 0;
+  const double pb = 0.5;
   for (int u = 0; u < p; u++) {
-    memset(H0[u], 0, sizeof(double) * (ch->nc[u] + 1));
-    memset(H1[u], 0, sizeof(double) * (ch->nc[u] + 1));
-    memset(Hn[u], 0, sizeof(int) * (ch->nc[u] + 1));
+    memset(H0[u], 0, sizeof H0[u]);
+    memset(H1[u], 0, sizeof H1[u]);
+    memset(Hn[u], 0, sizeof Hn[u]);
   }
-  const uint8_t* lf = ch->leaf + (size_t)j * n;
   double B0 = 0, B1 = 0;
   int N = 0;
   for (int i = 0; i < n; i++) {
-    const int s = lf[i];
-    if (l < 0 ? s != nx : (s != l && s != r)) continue;
-    const double w = ch->W[i], wr = w * (ch->Y[i] - ch->AF[i] + th[s]);
-    B0 += w;
+    if (!member[i]) continue;
+    const double wi = w[i], wr = wi * r[i];
+    B0 += wi;
     B1 += wr;
     N++;
     for (int u = 0; u < p; u++) {
-      const int k = ch->q[(size_t)u * n + i];
-      H0[u][k] += w;
+      const int k = q[(size_t)u * n + i];
+      H0[u][k] += wi;
       H1[u][k] += wr;
       Hn[u][k]++;
     }
   }
-  const int ngv = count_goodvars(t, nx, ch->nc, p), d = t->d[nx];
   int nr = 0;
   double lmax = -INFINITY;
   for (int u = 0; u < p; u++) {
-    int L, U;
-    node_range(t, nx, u, ch->nc[u], &L, &U);
+    const int L = ranges[u] & 255, U = ranges[u] >> 8;
     if (U < L) continue;
     double c0 = 0, c1 = 0;
     int cn = 0;
@@ -511,10 +506,10 @@ static double informed_rule(orc_chain* ch, const otree* t, int j, int nx, int l,
       if (k < L || cn < 5 || N - cn < 5) continue;
       double PGl, PGr;
       if (ngv > 1) {
-        PGl = PGr = ch->PG[d + 1];
+        PGl = PGr = PG[d + 1];
       } else {
-        PGl = (k - 1 < L) ? 0 : ch->PG[d + 1];
-        PGr = (U < k + 1) ? 0 : ch->PG[d + 1];
+        PGl = (k - 1 < L) ? 0 : PG[d + 1];
+        PGr = (U < k + 1) ? 0 : PG[d + 1];
       }
       const double PD = (ngood_pre > 1) ? 1 - pb : ((PGr == 0 && PGl == 0) ? 1 : 1 - pb);
       const double lw = log((1 - PGl) * (1 - PGr) * PD / (U - L + 1)) + lh(c0 * inv_s2, c1 * inv_s2, t2) +
@@ -526,6 +521,65 @@ static double informed_rule(orc_chain* ch, const otree* t, int j, int nx, int l,
       if (lw > lmax) lmax = lw;
     }
   }
+  *lmax_out = lmax;
+  return nr;
+}
+
+int orc_split_conditional(int n, int p, const uint8_t* q, const uint8_t* member, const double* w, const double* r,
+                          const uint16_t* ranges, int depth, int ngv, int ngood, const double* pg, double sigma,
+                          double tau, double* q_out, double* entropy) {
+  static __thread orule rules[ORC_PMAX * 256];
+  if (n < 1 || p < 1 || p > ORC_PMAX || depth < 0 || depth + 1 >= ORC_NODES) return -1;
+  double lmax;
+  const int nr = split_rules(n, p, q, member, w, r, ranges, depth, ngv, ngood, pg, 1.0 / (sigma * sigma), tau * tau,
+                             rules, &lmax);
+  memset(q_out, 0, sizeof(double) * (size_t)p * 255);
+  *entropy = -1;
+  if (nr == 0) return 0;
+  double Z = 0, E = 0;
+  for (int k = 0; k < nr; k++) {
+    const double x = rules[k].lw - lmax, e = exp(x);
+    Z += e;
+    E += e * x;
+  }
+  for (int k = 0; k < nr; k++) q_out[(size_t)rules[k].v * 255 + rules[k].c] = exp(rules[k].lw - lmax) / Z;
+  *entropy = log(Z) - E / Z;
+  return nr;
+}
+
+/* the conditional at node nx over the rows whose leaf in tree j is nx (or l or r: the merged node of a death);
+ * draw = 1: draw a rule into *v, *c and return its probability (0 if no rule qualifies); draw = 0: return the
+ * probability of rule (*v, *c) */
+static double informed_rule(orc_chain* ch, const otree* t, int j, int nx, int l, int r, const double* th, double inv_s2,
+                            int ngood_pre, int draw, int* v, int* c) {
+  static __thread orule rules[ORC_PMAX * 256];
+  static __thread uint8_t* mem = NULL;
+  static __thread double* res = NULL;
+  static __thread int cap = 0;
+  const int n = ch->n, p = ch->p;
+  if ((int)t->d[nx] + 1 >= ORC_MAXDEPTH) return 0;
+  if (cap < n) {
+    free(mem);
+    free(res);
+    mem = (uint8_t*)malloc((size_t)n);
+    res = (double*)malloc(sizeof(double) * (size_t)n);
+    cap = n;
+  }
+  const uint8_t* lf = ch->leaf + (size_t)j * n;
+  for (int i = 0; i < n; i++) {
+    const int s = lf[i];
+    mem[i] = (uint8_t)(l < 0 ? s == nx : (s == l || s == r));
+    res[i] = mem[i] ? ch->Y[i] - ch->AF[i] + th[s] : 0;
+  }
+  uint16_t ranges[ORC_PMAX];
+  for (int u = 0; u < p; u++) {
+    int L, U;
+    node_range(t, nx, u, ch->nc[u], &L, &U);
+    ranges[u] = U < L ? (uint16_t)0x00FF : (uint16_t)(L | (U << 8));
+  }
+  double lmax;
+  const int nr = split_rules(n, p, ch->q, mem, ch->W, res, ranges, t->d[nx], count_goodvars(t, nx, ch->nc, p),
+                             ngood_pre, ch->PG, inv_s2, ch->t2, rules, &lmax);
   if (nr == 0) return 0;
   double Z = 0;
   for (int k = 0; k < nr; k++) Z += exp(rules[k].lw - lmax);

@@ -88,6 +88,13 @@ long orc_chain_trace_len(const orc_chain* ch);
  * and deaths; 0 restores the reference's proposal */
 void orc_chain_set_informed(orc_chain* ch, double alpha);
 long orc_chain_informed_births(const orc_chain* ch);
+/* that conditional at one node given directly (no reference counterpart): rows with member[i] != 0, weights w,
+ * partial residuals r, cutpoint ranks q [p][n]; ranges[v] = L | U << 8 (U < L: no cut); the node's depth and
+ * good-variable count, the tree's good-leaf count before the move, the depth prior pg [64], sigma, tau.
+ * q_out [p][255] = q(v, c), *entropy = its entropy (-1 if no rule qualifies); returns the number of rules */
+int orc_split_conditional(int n, int p, const uint8_t* q, const uint8_t* member, const double* w, const double* r,
+                          const uint16_t* ranges, int depth, int ngv, int ngood, const double* pg, double sigma,
+                          double tau, double* q_out, double* entropy);
 
 /* whole fit, same contract as bart_fit_cpu with the internal generator; returns 0 or a negative error */
 int orc_fit(const uint8_t* q, const float* y, const float* wt, const int* numcut, const float* pgtab,

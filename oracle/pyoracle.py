@@ -127,6 +127,10 @@ class Oracle:
         L.orc_chain_set_informed.restype = None
         L.orc_chain_informed_births.argtypes = [C.c_void_p]
         L.orc_chain_informed_births.restype = C.c_long
+        L.orc_split_conditional.argtypes = [C.c_int, C.c_int, _u8p, _u8p, _f64p, _f64p,
+                                            np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS"), C.c_int,
+                                            C.c_int, C.c_int, _f64p, C.c_double, C.c_double, _f64p,
+                                            C.POINTER(C.c_double)]
         L.orc_tree_text.argtypes = [_u64p, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_size_t, C.c_void_p]
         L.orc_tree_text.restype = C.c_size_t
         L.orc_yhat.argtypes = [_u64p, _u8p, C.c_int, C.c_int, C.c_int, _f64p]
@@ -201,6 +205,22 @@ class Oracle:
         return OracleChain(self, prob, m, nskip, ndpost, seed, probit, stats_mode, node_cap, trace_cap)
 
     # -- glue
+    def split_conditional(self, q, member, w, r, ranges, depth, ngv, ngood, pg, sigma, tau):
+        """The informed birth rule's conditional at one node (no reference counterpart): (q [p][255], entropy,
+        number of rules)."""
+        q = np.ascontiguousarray(q, np.uint8)
+        p, n = q.shape
+        out = np.zeros((p, 255), np.float64)
+        h = C.c_double()
+        nr = self.L.orc_split_conditional(n, p, q, np.ascontiguousarray(member, np.uint8),
+                                          np.ascontiguousarray(w, np.float64), np.ascontiguousarray(r, np.float64),
+                                          np.ascontiguousarray(ranges, np.uint16), int(depth), int(ngv), int(ngood),
+                                          np.ascontiguousarray(pg, np.float64), float(sigma), float(tau), out,
+                                          C.byref(h))
+        if nr < 0:
+            raise ValueError("orc_split_conditional: bad argument")
+        return out, h.value, nr
+
     def tree_text(self, trees, p):
         trees = np.ascontiguousarray(trees, dtype=np.uint64)
         nd, m = trees.shape[0], trees.shape[1]

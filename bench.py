@@ -263,6 +263,7 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
     t_plain = time.perf_counter() - t0
     ess_plain, mean_plain, dist_plain = ess_of(b)
     births_plain = int(sum(int(b.fetch(c)["stats"][0]) for c in range(chains)))
+    sigma_plain = float(np.mean([b.fetch(c)["sigma"][nskip:].astype(np.float64).mean() for c in range(chains)]))
     b.close()
     log(f"plain: {total} sweeps {t_plain:.2f} s, sum min-ESS {ess_plain:.1f}")
     # learned schedule
@@ -272,15 +273,21 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
     t0 = time.perf_counter()
     b.run(n_plain0)
     b.learned_setup(net, 0.0, pair_capacity=args.pairs)
-    b.learned_target(1 if args.target == "conditional" else 0, seed=11)
+    cond = args.target == "conditional"
+    b.learned_target(1 if cond else 0, fraction=args.pair_fraction if cond else 1.0, seed=11)
     b.run_learned(args.collect)
     b.sync()
     t_a = time.perf_counter() - t0
     npairs = b.learned_pairs()
     ent, lall = b.learned_target_stats()
+    # the last tenth of the pairs (the latest sweeps) stays out of training: the loss there is out of sample
+    hold = npairs // 10
+    b.learned_holdout(hold)
+    held0 = b.learned_eval(npairs - hold, hold)[0] if hold else None
     t1 = time.perf_counter()
     losses = b.learned_train(args.train_steps, batch=512, lr=args.lr, seed=3)
     t_train = time.perf_counter() - t1
+    held = b.learned_eval(npairs - hold, hold) if hold else (None, None, None)
     b.learned_setup(net, args.alpha, pair_capacity=0)
     t2 = time.perf_counter()
     b.run_learned(total - n_plain0 - args.collect)
@@ -290,12 +297,14 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
     ms_sweep, nls = b.learned_timing()
     ess_learned, mean_learned, dist_learned = ess_of(b)
     st = np.array([b.fetch(c)["stats"] for c in range(chains)])
+    sigma_learned = float(np.mean([b.fetch(c)["sigma"][nskip:].astype(np.float64).mean() for c in range(chains)]))
     b.close()
     fwd_ms, _ = net.timing() if hasattr(net, "timing") else (0.0, 0)
     corr = float(np.mean([np.corrcoef(a, bb)[0, 1] for a, bb in zip(mean_plain, mean_learned)]))
     loss0, loss1 = float(losses[:20].mean()), float(losses[-20:].mean())
-    log(f"learned: {t_learned:.2f} s sampling + {t_train:.2f} s training on {npairs} pairs "
-        f"(loss {loss0:.3f} -> {loss1:.3f}; target entropy {ent:.3f}, uniform rule {lall:.3f} nats); "
+    log(f"learned: {t_learned:.2f} s sampling + {t_train:.2f} s training on {npairs - hold} pairs "
+        f"(training loss {loss0:.3f} -> {loss1:.3f}; held-out loss {held0} -> {held[0]}, "
+        f"its floor {held[1]} and uniform level {held[2]} nats); "
         f"sum min-ESS {ess_learned:.1f}; "
         f"posterior-mean correlation with plain {corr:.4f}")
     value = chains * M * total / t_learned
@@ -316,9 +325,15 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
                     "sum_min_ess": ess_learned, **dist_learned, "pairs": npairs, "ms_per_learned_sweep": ms_sweep,
                     "learned_sweeps": nls, "births": int(st[:, 0].sum()), "deaths": int(st[:, 1].sum()),
                     "target": args.target, "loss_first20": loss0, "loss_last20": loss1,
+                    "heldout_pairs": hold, "heldout_loss_untrained": held0, "heldout_loss": held[0],
+                    "heldout_entropy_floor": held[1] if held[1] is not None and np.isfinite(held[1]) else None,
+                    "heldout_uniform_rule": held[2] if held[2] is not None and np.isfinite(held[2]) else None,
                     "target_entropy_nats": ent if np.isfinite(ent) else None,
                     "uniform_rule_nats": lall if np.isfinite(lall) else None},
         "plain_births": births_plain,
+        # the chains' sigma still falls for tens of thousands of sweeps (burn-in): over the kept draws a lower mean
+        # is a chain further converged
+        "sigma_mean_kept": {"plain": sigma_plain, "learned": sigma_learned},
         "ess_per_sec_ratio_learned_over_plain": ess_ps / ess_pp if ess_pp else None,
         "posterior_mean_corr": corr,
         "roofline": None,
@@ -571,8 +586,10 @@ def main() -> int:
     ap.add_argument("--ndpost", type=int, default=None, help="kept sweeps (learned512; longer runs steady ESS)")
     ap.add_argument("--collect", type=int, default=100, help="learned512: sweeps collecting births")
     ap.add_argument("--learned-burn", type=int, default=100, help="learned512: burn-in sweeps at the mixture")
-    ap.add_argument("--train-steps", type=int, default=2000)
-    ap.add_argument("--pairs", type=int, default=2000000, help="learned512: training-pair capacity")
+    ap.add_argument("--train-steps", type=int, default=4000)
+    ap.add_argument("--pairs", type=int, default=600000, help="learned512: training-pair capacity")
+    ap.add_argument("--pair-fraction", type=float, default=0.12,
+                    help="learned512, conditional target: share of the proposed births taken as pairs")
     ap.add_argument("--alpha", type=float, default=0.8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--target", choices=("conditional", "accepted"), default="conditional",

@@ -90,7 +90,7 @@ EXPORTS = ["bart_fit_cpu", "bart_wfit", "bart_set_rng", "fbh_last_status", "fbh_
            "fbh_mlp_timing", "fbh_mlp_destroy", "fbh_batch_learned_setup", "fbh_batch_run_learned",
            "fbh_batch_learned_pairs", "fbh_batch_learned_train", "fbh_batch_learned_timing",
            "fbh_batch_learned_target", "fbh_batch_learned_target_stats", "fbh_debug_split_conditional",
-           "fbh_batch_learned_pairs_copy",
+           "fbh_batch_learned_pairs_copy", "fbh_batch_learned_holdout", "fbh_batch_learned_eval",
            "fbh_prep_defaults",
            "fbh_wbart_inputs", "fbh_pbart_inputs", "fbh_inputs_free", "fbh_qnorm", "fbh_qchisq"]
 
@@ -170,7 +170,9 @@ def _declare(L):
     L.fbh_batch_learned_pairs.restype = C.c_long
     L.fbh_batch_learned_train.argtypes = [vp, i, i, C.c_float, C.c_uint, vp]
     L.fbh_batch_learned_timing.argtypes = [vp, C.POINTER(d), C.POINTER(i)]
-    L.fbh_batch_learned_target.argtypes = [vp, i, C.c_ulonglong]
+    L.fbh_batch_learned_target.argtypes = [vp, i, C.c_float, C.c_ulonglong]
+    L.fbh_batch_learned_holdout.argtypes = [vp, C.c_long]
+    L.fbh_batch_learned_eval.argtypes = [vp, C.c_long, C.c_long, C.POINTER(d), C.POINTER(d), C.POINTER(d)]
     L.fbh_batch_learned_pairs_copy.argtypes = [vp, C.c_long, C.c_long, vp, vp, vp, vp]
     L.fbh_batch_learned_target_stats.argtypes = [vp, C.POINTER(d), C.POINTER(d)]
     L.fbh_debug_split_conditional.argtypes = [i, i, vp, vp, vp, vp, vp, i, i, i, vp, d, d, vp, i, vp, vp,

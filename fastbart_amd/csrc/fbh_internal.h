@@ -88,7 +88,9 @@ int ensure_device(int device);
 
 // the proposal network's device-level steps (fbh_mlp.hip), on a caller's stream
 int mlp_forward_dev(fbh_mlp* n, const float* xin_dev, int M, hipStream_t s);
-int mlp_backward_dev(fbh_mlp* n, int M, hipStream_t s);
+int mlp_backward_dev(fbh_mlp* n, int M, hipStream_t s, bool soft = false);
+int mlp_loss_dev(fbh_mlp* n, int M, bool soft, hipStream_t s);
+float* mlp_soft_buffer(fbh_mlp* n);
 int mlp_adam_dev(fbh_mlp* n, float lr, hipStream_t s);
 const float* mlp_logits(const fbh_mlp* n, int* ld);
 void mlp_dims(const fbh_mlp* n, int* d, int* max_batch);

@@ -98,10 +98,13 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16* __restrict__ A, const 
 }
 
 // Masked softmax cross-entropy of one row per block: loss[m] = -log softmax(z_valid)[target]; dZ = (softmax - onehot) *
-// scale over the valid entries, 0 elsewhere (rows >= M are all zero).
+// scale over the valid entries, 0 elsewhere (rows >= M are all zero). SOFT: the target is a distribution q [M][d_out]
+// over the valid entries instead of a class: loss[m] = sum_n q_n (lse - z_n), dZ = (sum(q) softmax - q) * scale.
+template <bool SOFT>
 __global__ __launch_bounds__(256) void k_softmax_xent(const float* __restrict__ Z, int ldz, int d_out, int M,
                                                       const int* __restrict__ target, const uint8_t* __restrict__ mask,
-                                                      float scale, float* __restrict__ dZ, bf16* __restrict__ dZb,
+                                                      const float* __restrict__ soft, float scale,
+                                                      float* __restrict__ dZ, bf16* __restrict__ dZb,
                                                       float* __restrict__ loss) {
   __shared__ float red[256];
   const int m = blockIdx.x, tid = threadIdx.x;
@@ -137,15 +140,48 @@ __global__ __launch_bounds__(256) void k_softmax_xent(const float* __restrict__ 
     __syncthreads();
   }
   se = red[0];
-  const int t = target[m];
   const float lse = mx + logf(se);
-  for (int n = tid; n < ldz; n += 256) {
-    float g = 0.f;
-    if (n < d_out && (!mk || mk[n])) g = (expf(z[n] - lse) - (n == t ? 1.f : 0.f)) * scale;
-    dz[n] = g;
-    dzb[n] = (bf16)g;
+  if constexpr (SOFT) {
+    const float* q = soft + (size_t)m * d_out;
+    __syncthreads();
+    float sq = 0.f, sl = 0.f;
+    for (int n = tid; n < d_out; n += 256)
+      if ((!mk || mk[n]) && q[n] > 0.f) {
+        sq += q[n];
+        sl += q[n] * (lse - z[n]);
+      }
+    red[tid] = sq;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) red[tid] += red[tid + s];
+      __syncthreads();
+    }
+    sq = red[0];
+    __syncthreads();
+    red[tid] = sl;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) red[tid] += red[tid + s];
+      __syncthreads();
+    }
+    sl = red[0];
+    for (int n = tid; n < ldz; n += 256) {
+      float g = 0.f;
+      if (n < d_out && (!mk || mk[n])) g = (sq * expf(z[n] - lse) - q[n]) * scale;
+      dz[n] = g;
+      dzb[n] = (bf16)g;
+    }
+    if (tid == 0) loss[m] = sl;
+  } else {
+    const int t = target[m];
+    for (int n = tid; n < ldz; n += 256) {
+      float g = 0.f;
+      if (n < d_out && (!mk || mk[n])) g = (expf(z[n] - lse) - (n == t ? 1.f : 0.f)) * scale;
+      dz[n] = g;
+      dzb[n] = (bf16)g;
+    }
+    if (tid == 0) loss[m] = lse - z[t];
   }
-  if (tid == 0) loss[m] = lse - z[t];
 }
 
 // out[n] = sum over rows m < M of X[m][n] (bias gradients), one column per thread
@@ -222,6 +258,7 @@ struct fbh_mlp {
   float *Z = nullptr, *dZ = nullptr, *dH2 = nullptr, *dH1 = nullptr, *loss = nullptr, *xin = nullptr;
   int* tgt = nullptr;
   uint8_t* mask = nullptr;
+  float* soft = nullptr;  // soft targets [maxM][d_out] (allocated on first use)
   int adam_t = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double fwd_ms = 0;
@@ -237,7 +274,7 @@ static void mlp_free(fbh_mlp* n) {
       if (q) (void)hipFree(q);
   }
   void* ps[] = {n->Xb, n->H1, n->H2, n->dZb, n->dH2b, n->dH1b, n->Z, n->dZ, n->dH2, n->dH1, n->loss, n->xin,
-                n->tgt, n->mask};
+                n->tgt, n->mask, n->soft};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (n->e0) (void)hipEventDestroy(n->e0);
@@ -395,13 +432,38 @@ int mlp_forward_dev(fbh_mlp* n, const float* xin_dev, int M, hipStream_t s) {
   return FBH_OK;
 }
 
-// masked cross-entropy and the gradients of the mean loss over rows [0, M) (after mlp_forward_dev); targets and masks
-// in n->tgt / n->mask
-int mlp_backward_dev(fbh_mlp* n, int M, hipStream_t s) {
+// the masked cross-entropy of rows [0, M) into n->loss and its gradient into n->dZ / n->dZb (after mlp_forward_dev);
+// targets and masks in n->tgt / n->mask, or soft targets in n->soft
+int mlp_loss_dev(fbh_mlp* n, int M, bool soft, hipStream_t s) {
   const int Mp = up64(M);
   const int* p = n->p;
-  hipLaunchKernelGGL(k_softmax_xent, dim3(Mp), dim3(256), 0, s, n->Z, p[3], n->d[3], M, n->tgt, n->mask,
-                     1.f / (float)M, n->dZ, n->dZb, n->loss);
+  if (soft)
+    hipLaunchKernelGGL(k_softmax_xent<true>, dim3(Mp), dim3(256), 0, s, n->Z, p[3], n->d[3], M, n->tgt, n->mask,
+                       n->soft, 1.f / (float)M, n->dZ, n->dZb, n->loss);
+  else
+    hipLaunchKernelGGL(k_softmax_xent<false>, dim3(Mp), dim3(256), 0, s, n->Z, p[3], n->d[3], M, n->tgt, n->mask,
+                       (const float*)nullptr, 1.f / (float)M, n->dZ, n->dZb, n->loss);
+  FBH_HIP_CHECK(hipGetLastError());
+  return FBH_OK;
+}
+
+// the soft-target buffer [max_batch][d_out] (allocated on first use)
+float* mlp_soft_buffer(fbh_mlp* n) {
+  if (!n->soft && hipMalloc(reinterpret_cast<void**>(&n->soft), sizeof(float) * (size_t)n->maxM * n->d[3]) !=
+                      hipSuccess) {
+    n->soft = nullptr;
+    set_error(FBH_ENOMEM, "fbh_mlp: soft-target buffer allocation failed");
+  }
+  return n->soft;
+}
+
+// masked cross-entropy and the gradients of the mean loss over rows [0, M) (after mlp_forward_dev); targets and masks
+// in n->tgt / n->mask (soft = false) or soft targets in n->soft
+int mlp_backward_dev(fbh_mlp* n, int M, hipStream_t s, bool soft) {
+  const int Mp = up64(M);
+  const int* p = n->p;
+  int rc = mlp_loss_dev(n, M, soft, s);
+  if (rc) return rc;
   // layer 3
   hipLaunchKernelGGL((k_gemm<true, false, EPI_F32>), dim3(p[3] / kBN, p[2] / kBM), dim3(256), 0, s, n->H2, n->dZb,
                      Mp, p[2], p[3], n->gW[2], nullptr, p[3], nullptr, nullptr);

@@ -1604,20 +1604,34 @@ constexpr int kRowsStreamL = kRowsStream;  // the learned pipeline streams the r
 // leaf values). The proposal correction (no reference counterpart; it plugs into the ratios of
 // src/bart_kernel.cpp:451 and :462): a birth with rule (v, c) multiplies pr by q_unif / q_mix, a death of a node with
 // rule (v, c) by q_mix / q_unif, where q_unif = (1/p) / (U - L + 1) is BART's rule draw and q_mix = (1 - alpha) q_unif +
-// alpha q_net(v, c | features of the node). The features are the node's partial residuals (excluding tree j) binned
-// by each variable's cutpoint rank, computed from an int64 fixed-point running fit so that they are bitwise the same
-// in the state before a birth and in the state after it (where the reverse death evaluates them). Each bin holds
-// sum(w r) sqrt(1 / (sigma^2 sum(w))) over the node's rows: in these units half the square of a child's partial sum
-// is its log-likelihood gain for large sums, so the scale of the split posterior is visible to the network (sigma is
-// fixed within a sweep, so the forward and the reverse move see the same value).
-constexpr int kFeatBins = 16;                          // cutpoint-rank bins per variable (rank >> 4)
-constexpr int kFeat = FBH_PMAX * kFeatBins + 1;        // 513: histograms + depth
+// alpha q_net(v, c | features of the node). The features come from the node's partial residuals (excluding tree j),
+// computed from an int64 fixed-point running fit so that they are bitwise the same in the state before a birth and in
+// the state after it (where the reverse death evaluates them; sigma is fixed within a sweep, and the node's rows and
+// cut ranges are the same in both states). Per variable v with cut range [L, U] at the node, 16 cut positions
+// log-spaced from both ends of the range (L + 2^k - 1 and U - 2^k + 1, k = 0..7) split the node's rows into 17
+// segments; the node's histograms of w and w r over those segments give, at each position c, the left child's sums
+// (W_c, S_c) and the split's standardized log-likelihood gain
+//   g = z^2 / 2 - log(f (1 - f)) / 2,  f = W_c / W,  z = (S_c - f S) / sqrt(W f (1 - f)) / sigma
+// (W, S the node's totals): the large-sum form of lh(left) + lh(right) - lh(node) (src/bart_kernel.cpp:278) up to
+// terms equal for every split. Positions outside [L, U] give 0, the last feature is the node's depth / 8. The
+// likelihood gain concentrates the split conditional near the range ends, where log-spaced positions resolve it; a
+// plain 16-bin residual histogram over ranks (rank >> 4) leaves the network at the level of the class prior.
+constexpr int kFeatPos = 16;                           // cut positions per variable
+constexpr int kSegs = kFeatPos + 1;                    // row segments per variable
+constexpr int kFeat = FBH_PMAX * kFeatPos + 1;         // 513: gains + depth
 constexpr int kCuts = 255;                             // classes (v, c) -> v * 255 + c
 constexpr int kClasses = FBH_PMAX * kCuts;             // 8160
 constexpr double kFxFit = 4294967296.0;                // running fit at 2^-32
 constexpr double kFxFeat = 1048576.0;                  // feature sums at 2^-20
 constexpr uint16_t kNoRange = 0x00FF;                  // L = 255 > U = 0
-constexpr int kHistCopies = 4;                         // copies of the node histograms (a power of two)
+constexpr int kHistCopies = 2;                         // copies of the node histograms (a power of two)
+
+// position k of variable v's 16 at a node with cut range [L, U] (U < L: none); -1 when outside the range
+__device__ __forceinline__ int feat_pos(int L, int U, int k) {
+  if (U < L) return -1;
+  const int c = k < 8 ? L + (1 << k) - 1 : U - (1 << (15 - k)) + 1;
+  return (c >= L && c <= U) ? c : -1;
+}
 
 struct LpState {
   uint32_t rng[4];
@@ -1641,8 +1655,10 @@ struct LpArgs {
   unsigned long long* pcount;
   long pcap;
   float* pent;              // [pcap][2]: entropy of the pair's split conditional, log of its allowed class count
-  unsigned long long tseed; // label draws of the conditional target (a stream apart from the chains' generators)
-  int target;               // pairs: 0 = accepted births with BART's rule, 1 = every proposed birth, conditional label
+  float* pq;                // [pcap][kClasses]: the pair's split conditional (target kind 1)
+  unsigned long long tseed; // label and selection draws of the conditional target (apart from the chains' generators)
+  float keep;               // target kind 1: share of the proposed births taken as pairs
+  int target;               // pairs: 0 = accepted births with BART's rule, 1 = proposed births, conditional target
   int j;
 };
 
@@ -1741,15 +1757,17 @@ __device__ void lp_choose(Ctl& c, Rng& g, int lane, int* mode_out, int* nx_out) 
 template <bool FX>
 __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(4))) void k_lp_stage1(LpArgs A) {
   __shared__ Ctl c;
-  // the node's histograms in kHistCopies copies, lane l adding into copy l % kHistCopies: a wave's 64 lanes then spread
-  // over 4 x 16 bins of a variable instead of 16 (fewer lanes serialised on one address); the copies are added after
-  __shared__ unsigned long long H[kHistCopies][kFeat];
+  // the node's histograms of w and w r over the row segments of each variable (int64 at 2^-20) in kHistCopies copies,
+  // lane l adding into copy l % kHistCopies (fewer lanes serialised on one address); the copies are added after
+  __shared__ unsigned long long H[kHistCopies][FBH_PMAX][kSegs][2];
+  __shared__ uint8_t seg[FBH_PMAX][256];  // segment of each cutpoint rank: the number of positions below it
+  __shared__ uint16_t rgs[FBH_PMAX];      // cut ranges at the node, L | U << 8
   unsigned char* const dyn = fbh_dyn;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, dw = wave - 1;
   const int chain = blockIdx.x, j = A.j;
   const KArgs& a = A.a;
   img_copy(&c, A.img + chain, tid);
-  for (int k = tid; k < kHistCopies * kFeat; k += kBlock) (&H[0][0])[k] = 0;
+  for (int k = tid; k < kHistCopies * FBH_PMAX * kSegs * 2; k += kBlock) (&H[0][0][0][0])[k] = 0;
   __syncthreads();
   const RowPtrs P = lp_rows(a, c.cd);
   const int n = c.cd.n, p = c.cd.p;
@@ -1777,6 +1795,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       c.cnx[0] = nx;
       c.cv[0] = mode;  // stage 1 keeps the move type here until stage 2 sets the rule
     }
+    if (lane < FBH_PMAX) {
+      uint16_t rv = kNoRange;
+      if (mode && lane < p) {
+        int L, U;
+        node_range(c.tree[0], nx, lane, c.cd.nc[lane], &L, &U);
+        if (U >= L) rv = (uint16_t)(L | (U << 8));
+      }
+      rgs[lane] = rv;
+    }
   }
   __syncthreads();
   if (wave > 0) {
@@ -1787,79 +1814,111 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
   __syncthreads();
   if (wave == 0) ctl_combine<FX>(c, 0, false, 0, lane);
   const int mode = c.cv[0], nx = c.cnx[0];
-  if (mode) {
+  if (mode) {  // the same value in every thread
+    for (int e = tid; e < p * 256; e += kBlock) {
+      const int v = e >> 8, r = e & 255;
+      const int L = rgs[v] & 255, U = rgs[v] >> 8;
+      int sg = 0;
+#pragma unroll
+      for (int k = 0; k < kFeatPos; k++) {
+        const int cp = feat_pos(L, U, k);
+        sg += (cp >= 0 && cp < r);
+      }
+      seg[v][r] = (uint8_t)sg;
+    }
+    __syncthreads();
     // the node's rows: the leaf nx (birth) or the two children of nx (death)
     const TreeDev& t = c.tree[0];
     const int l = mode == 2 ? t.l[nx] : nx, r = mode == 2 ? t.r[nx] : nx;
     const uint8_t* lc = P.LEAF + (size_t)j * P.ns;
-    unsigned long long* Hc = H[lane & (kHistCopies - 1)];
-    long long wsum = 0;
+    auto& Hc = H[lane & (kHistCopies - 1)];
     // four consecutive rows per thread and iteration: the byte columns (16-byte aligned, padded to a multiple of 16
-    // rows) as one dword each, and the cutpoint ranks of eight variables loaded together before their adds, so that
-    // an iteration waits for memory three or four times instead of once per variable and row
+    // rows) as one dword each, and the cutpoint ranks of four variables loaded together before their adds, so that
+    // an iteration waits for memory a few times instead of once per variable and row
     for (int i4 = 4 * tid; i4 < n; i4 += 4 * kBlock) {
       const uint32_t lw = *reinterpret_cast<const uint32_t*>(lc + i4);
       bool in[4];
-      unsigned long long wr[4];
+      unsigned long long wf[4], wr[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int i = i4 + u;
         const int lf = (int)((lw >> (8 * u)) & 255u);
         in[u] = i < n && (lf == l || lf == r);
-        wr[u] = 0;
+        wf[u] = wr[u] = 0;
         if (in[u]) {
           const double res = (double)P.Y[i] - (double)(AFX[i] - fit_fx(c.thr[0][lf].th)) * (1.0 / kFxFit);
           const double w = (double)P.W[i];
+          wf[u] = (unsigned long long)(long long)rint(w * kFxFeat);
           wr[u] = (unsigned long long)(long long)rint(w * res * kFxFeat);
-          wsum += (long long)rint(w * kFxFeat);
         }
       }
       if (!(in[0] || in[1] || in[2] || in[3])) continue;
-      for (int v0 = 0; v0 < p; v0 += 8) {
-        uint32_t qw[8];
+      for (int v0 = 0; v0 < p; v0 += 4) {
+        uint32_t qw[4];
 #pragma unroll
-        for (int k = 0; k < 8; k++)
+        for (int k = 0; k < 4; k++)
           qw[k] = v0 + k < p ? *reinterpret_cast<const uint32_t*>(P.Q + (size_t)(v0 + k) * P.ns + i4) : 0u;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < 4; k++) {
           if (v0 + k >= p) break;
 #pragma unroll
           for (int u = 0; u < 4; u++)
-            if (in[u]) atomicAdd(&Hc[(v0 + k) * kFeatBins + (((qw[k] >> (8 * u)) & 255u) >> 4)], wr[u]);
+            if (in[u]) {
+              const int sg = seg[v0 + k][(qw[k] >> (8 * u)) & 255u];
+              atomicAdd(&Hc[v0 + k][sg][0], wf[u]);
+              atomicAdd(&Hc[v0 + k][sg][1], wr[u]);
+            }
         }
       }
     }
-    // the weight total: one add per thread
-    if (wsum) atomicAdd(&Hc[kFeat - 1], (unsigned long long)wsum);
   }
   __syncthreads();
   if (mode)
-    for (int k = tid; k < kFeat; k += kBlock) {
-      unsigned long long t = H[0][k];
-      for (int h = 1; h < kHistCopies; h++) t += H[h][k];
-      H[0][k] = t;
+    for (int k = tid; k < FBH_PMAX * kSegs * 2; k += kBlock) {
+      unsigned long long t = (&H[0][0][0][0])[k];
+      for (int h = 1; h < kHistCopies; h++) t += (&H[h][0][0][0])[k];
+      (&H[0][0][0][0])[k] = t;
     }
   __syncthreads();
   float* X = A.X + (size_t)chain * kFeat;
   if (mode) {
-    const double sw = (double)(long long)H[0][kFeat - 1] * (1.0 / kFxFeat);
-    const double scale = sw > 0 ? __builtin_sqrt(c.inv_s2 / sw) * (1.0 / kFxFeat) : 0.0;
-    for (int k = tid; k < kFeat - 1; k += kBlock)
-      X[k] = (k / kFeatBins) < p ? (float)((double)(long long)H[0][k] * scale) : 0.f;
-    if (tid == 0) X[kFeat - 1] = (float)c.tree[0].d[nx] * 0.125f;
-    if (wave == 0 && lane < FBH_PMAX) {
-      uint16_t rv = kNoRange;
-      if (lane < p) {
-        int L, U;
-        node_range(c.tree[0], nx, lane, c.cd.nc[lane], &L, &U);
-        if (U >= L) rv = (uint16_t)(L | (U << 8));
+    for (int k = tid; k < FBH_PMAX * kFeatPos; k += kBlock) {
+      const int v = k / kFeatPos, kk = k - v * kFeatPos;
+      float g = 0.f;
+      const int L = rgs[v] & 255, U = rgs[v] >> 8;
+      const int cp = v < p ? feat_pos(L, U, kk) : -1;
+      if (cp >= 0) {
+        // left child's sums: the segments up to the number of positions below cp
+        int sk = 0;
+        for (int q = 0; q < kFeatPos; q++) {
+          const int cq = feat_pos(L, U, q);
+          sk += (cq >= 0 && cq < cp);
+        }
+        long long w0 = 0, w1 = 0, t0 = 0, t1 = 0;
+        for (int q = 0; q < kSegs; q++) {
+          const long long h0 = (long long)H[0][v][q][0], h1 = (long long)H[0][v][q][1];
+          t0 += h0;
+          t1 += h1;
+          if (q <= sk) {
+            w0 += h0;
+            w1 += h1;
+          }
+        }
+        if (t0 > 0) {
+          const double W = (double)t0 * (1.0 / kFxFeat), Sm = (double)t1 * (1.0 / kFxFeat);
+          const double f = fmin(fmax((double)w0 / (double)t0, 1e-6), 1.0 - 1e-6);
+          const double z = ((double)w1 * (1.0 / kFxFeat) - f * Sm) / __builtin_sqrt(W * f * (1.0 - f)) *
+                           __builtin_sqrt(c.inv_s2);
+          g = (float)(0.5 * z * z - 0.5 * log(f * (1.0 - f)));
+        }
       }
-      A.rg[(size_t)chain * FBH_PMAX + lane] = rv;
+      X[k] = g;
     }
+    if (tid == 0) X[kFeat - 1] = (float)c.tree[0].d[nx] * 0.125f;
   } else {
     for (int k = tid; k < kFeat; k += kBlock) X[k] = 0.f;
-    if (wave == 0 && lane < FBH_PMAX) A.rg[(size_t)chain * FBH_PMAX + lane] = kNoRange;
   }
+  if (tid < FBH_PMAX) A.rg[(size_t)chain * FBH_PMAX + tid] = rgs[tid];
   __syncthreads();
   img_copy(A.img + chain, &c, tid);
 }
@@ -2097,7 +2156,11 @@ __global__ __launch_bounds__(kTgtBlock) void k_lp_target(LpArgs A) {
   if (tid == 0) {
     const int mode = c.cv[0], nx = c.cnx[0];
     const unsigned long long used = __atomic_load_n(A.pcount, __ATOMIC_RELAXED);
-    go = mode == 1 && (int)c.tree[0].d[nx] + 1 < kMaxDepth && (long long)used < (long long)A.pcap;
+    const int it = A.st[chain].it;
+    const unsigned long long key = ((unsigned long long)chain << 40) ^ ((unsigned long long)it * (unsigned)a.m + j);
+    const bool take = A.keep >= 1.f || mix_unif(A.tseed ^ 0xD1B54A32D192ED03ull ^ (key * 0x9E3779B97F4A7C15ull)) <
+                                           (double)A.keep;
+    go = mode == 1 && take && (int)c.tree[0].d[nx] + 1 < kMaxDepth && (long long)used < (long long)A.pcap;
     nx_s = nx;
     N.n = c.cd.n;
     N.p = c.cd.p;
@@ -2153,6 +2216,14 @@ __global__ __launch_bounds__(kTgtBlock) void k_lp_target(LpArgs A) {
   const float* X = A.X + (size_t)chain * kFeat;
   for (int q = tid; q < kFeat; q += kTgtBlock) A.px[(size_t)slot * kFeat + q] = X[q];
   if (tid < FBH_PMAX) A.prg[(size_t)slot * FBH_PMAX + tid] = N.rg[tid];
+  if (A.pq) {
+    float* q = A.pq + (size_t)slot * kClasses;
+    for (int k = tid; k < kClasses; k += kTgtBlock) {
+      const int v = k / kCuts, cc = k - v * kCuts;
+      const float l = v < N.p ? S.lw[v][cc] : -INFINITY;
+      q[k] = l > -INFINITY ? (float)(exp((double)l - S.mx) / S.Z) : 0.f;
+    }
+  }
 }
 
 // the split conditional of one node given on the host (the check of split_conditional / split_draw): q over the
@@ -2610,13 +2681,15 @@ __global__ __launch_bounds__(kBlock) void k_lp_begin(LpArgs A) {
 
 // a minibatch of training pairs into the network's batch buffers: features, class, allowed classes from the ranges
 __global__ void k_lp_gather(const float* __restrict__ px, const uint16_t* __restrict__ prg, const int* __restrict__ ptg,
-                            const int* __restrict__ idx, int M, float* __restrict__ x, int* __restrict__ tgt,
-                            uint8_t* __restrict__ mask) {
+                            const float* __restrict__ pq, const int* __restrict__ idx, int M, float* __restrict__ x,
+                            int* __restrict__ tgt, uint8_t* __restrict__ mask, float* __restrict__ soft) {
   const int b = blockIdx.x, tid = threadIdx.x;
   if (b >= M) return;
   const long r = idx[b];
   for (int k = tid; k < kFeat; k += blockDim.x) x[(size_t)b * kFeat + k] = px[(size_t)r * kFeat + k];
   for (int k = tid; k < kClasses; k += blockDim.x) mask[(size_t)b * kClasses + k] = lp_valid(prg + r * FBH_PMAX, k);
+  if (pq)
+    for (int k = tid; k < kClasses; k += blockDim.x) soft[(size_t)b * kClasses + k] = pq[(size_t)r * kClasses + k];
   if (tid == 0) tgt[b] = ptg[r];
 }
 
@@ -2709,8 +2782,10 @@ struct fbh_batch {
   unsigned long long* d_pcount = nullptr;
   int* d_idx = nullptr;
   float* d_pent = nullptr;
-  long pcap = 0;
+  float* d_pq = nullptr;
+  long pcap = 0, hold = 0;
   int target = 0;
+  float keep = 1.f;
   unsigned long long tseed = 0;
   std::vector<hipEvent_t> lev;  // start/stop pairs per learned sweep
 };
@@ -2721,7 +2796,7 @@ static void free_batch(fbh_batch* b) {
   void* ptrs[] = {b->d_desc, b->d_state, b->d_trees, b->d_q,   b->d_Y,   b->d_W,   b->d_yin,    b->d_AF,
                   b->d_TM,   b->d_ybin,  b->d_leaf,  b->d_sigma, b->d_tout, b->d_PG, b->d_trm, b->d_trace,
                   b->d_img,  b->d_lps,   b->d_afx,   b->d_X,     b->d_rg,   b->d_px, b->d_prg, b->d_ptg,
-                  b->d_pcount, b->d_idx, b->d_xbuf, b->d_xctr, b->d_order, b->d_pent};
+                  b->d_pcount, b->d_idx, b->d_xbuf, b->d_xctr, b->d_order, b->d_pent, b->d_pq};
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto g : b->gexecs) (void)hipGraphExecDestroy(g);
   for (auto g : b->graphs) (void)hipGraphDestroy(g);
@@ -3738,7 +3813,9 @@ static LpArgs lp_args(const fbh_batch* b, int j) {
   A.pcount = b->d_pcount;
   A.pcap = b->pcap;
   A.pent = b->d_pent;
+  A.pq = b->d_pq;
   A.tseed = b->tseed;
+  A.keep = b->keep;
   A.target = b->target;
   A.j = j;
   return A;
@@ -3777,8 +3854,11 @@ extern "C" int fbh_batch_learned_setup(fbh_batch* b, fbh_mlp* net, float alpha, 
                                       hipFuncAttributeMaxDynamicSharedMemorySize, Geo<kRowsStreamL>::kScratchBytes));
   }
   if (pair_capacity != b->pcap) {
-    for (void* q : {(void*)b->d_px, (void*)b->d_prg, (void*)b->d_ptg, (void*)b->d_idx, (void*)b->d_pent})
+    for (void* q : {(void*)b->d_px, (void*)b->d_prg, (void*)b->d_ptg, (void*)b->d_idx, (void*)b->d_pent,
+                    (void*)b->d_pq})
       if (q) (void)hipFree(q);
+    b->d_pq = nullptr;
+    b->hold = 0;
     b->d_px = nullptr;
     b->d_prg = nullptr;
     b->d_ptg = nullptr;
@@ -3793,6 +3873,7 @@ extern "C" int fbh_batch_learned_setup(fbh_batch* b, fbh_mlp* net, float alpha, 
           (rc = dalloc(&b->d_idx, mb)) || (rc = dalloc(&b->d_pent, (size_t)pair_capacity * 2)))
         return rc;
       b->pcap = pair_capacity;
+      if (b->target == 1 && (rc = dalloc(&b->d_pq, (size_t)pair_capacity * kClasses))) return rc;
     }
     FBH_HIP_CHECK(hipMemset(b->d_pcount, 0, sizeof(unsigned long long)));
   }
@@ -3852,14 +3933,82 @@ extern "C" int fbh_batch_run_learned(fbh_batch* b, int nsweeps) {
   return b->stats_mode == FBH_STATS_FX64 ? lp_run<true>(b, nsweeps) : lp_run<false>(b, nsweeps);
 }
 
-extern "C" int fbh_batch_learned_target(fbh_batch* b, int kind, unsigned long long seed) {
-  if (!b || (kind != 0 && kind != 1)) {
-    set_error(FBH_EINVAL, "fbh_batch_learned_target: kind must be 0 (accepted births, BART's rule) or 1 (every "
-                          "proposed birth, a rule drawn from the exact split conditional)");
+extern "C" int fbh_batch_learned_target(fbh_batch* b, int kind, float fraction, unsigned long long seed) {
+  if (!b || (kind != 0 && kind != 1) || !(fraction > 0.f && fraction <= 1.f)) {
+    set_error(FBH_EINVAL, "fbh_batch_learned_target: kind must be 0 (accepted births, BART's rule) or 1 (proposed "
+                          "births, the exact split conditional), fraction in (0, 1]");
     return FBH_EINVAL;
   }
+  FBH_HIP_CHECK(hipSetDevice(b->device));
+  FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
   b->target = kind;
+  b->keep = fraction;
   b->tseed = seed;
+  if (kind == 1 && b->pcap > 0 && !b->d_pq) {
+    const int rc = dalloc(&b->d_pq, (size_t)b->pcap * kClasses);
+    if (rc) return rc;
+  }
+  return FBH_OK;
+}
+
+extern "C" int fbh_batch_learned_holdout(fbh_batch* b, long count) {
+  if (!b || count < 0) {
+    set_error(FBH_EINVAL, "fbh_batch_learned_holdout: bad argument");
+    return FBH_EINVAL;
+  }
+  b->hold = count;
+  return FBH_OK;
+}
+
+extern "C" int fbh_batch_learned_eval(fbh_batch* b, long k0, long count, double* mean_loss, double* mean_entropy,
+                                      double* mean_log_allowed) {
+  const long np = fbh_batch_learned_pairs(b);
+  if (!b || !b->net || np < 0 || k0 < 0 || count < 1 || k0 + count > np || !mean_loss) {
+    set_error(FBH_EINVAL, "fbh_batch_learned_eval: no network or a range outside the collected pairs");
+    return FBH_EINVAL;
+  }
+  int d[4], mb = 0;
+  mlp_dims(b->net, d, &mb);
+  float *x, *loss;
+  int* tgt;
+  uint8_t* mask;
+  mlp_batch_buffers(b->net, &x, &tgt, &mask, &loss);
+  const bool soft = b->target == 1 && b->d_pq;
+  float* sb = soft ? mlp_soft_buffer(b->net) : nullptr;
+  if (soft && !sb) return FBH_ENOMEM;
+  std::vector<int> idx(mb);
+  std::vector<float> l(mb);
+  double tot = 0;
+  for (long k = k0; k < k0 + count; k += mb) {
+    const int M = (int)std::min<long>(mb, k0 + count - k);
+    for (int t = 0; t < M; t++) idx[t] = (int)(k + t);
+    FBH_HIP_CHECK(hipMemcpyAsync(b->d_idx, idx.data(), sizeof(int) * M, hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(k_lp_gather, dim3(M), dim3(256), 0, b->stream, b->d_px, b->d_prg, b->d_ptg,
+                       soft ? b->d_pq : nullptr, b->d_idx, M, x, tgt, mask, sb);
+    FBH_HIP_CHECK(hipGetLastError());
+    int rc;
+    if ((rc = mlp_forward_dev(b->net, x, M, b->stream)) || (rc = mlp_loss_dev(b->net, M, soft, b->stream))) return rc;
+    FBH_HIP_CHECK(hipMemcpyAsync(l.data(), loss, sizeof(float) * M, hipMemcpyDeviceToHost, b->stream));
+    FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
+    for (int t = 0; t < M; t++) tot += l[t];
+  }
+  *mean_loss = tot / count;
+  if (mean_entropy || mean_log_allowed) {
+    double e = std::nan(""), la = std::nan("");
+    if (b->target == 1) {
+      std::vector<float> h((size_t)count * 2);
+      FBH_HIP_CHECK(hipMemcpy(h.data(), b->d_pent + 2 * k0, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+      e = la = 0;
+      for (long k = 0; k < count; k++) {
+        e += h[2 * k];
+        la += h[2 * k + 1];
+      }
+      e /= count;
+      la /= count;
+    }
+    if (mean_entropy) *mean_entropy = e;
+    if (mean_log_allowed) *mean_log_allowed = la;
+  }
   return FBH_OK;
 }
 
@@ -4004,15 +4153,18 @@ extern "C" int fbh_batch_learned_train(fbh_batch* b, int steps, int batch, float
     set_error(FBH_EINVAL, "fbh_batch_learned_train: batch above the network's max_batch");
     return FBH_EINVAL;
   }
-  const long npairs = fbh_batch_learned_pairs(b);
+  const long npairs = fbh_batch_learned_pairs(b) - b->hold;
   if (npairs < 1) {
-    set_error(FBH_EINVAL, "fbh_batch_learned_train: no training pairs collected");
+    set_error(FBH_EINVAL, "fbh_batch_learned_train: no training pairs collected (past the held-out ones)");
     return FBH_EINVAL;
   }
   float *x, *loss;
   int* tgt;
   uint8_t* mask;
   mlp_batch_buffers(b->net, &x, &tgt, &mask, &loss);
+  const bool soft = b->target == 1 && b->d_pq;
+  float* sb = soft ? mlp_soft_buffer(b->net) : nullptr;
+  if (soft && !sb) return FBH_ENOMEM;
   uint64_t st = (uint64_t)seed * 0x9E3779B97F4A7C15ull + 3;
   std::vector<int> idx(batch);
   std::vector<float> l(batch);
@@ -4024,11 +4176,12 @@ extern "C" int fbh_batch_learned_train(fbh_batch* b, int steps, int batch, float
       idx[k] = (int)((z ^ (z >> 31)) % (uint64_t)npairs);
     }
     FBH_HIP_CHECK(hipMemcpyAsync(b->d_idx, idx.data(), sizeof(int) * batch, hipMemcpyHostToDevice, b->stream));
-    hipLaunchKernelGGL(k_lp_gather, dim3(batch), dim3(256), 0, b->stream, b->d_px, b->d_prg, b->d_ptg, b->d_idx,
-                       batch, x, tgt, mask);
+    hipLaunchKernelGGL(k_lp_gather, dim3(batch), dim3(256), 0, b->stream, b->d_px, b->d_prg, b->d_ptg,
+                       soft ? b->d_pq : nullptr, b->d_idx, batch, x, tgt, mask, sb);
     FBH_HIP_CHECK(hipGetLastError());
     int rc;
-    if ((rc = mlp_forward_dev(b->net, x, batch, b->stream)) || (rc = mlp_backward_dev(b->net, batch, b->stream)) ||
+    if ((rc = mlp_forward_dev(b->net, x, batch, b->stream)) ||
+        (rc = mlp_backward_dev(b->net, batch, b->stream, soft)) ||
         (rc = mlp_adam_dev(b->net, lr, b->stream)))
       return rc;
     if (loss_out) {

@@ -156,11 +156,22 @@ class BatchSampler:
     def run_learned(self, nsweeps):
         check(lib().fbh_batch_run_learned(self.h, int(nsweeps)), "fbh_batch_run_learned")
 
-    def learned_target(self, kind, seed=1):
-        """0: accepted births with BART's rule; 1: every proposed birth's node, labelled by a draw from the exact
-        split conditional there (set before collecting)."""
-        check(lib().fbh_batch_learned_target(self.h, int(kind), int(seed) & 0xFFFFFFFFFFFFFFFF),
+    def learned_target(self, kind, fraction=1.0, seed=1):
+        """0: accepted births with BART's rule; 1: proposed births (each with probability fraction), the exact split
+        conditional there as a soft target (set before collecting)."""
+        check(lib().fbh_batch_learned_target(self.h, int(kind), float(fraction), int(seed) & 0xFFFFFFFFFFFFFFFF),
               "fbh_batch_learned_target")
+
+    def learned_holdout(self, count):
+        """keep the last count pairs out of training"""
+        check(lib().fbh_batch_learned_holdout(self.h, int(count)), "fbh_batch_learned_holdout")
+
+    def learned_eval(self, k0, count):
+        """(mean loss of the network, mean entropy, mean log allowed class count) on pairs [k0, k0 + count)"""
+        l, e, a = C.c_double(), C.c_double(), C.c_double()
+        check(lib().fbh_batch_learned_eval(self.h, int(k0), int(count), C.byref(l), C.byref(e), C.byref(a)),
+              "fbh_batch_learned_eval")
+        return l.value, e.value, a.value
 
     def learned_target_stats(self):
         """(mean entropy of the pairs' split conditionals, mean log of their allowed class counts), nats"""

@@ -317,7 +317,8 @@ void fbh_mlp_destroy(fbh_mlp* n);
 /* ---------------------------------------------------------------- learned birth proposal: sampling with it
  * No reference counterpart. The batch's next sweeps run tree update by tree update over all chains in lock-step:
  * the birth rule comes from the mixture (1 - alpha) x BART's uniform rule + alpha x the network's masked softmax given
- * the node's 32 x 16 partial-residual histogram and depth, and the Metropolis-Hastings ratios carry the exact proposal
+ * the node's features (per variable, the split's log-likelihood gain at 16 cut positions from the node's histograms
+ * of w and w r over its rows, and the depth), and the Metropolis-Hastings ratios carry the exact proposal
  * correction (births x q_unif / q_mix, deaths x q_mix / q_unif), so the posterior is unchanged. alpha = 0 is BART's
  * proposal. wbart batches only; the network maps 513 features to 8160 = 32 x 255 classes (v * 255 + c). */
 /* attach the network (may be null when alpha = 0 and nothing is collected) and the mixture weight; pair_capacity > 0
@@ -327,20 +328,28 @@ int fbh_batch_learned_setup(fbh_batch* b, fbh_mlp* net, float alpha, long pair_c
 int fbh_batch_run_learned(fbh_batch* b, int nsweeps);
 /* training pairs collected so far (at most the capacity) */
 long fbh_batch_learned_pairs(fbh_batch* b);
-/* what a collected pair is: kind 0 (the default) = each accepted birth with BART's rule; kind 1 = the node of each
- * proposed birth labelled by a rule drawn from the exact split conditional there (BART's rule probability times the
- * rule-dependent factors of the birth ratio, over the rules whose children hold at least 5 rows), drawn from a stream
- * seeded by seed apart from the chains' generators, so the chains are unchanged. Set before collecting. */
-int fbh_batch_learned_target(fbh_batch* b, int kind, unsigned long long seed);
+/* what a collected pair is: kind 0 (the default) = each accepted birth with BART's rule; kind 1 = the node of a
+ * proposed birth (each taken with probability fraction) with the exact split conditional there as a soft target
+ * (BART's rule probability times the rule-dependent factors of the birth ratio, over the rules whose children hold at
+ * least 5 rows; 32 KB per pair) and a rule drawn from it. Its draws come from a stream seeded by seed apart from the
+ * chains' generators, so the chains are unchanged. Set before collecting. */
+int fbh_batch_learned_target(fbh_batch* b, int kind, float fraction, unsigned long long seed);
 /* kind 1 only (else NaN): the mean over the collected pairs of the conditional's entropy (the cross-entropy floor a
  * network can reach) and of the log of the allowed class count (the uniform rule's cross-entropy), in nats */
 int fbh_batch_learned_target_stats(fbh_batch* b, double* mean_entropy, double* mean_log_allowed);
+/* keep the last count collected pairs out of fbh_batch_learned_train (for fbh_batch_learned_eval) */
+int fbh_batch_learned_holdout(fbh_batch* b, long count);
+/* the network's mean loss on pairs [k0, k0 + count): cross-entropy against the conditional (kind 1) or the recorded
+ * rule (kind 0); for kind 1 also the pairs' mean entropy and log allowed class count (either pointer may be null) */
+int fbh_batch_learned_eval(fbh_batch* b, long k0, long count, double* mean_loss, double* mean_entropy,
+                           double* mean_log_allowed);
 /* copy pairs [k0, k0 + count) to the host (any pointer may be null): features [count][513], ranges [count][32]
  * (L | U << 8), cls [count] (v * 255 + c), entropy [count][2] (kind 1: the conditional's entropy and the log of the
  * allowed class count; kind 0: NaN) */
 int fbh_batch_learned_pairs_copy(fbh_batch* b, long k0, long count, float* features, uint16_t* ranges, int* cls,
                                  float* entropy);
-/* steps Adam steps on minibatches of the collected pairs (device-resident), loss_out [steps] may be null */
+/* steps Adam steps on minibatches of the collected pairs (device-resident; soft targets for kind 1), loss_out [steps]
+ * may be null */
 int fbh_batch_learned_train(fbh_batch* b, int steps, int batch, float lr, unsigned seed, float* loss_out);
 /* average duration (ms) of the learned sweeps so far, and their count */
 int fbh_batch_learned_timing(const fbh_batch* b, double* avg_ms, int* nsweeps);

@@ -125,9 +125,10 @@ def test_gpu_split_conditional_matches_oracle_on_chain_nodes(oracle):
 
 def test_gpu_conditional_target_collects_pairs_and_leaves_the_chains_unchanged(oracle):
     """16 workload chains: 4 plain sweeps, then 4 lock-step sweeps at alpha 0 collecting every proposed birth's node
-    with a label drawn from the split conditional. The chains' outputs equal the oracle's plain fit bit for bit (the
-    labels come from a stream apart from the chains' generators); the pairs' mean entropy lies below the uniform
-    rule's level, and training on them is finite and brings the loss below that level."""
+    with its split conditional as a soft target. The chains' outputs equal the oracle's plain fit bit for bit (the
+    target's draws come from a stream apart from the chains' generators); the labels lie in their pairs' ranges, the
+    conditionals' mean entropy lies below the uniform rule's level, and training on them lowers the loss on held-out
+    pairs (the last fifth, from the latest sweeps)."""
     nch = 16
     probs = [W.batch512_chain(c) for c in range(nch)]
     seeds = [1 + c for c in range(nch)]
@@ -141,9 +142,14 @@ def test_gpu_conditional_target_collects_pairs_and_leaves_the_chains_unchanged(o
         outs = [b.fetch(c) for c in range(nch)]
         npairs = b.learned_pairs()
         ent, lall = b.learned_target_stats()
-        losses = b.learned_train(200, batch=512, lr=1e-3, seed=3)
-    _say("pairs", npairs, "mean entropy", ent, "mean log allowed", lall, "loss", float(losses[:10].mean()), "->",
-         float(losses[-10:].mean()))
+        pairs = b.learned_pairs_copy(0, 50)
+        hold = npairs // 5
+        b.learned_holdout(hold)
+        before = b.learned_eval(npairs - hold, hold)
+        losses = b.learned_train(300, batch=512, lr=1e-3, seed=3)
+        after = b.learned_eval(npairs - hold, hold)
+    _say("pairs", npairs, "mean entropy", ent, "mean log allowed", lall, "training loss", float(losses[:10].mean()),
+         "->", float(losses[-10:].mean()), "held-out", before, "->", after)
     for c in range(nch):
         e = oracle.fit(probs[c], m, nskip, nd, seeds[c])
         for k in ("sigma", "trmean", "trees", "stats"):
@@ -151,5 +157,11 @@ def test_gpu_conditional_target_collects_pairs_and_leaves_the_chains_unchanged(o
     # about half the tree updates propose a birth; nodes with no qualifying rule give no pair
     assert 0.25 * nch * m * 4 < npairs <= 0.55 * nch * m * 4
     assert np.isfinite(ent) and np.isfinite(lall) and 0 < ent < lall
+    # every label lies within its pair's allowed ranges
+    for k in range(50):
+        v, cut = divmod(int(pairs["cls"][k]), 255)
+        r = int(pairs["ranges"][k, v])
+        assert (r & 255) <= cut <= (r >> 8), k
     assert np.isfinite(losses).all()
-    assert losses[-20:].mean() < lall - 0.05
+    assert after[0] < before[0] and after[0] < after[2]
+    assert after[1] < after[0]

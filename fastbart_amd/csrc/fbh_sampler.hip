@@ -1609,7 +1609,9 @@ constexpr int kRowsStreamL = kRowsStream;  // the learned pipeline streams the r
 // the state after it (where the reverse death evaluates them; sigma is fixed within a sweep, and the node's rows and
 // cut ranges are the same in both states). Per variable v with cut range [L, U] at the node, 16 cut positions
 // log-spaced from both ends of the range (L + 2^k - 1 and U - 2^k + 1, k = 0..7) split the node's rows into 17
-// segments; the node's histograms of w and w r over those segments give, at each position c, the left child's sums
+// segments, each cut further at every 16th rank (bin of rank r: the positions below r plus r / 16, at most 31, which
+// keeps the lanes of a wave on different histogram words); the node's histograms of w and w r over those bins give,
+// at each position c, the left child's sums
 // (W_c, S_c) and the split's standardized log-likelihood gain
 //   g = z^2 / 2 - log(f (1 - f)) / 2,  f = W_c / W,  z = (S_c - f S) / sqrt(W f (1 - f)) / sigma
 // (W, S the node's totals): the large-sum form of lh(left) + lh(right) - lh(node) (src/bart_kernel.cpp:278) up to
@@ -1617,7 +1619,7 @@ constexpr int kRowsStreamL = kRowsStream;  // the learned pipeline streams the r
 // likelihood gain concentrates the split conditional near the range ends, where log-spaced positions resolve it; a
 // plain 16-bin residual histogram over ranks (rank >> 4) leaves the network at the level of the class prior.
 constexpr int kFeatPos = 16;                           // cut positions per variable
-constexpr int kSegs = kFeatPos + 1;                    // row segments per variable
+constexpr int kSegs = 32;                              // histogram bins per variable
 constexpr int kFeat = FBH_PMAX * kFeatPos + 1;         // 513: gains + depth
 constexpr int kCuts = 255;                             // classes (v, c) -> v * 255 + c
 constexpr int kClasses = FBH_PMAX * kCuts;             // 8160
@@ -1626,11 +1628,24 @@ constexpr double kFxFeat = 1048576.0;                  // feature sums at 2^-20
 constexpr uint16_t kNoRange = 0x00FF;                  // L = 255 > U = 0
 constexpr int kHistCopies = 2;                         // copies of the node histograms (a power of two)
 
+constexpr double kFxW = 4096.0;                        // weight histogram at 2^-12 (uint32: a node's total < 2^31)
+
 // position k of variable v's 16 at a node with cut range [L, U] (U < L: none); -1 when outside the range
 __device__ __forceinline__ int feat_pos(int L, int U, int k) {
   if (U < L) return -1;
   const int c = k < 8 ? L + (1 << k) - 1 : U - (1 << (15 - k)) + 1;
   return (c >= L && c <= U) ? c : -1;
+}
+// number of k with 2^k <= x, k = 0..7 (x >= 0)
+__device__ __forceinline__ int pow2_upto(int x) { return x > 0 ? min(8, 32 - __clz(x)) : 0; }
+// bin of cutpoint rank r: r / 16 plus the positions below r. The positions below r from the left end are the k with
+// L + 2^k - 1 < r and 2^k <= U - L + 1; from the right end the k with U - 2^k + 1 < r and 2^k <= U - L + 1.
+__device__ __forceinline__ int feat_bin(int L, int U, int r) {
+  if (U < L) return r >> 4;
+  const int K = U - L + 1;
+  const int left = pow2_upto(min(r - L, K));
+  const int right = pow2_upto(K) - pow2_upto(U + 1 - r);
+  return (r >> 4) + left + max(right, 0);
 }
 
 struct LpState {
@@ -1757,17 +1772,22 @@ __device__ void lp_choose(Ctl& c, Rng& g, int lane, int* mode_out, int* nx_out) 
 template <bool FX>
 __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(4))) void k_lp_stage1(LpArgs A) {
   __shared__ Ctl c;
-  // the node's histograms of w and w r over the row segments of each variable (int64 at 2^-20) in kHistCopies copies,
-  // lane l adding into copy l % kHistCopies (fewer lanes serialised on one address); the copies are added after
-  __shared__ unsigned long long H[kHistCopies][FBH_PMAX][kSegs][2];
-  __shared__ uint8_t seg[FBH_PMAX][256];  // segment of each cutpoint rank: the number of positions below it
+  // the node's histograms of w (uint32 at 2^-12) and w r (int64 at 2^-20) over the row bins of each variable, in
+  // kHistCopies copies, lane l adding into copy l % kHistCopies (fewer lanes serialised on one address); the copies
+  // are added after
+  __shared__ unsigned H0[kHistCopies][FBH_PMAX][kSegs];               // w at 2^-12
+  __shared__ unsigned long long H1[kHistCopies][FBH_PMAX][kSegs];     // w r at 2^-20
+  __shared__ uint8_t seg[FBH_PMAX][256];  // bin of each cutpoint rank: the positions below it plus rank / 16
   __shared__ uint16_t rgs[FBH_PMAX];      // cut ranges at the node, L | U << 8
   unsigned char* const dyn = fbh_dyn;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, dw = wave - 1;
   const int chain = blockIdx.x, j = A.j;
   const KArgs& a = A.a;
   img_copy(&c, A.img + chain, tid);
-  for (int k = tid; k < kHistCopies * FBH_PMAX * kSegs * 2; k += kBlock) (&H[0][0][0][0])[k] = 0;
+  for (int k = tid; k < kHistCopies * FBH_PMAX * kSegs; k += kBlock) {
+    (&H0[0][0][0])[k] = 0;
+    (&H1[0][0][0])[k] = 0;
+  }
   __syncthreads();
   const RowPtrs P = lp_rows(a, c.cd);
   const int n = c.cd.n, p = c.cd.p;
@@ -1818,27 +1838,23 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
     for (int e = tid; e < p * 256; e += kBlock) {
       const int v = e >> 8, r = e & 255;
       const int L = rgs[v] & 255, U = rgs[v] >> 8;
-      int sg = 0;
-#pragma unroll
-      for (int k = 0; k < kFeatPos; k++) {
-        const int cp = feat_pos(L, U, k);
-        sg += (cp >= 0 && cp < r);
-      }
-      seg[v][r] = (uint8_t)sg;
+      seg[v][r] = (uint8_t)feat_bin(L, U, r);
     }
     __syncthreads();
     // the node's rows: the leaf nx (birth) or the two children of nx (death)
     const TreeDev& t = c.tree[0];
     const int l = mode == 2 ? t.l[nx] : nx, r = mode == 2 ? t.r[nx] : nx;
     const uint8_t* lc = P.LEAF + (size_t)j * P.ns;
-    auto& Hc = H[lane & (kHistCopies - 1)];
+    auto& H0c = H0[lane & (kHistCopies - 1)];
+    auto& H1c = H1[lane & (kHistCopies - 1)];
     // four consecutive rows per thread and iteration: the byte columns (16-byte aligned, padded to a multiple of 16
     // rows) as one dword each, and the cutpoint ranks of four variables loaded together before their adds, so that
     // an iteration waits for memory a few times instead of once per variable and row
     for (int i4 = 4 * tid; i4 < n; i4 += 4 * kBlock) {
       const uint32_t lw = *reinterpret_cast<const uint32_t*>(lc + i4);
       bool in[4];
-      unsigned long long wf[4], wr[4];
+      unsigned wf[4];
+      unsigned long long wr[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int i = i4 + u;
@@ -1848,7 +1864,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         if (in[u]) {
           const double res = (double)P.Y[i] - (double)(AFX[i] - fit_fx(c.thr[0][lf].th)) * (1.0 / kFxFit);
           const double w = (double)P.W[i];
-          wf[u] = (unsigned long long)(long long)rint(w * kFxFeat);
+          wf[u] = (unsigned)rint(w * kFxW);
           wr[u] = (unsigned long long)(long long)rint(w * res * kFxFeat);
         }
       }
@@ -1865,8 +1881,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
           for (int u = 0; u < 4; u++)
             if (in[u]) {
               const int sg = seg[v0 + k][(qw[k] >> (8 * u)) & 255u];
-              atomicAdd(&Hc[v0 + k][sg][0], wf[u]);
-              atomicAdd(&Hc[v0 + k][sg][1], wr[u]);
+              atomicAdd(&H0c[v0 + k][sg], wf[u]);
+              atomicAdd(&H1c[v0 + k][sg], wr[u]);
             }
         }
       }
@@ -1874,10 +1890,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
   }
   __syncthreads();
   if (mode)
-    for (int k = tid; k < FBH_PMAX * kSegs * 2; k += kBlock) {
-      unsigned long long t = (&H[0][0][0][0])[k];
-      for (int h = 1; h < kHistCopies; h++) t += (&H[h][0][0][0])[k];
-      (&H[0][0][0][0])[k] = t;
+    for (int k = tid; k < FBH_PMAX * kSegs; k += kBlock) {
+      unsigned t0 = (&H0[0][0][0])[k];
+      unsigned long long t1 = (&H1[0][0][0])[k];
+      for (int h = 1; h < kHistCopies; h++) {
+        t0 += (&H0[h][0][0])[k];
+        t1 += (&H1[h][0][0])[k];
+      }
+      (&H0[0][0][0])[k] = t0;
+      (&H1[0][0][0])[k] = t1;
     }
   __syncthreads();
   float* X = A.X + (size_t)chain * kFeat;
@@ -1888,15 +1909,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       const int L = rgs[v] & 255, U = rgs[v] >> 8;
       const int cp = v < p ? feat_pos(L, U, kk) : -1;
       if (cp >= 0) {
-        // left child's sums: the segments up to the number of positions below cp
-        int sk = 0;
-        for (int q = 0; q < kFeatPos; q++) {
-          const int cq = feat_pos(L, U, q);
-          sk += (cq >= 0 && cq < cp);
-        }
+        // left child's sums: the bins up to cp's own (bins are monotone in the rank, and a rank above cp lands in a
+        // later bin: it has cp among the positions below it)
+        const int sk = feat_bin(L, U, cp);
         long long w0 = 0, w1 = 0, t0 = 0, t1 = 0;
         for (int q = 0; q < kSegs; q++) {
-          const long long h0 = (long long)H[0][v][q][0], h1 = (long long)H[0][v][q][1];
+          const long long h0 = (long long)H0[0][v][q], h1 = (long long)H1[0][v][q];
           t0 += h0;
           t1 += h1;
           if (q <= sk) {
@@ -1905,7 +1923,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
           }
         }
         if (t0 > 0) {
-          const double W = (double)t0 * (1.0 / kFxFeat), Sm = (double)t1 * (1.0 / kFxFeat);
+          const double W = (double)t0 * (1.0 / kFxW), Sm = (double)t1 * (1.0 / kFxFeat);
           const double f = fmin(fmax((double)w0 / (double)t0, 1e-6), 1.0 - 1e-6);
           const double z = ((double)w1 * (1.0 / kFxFeat) - f * Sm) / __builtin_sqrt(W * f * (1.0 - f)) *
                            __builtin_sqrt(c.inv_s2);

@@ -173,6 +173,8 @@ def ess_geyer_cols(x: np.ndarray) -> np.ndarray:
     ok = var > 0
     rho = np.where(ok, acov / np.where(ok, var, 1.0), 0.0)
     m = (n - 1) // 2
+    if m < 1:  # fewer than three draws: no pair sums
+        return np.full(k, float(n))
     pairs = rho[0:2 * m:2] + rho[1:2 * m:2]               # [m][k]
     pos = pairs > 0
     first_neg = np.where(pos.all(axis=0), m, np.argmin(pos, axis=0))
@@ -309,6 +311,11 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
         f"posterior-mean correlation with plain {corr:.4f}")
     value = chains * M * total / t_learned
     ess_ps, ess_pp = ess_learned / t_learned, ess_plain / t_plain
+    ess_note = None
+    if ndpost < ESS_MIN_DRAWS:
+        ess_ps = ess_pp = None
+        ess_note = (f"not computed: {ndpost} kept draws per chain; Geyer's estimator needs at least {ESS_MIN_DRAWS} "
+                    f"(bench.py --workload learned512 --ndpost 1000)")
     line = {
         "metric": METRIC, "value": value, "unit": "tree-updates/s", "n_gpus": 1, "steps": total, "warmup": 0,
         "ms_per_step": t_learned / total * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -319,6 +326,7 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
                                f"{args.train_steps} Adam steps, {total - n_plain0 - args.collect} sweeps at alpha "
                                f"{args.alpha}"},
         "min_ess_per_sec": ess_ps,
+        "min_ess_note": ess_note,
         "ndpost": ndpost,
         "plain": {"min_ess_per_sec": ess_pp, "seconds": t_plain, "sum_min_ess": ess_plain, **dist_plain},
         "learned": {"min_ess_per_sec": ess_ps, "seconds_sampling": t_learned, "seconds_training": t_train,
@@ -334,7 +342,7 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
         # the chains' sigma still falls for tens of thousands of sweeps (burn-in): over the kept draws a lower mean
         # is a chain further converged
         "sigma_mean_kept": {"plain": sigma_plain, "learned": sigma_learned},
-        "ess_per_sec_ratio_learned_over_plain": ess_ps / ess_pp if ess_pp else None,
+        "ess_per_sec_ratio_learned_over_plain": ess_ps / ess_pp if ess_ps is not None and ess_pp else None,
         "posterior_mean_corr": corr,
         "roofline": None,
         "cpu_baseline": cpu,

@@ -17,13 +17,13 @@ typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kBM = 64, kBN = 64, kBK = 32;  // block tile; 4 waves of 32 x 32
+constexpr int kBM = 64, kBN = 64, kBK = 64;  // block tile; 4 waves of 32 x 32, four MFMAs per wave and k-step
 constexpr int kLdsK = kBK + 8;                // k stride in LDS (bf16 elements), keeps 16-byte alignment
 
 enum Epi { EPI_RELU_BF16 = 0, EPI_BIAS_F32 = 1, EPI_F32 = 2, EPI_DRELU = 3 };
 
 // C[M x N] = A[M x K] * B[K x N] (+ epilogue). A(m,k) = TA ? A[k*lda + m] : A[m*lda + k];
-// B(k,n) = TB ? B[n*ldb + k] : B[k*ldb + n]. M, N multiples of 64, K a multiple of 32.
+// B(k,n) = TB ? B[n*ldb + k] : B[k*ldb + n]. M, N multiples of 64, K a multiple of 64.
 //   EPI_RELU_BF16: Cb = bf16(relu(acc + bias))        EPI_BIAS_F32: Cf = acc + bias
 //   EPI_F32:       Cf = acc                           EPI_DRELU: g = acc * (aux > 0): Cf = g, Cb = bf16(g)
 template <bool TA, bool TB, int EPI>
@@ -37,25 +37,27 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16* __restrict__ A, const 
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   f32x16 acc = {};
   for (int k0 = 0; k0 < K; k0 += kBK) {
-    // global -> LDS: 64 x 32 of each operand, 8 bf16 per thread, stored [row][k]
-    {
+    // global -> LDS: 64 x 64 of each operand, 16 bf16 per thread (two 8-wide pieces), stored [row][k]
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int t = tid + 256 * h;
       if (!TA) {
-        const int r = tid >> 2, kc = (tid & 3) * 8;
+        const int r = t >> 3, kc = (t & 7) * 8;
         *reinterpret_cast<uint4*>(&As[r][kc]) =
             *reinterpret_cast<const uint4*>(A + (size_t)(m0 + r) * lda + k0 + kc);
       } else {
-        const int kr = tid >> 3, mc = (tid & 7) * 8;
+        const int kr = t >> 3, mc = (t & 7) * 8;
         const uint4 v = *reinterpret_cast<const uint4*>(A + (size_t)(k0 + kr) * lda + m0 + mc);
         const bf16* e = reinterpret_cast<const bf16*>(&v);
 #pragma unroll
         for (int u = 0; u < 8; u++) As[mc + u][kr] = e[u];
       }
       if (TB) {
-        const int r = tid >> 2, kc = (tid & 3) * 8;
+        const int r = t >> 3, kc = (t & 7) * 8;
         *reinterpret_cast<uint4*>(&Bs[r][kc]) =
             *reinterpret_cast<const uint4*>(B + (size_t)(n0 + r) * ldb + k0 + kc);
       } else {
-        const int kr = tid >> 3, nc = (tid & 7) * 8;
+        const int kr = t >> 3, nc = (t & 7) * 8;
         const uint4 v = *reinterpret_cast<const uint4*>(B + (size_t)(k0 + kr) * ldb + n0 + nc);
         const bf16* e = reinterpret_cast<const bf16*>(&v);
 #pragma unroll
@@ -184,19 +186,31 @@ __global__ __launch_bounds__(256) void k_softmax_xent(const float* __restrict__ 
   }
 }
 
-// out[n] = sum over rows m < M of X[m][n] (bias gradients), one column per thread
-__global__ void k_colsum(const float* __restrict__ X, int M, int ld, int N, float* __restrict__ out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// out[n] = sum over rows m < M of X[m][n] (bias gradients): a block takes 32 columns, its 8 row slices (threads
+// 32 s .. 32 s + 31) sum rows s, s + 8, ... of them, and the slices are added in a fixed order
+constexpr int kColW = 32, kColS = 8;
+__global__ __launch_bounds__(kColW * kColS) void k_colsum(const float* __restrict__ X, int M, int ld, int N,
+                                                          float* __restrict__ out) {
+  __shared__ float part[kColS][kColW];
+  const int c = threadIdx.x % kColW, sl = threadIdx.x / kColW;
+  const int n = blockIdx.x * kColW + c;
   float s = 0.f;
-  for (int m = 0; m < M; m++) s += X[(size_t)m * ld + n];
-  out[n] = s;
+  if (n < N)
+    for (int m = sl; m < M; m += kColS) s += X[(size_t)m * ld + n];
+  part[sl][c] = s;
+  __syncthreads();
+  if (sl == 0 && n < N) {
+    float t = part[0][c];
+    for (int k = 1; k < kColS; k++) t += part[k][c];
+    out[n] = t;
+  }
 }
 
-// Adam on fp32 masters (Kingma & Ba 2015), bf16 copy refreshed; bias-corrected step size passed in
+// Adam on fp32 masters (Kingma & Ba 2015), bf16 copies refreshed (wb [rows][cols], and its transpose wbt when given);
+// bias-corrected step size passed in
 __global__ void k_adam(float* __restrict__ w, float* __restrict__ mo, float* __restrict__ ve,
                        const float* __restrict__ g, bf16* __restrict__ wb, long count, float step, float b1, float b2,
-                       float eps) {
+                       float eps, bf16* __restrict__ wbt = nullptr, int rows = 0, int cols = 0) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const float gi = g[i];
@@ -207,11 +221,17 @@ __global__ void k_adam(float* __restrict__ w, float* __restrict__ mo, float* __r
   const float nw = w[i] - step * m / (sqrtf(v) + eps);
   w[i] = nw;
   if (wb) wb[i] = (bf16)nw;
+  if (wbt) wbt[(i % cols) * (long)rows + i / cols] = (bf16)nw;
 }
 
-__global__ void k_to_bf16(const float* __restrict__ x, bf16* __restrict__ y, long count) {
+// x [rows][cols] fp32 -> y (same layout) and yt [cols][rows] in bf16
+__global__ void k_to_bf16(const float* __restrict__ x, bf16* __restrict__ y, bf16* __restrict__ yt, int rows,
+                          int cols) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) y[i] = (bf16)x[i];
+  if (i >= (long)rows * cols) return;
+  const bf16 v = (bf16)x[i];
+  y[i] = v;
+  yt[(i % cols) * (long)rows + i / cols] = v;
 }
 
 // rows of x [M][d_in] (fp32, device) into the zero-padded bf16 batch [Mp][ld]
@@ -253,6 +273,7 @@ struct fbh_mlp {
   // parameters: fp32 masters, Adam moments, bf16 copies, gradients (padded layouts)
   float *W[3] = {}, *Bv[3] = {}, *mW[3] = {}, *vW[3] = {}, *mB[3] = {}, *vB[3] = {}, *gW[3] = {}, *gB[3] = {};
   bf16* Wb[3] = {};
+  bf16* WbT[3] = {};  // transposed bf16 copies [out][in]: the forward's weight operand, contiguous along k
   // activations
   bf16 *Xb = nullptr, *H1 = nullptr, *H2 = nullptr, *dZb = nullptr, *dH2b = nullptr, *dH1b = nullptr;
   float *Z = nullptr, *dZ = nullptr, *dH2 = nullptr, *dH1 = nullptr, *loss = nullptr, *xin = nullptr;
@@ -269,7 +290,7 @@ static void mlp_free(fbh_mlp* n) {
   if (!n) return;
   (void)hipSetDevice(n->device);
   for (int l = 0; l < 3; l++) {
-    void* ps[] = {n->W[l], n->Bv[l], n->mW[l], n->vW[l], n->mB[l], n->vB[l], n->gW[l], n->gB[l], n->Wb[l]};
+    void* ps[] = {n->W[l], n->Bv[l], n->mW[l], n->vW[l], n->mB[l], n->vB[l], n->gW[l], n->gB[l], n->Wb[l], n->WbT[l]};
     for (void* q : ps)
       if (q) (void)hipFree(q);
   }
@@ -334,7 +355,8 @@ extern "C" int fbh_mlp_create(const int* dims, int max_batch, unsigned seed, int
     n->bcount[l] = n->p[l + 1];
     if ((rc = zalloc(&n->W[l], n->wcount[l])) || (rc = zalloc(&n->mW[l], n->wcount[l])) ||
         (rc = zalloc(&n->vW[l], n->wcount[l])) || (rc = zalloc(&n->gW[l], n->wcount[l])) ||
-        (rc = zalloc(&n->Wb[l], n->wcount[l])) || (rc = zalloc(&n->Bv[l], n->bcount[l])) ||
+        (rc = zalloc(&n->Wb[l], n->wcount[l])) || (rc = zalloc(&n->WbT[l], n->wcount[l])) ||
+        (rc = zalloc(&n->Bv[l], n->bcount[l])) ||
         (rc = zalloc(&n->mB[l], n->bcount[l])) || (rc = zalloc(&n->vB[l], n->bcount[l])) ||
         (rc = zalloc(&n->gB[l], n->bcount[l]))) {
       mlp_free(n);
@@ -386,7 +408,7 @@ extern "C" int fbh_mlp_params(fbh_mlp* n, float* get, const float* set) {
       for (int r = 0; r < rows; r++) std::memcpy(buf.data() + (size_t)r * ld, set + o + (size_t)r * cols, 4 * cols);
       FBH_HIP_CHECK(hipMemcpy(n->W[l], buf.data(), sizeof(float) * buf.size(), hipMemcpyHostToDevice));
       hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((n->wcount[l] + 255) / 256)), dim3(256), 0, n->stream, n->W[l],
-                         n->Wb[l], n->wcount[l]);
+                         n->Wb[l], n->WbT[l], n->p[l], n->p[l + 1]);
       FBH_HIP_CHECK(hipGetLastError());
     }
     o += (size_t)rows * cols;
@@ -416,18 +438,20 @@ extern "C" int fbh_mlp_params(fbh_mlp* n, float* get, const float* set) {
 }
 
 namespace fbh {
-// forward of rows [0, M) of the device batch xin (fp32 [M][d_in]) into n->Z (fp32 logits [Mp][p3])
+// forward of rows [0, M) of the device batch xin (fp32 [M][d_in]) into n->Z (fp32 logits [Mp][p3]): the input packed to
+// the padded bf16 batch n->Xb (also the backward's first operand), then the three layers with the weights read through
+// their transposed bf16 copies (contiguous along k, so every operand tile is loaded with 16-byte vectors)
 int mlp_forward_dev(fbh_mlp* n, const float* xin_dev, int M, hipStream_t s) {
   const int Mp = up64(M);
   const int* p = n->p;
   hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)(((long)Mp * p[0] + 255) / 256)), dim3(256), 0, s, xin_dev, M,
                      n->d[0], n->Xb, Mp, p[0]);
-  hipLaunchKernelGGL((k_gemm<false, false, EPI_RELU_BF16>), dim3(p[1] / kBN, Mp / kBM), dim3(256), 0, s, n->Xb,
-                     n->Wb[0], p[0], p[0], p[1], nullptr, n->H1, p[1], n->Bv[0], nullptr);
-  hipLaunchKernelGGL((k_gemm<false, false, EPI_RELU_BF16>), dim3(p[2] / kBN, Mp / kBM), dim3(256), 0, s, n->H1,
-                     n->Wb[1], p[1], p[1], p[2], nullptr, n->H2, p[2], n->Bv[1], nullptr);
-  hipLaunchKernelGGL((k_gemm<false, false, EPI_BIAS_F32>), dim3(p[3] / kBN, Mp / kBM), dim3(256), 0, s, n->H2,
-                     n->Wb[2], p[2], p[2], p[3], n->Z, nullptr, p[3], n->Bv[2], nullptr);
+  hipLaunchKernelGGL((k_gemm<false, true, EPI_RELU_BF16>), dim3(p[1] / kBN, Mp / kBM), dim3(256), 0, s, n->Xb,
+                     n->WbT[0], p[0], p[0], p[0], nullptr, n->H1, p[1], n->Bv[0], nullptr);
+  hipLaunchKernelGGL((k_gemm<false, true, EPI_RELU_BF16>), dim3(p[2] / kBN, Mp / kBM), dim3(256), 0, s, n->H1,
+                     n->WbT[1], p[1], p[1], p[1], nullptr, n->H2, p[2], n->Bv[1], nullptr);
+  hipLaunchKernelGGL((k_gemm<false, true, EPI_BIAS_F32>), dim3(p[3] / kBN, Mp / kBM), dim3(256), 0, s, n->H2,
+                     n->WbT[2], p[2], p[2], p[2], n->Z, nullptr, p[3], n->Bv[2], nullptr);
   FBH_HIP_CHECK(hipGetLastError());
   return FBH_OK;
 }
@@ -465,21 +489,24 @@ int mlp_backward_dev(fbh_mlp* n, int M, hipStream_t s, bool soft) {
   int rc = mlp_loss_dev(n, M, soft, s);
   if (rc) return rc;
   // layer 3
-  hipLaunchKernelGGL((k_gemm<true, false, EPI_F32>), dim3(p[3] / kBN, p[2] / kBM), dim3(256), 0, s, n->H2, n->dZb,
-                     Mp, p[2], p[3], n->gW[2], nullptr, p[3], nullptr, nullptr);
-  hipLaunchKernelGGL(k_colsum, dim3((p[3] + 255) / 256), dim3(256), 0, s, n->dZ, Mp, p[3], p[3], n->gB[2]);
-  hipLaunchKernelGGL((k_gemm<false, true, EPI_DRELU>), dim3(p[2] / kBN, Mp / kBM), dim3(256), 0, s, n->dZb, n->Wb[2],
-                     p[3], p[3], p[3], n->dH2, n->dH2b, p[2], nullptr, n->H2);
+  hipLaunchKernelGGL((k_gemm<true, false, EPI_F32>), dim3(p[3] / kBN, p[2] / kBM), dim3(256), 0, s,
+                     n->H2, n->dZb, Mp, p[2], p[3], n->gW[2], nullptr, p[3], nullptr, nullptr);
+  hipLaunchKernelGGL(k_colsum, dim3((p[3] + kColW - 1) / kColW), dim3(kColW * kColS), 0, s, n->dZ, Mp, p[3], p[3],
+                     n->gB[2]);
+  hipLaunchKernelGGL((k_gemm<false, true, EPI_DRELU>), dim3(p[2] / kBN, Mp / kBM), dim3(256), 0, s,
+                     n->dZb, n->Wb[2], p[3], p[3], p[3], n->dH2, n->dH2b, p[2], nullptr, n->H2);
   // layer 2
-  hipLaunchKernelGGL((k_gemm<true, false, EPI_F32>), dim3(p[2] / kBN, p[1] / kBM), dim3(256), 0, s, n->H1, n->dH2b,
-                     Mp, p[1], p[2], n->gW[1], nullptr, p[2], nullptr, nullptr);
-  hipLaunchKernelGGL(k_colsum, dim3((p[2] + 255) / 256), dim3(256), 0, s, n->dH2, Mp, p[2], p[2], n->gB[1]);
-  hipLaunchKernelGGL((k_gemm<false, true, EPI_DRELU>), dim3(p[1] / kBN, Mp / kBM), dim3(256), 0, s, n->dH2b,
-                     n->Wb[1], p[2], p[2], p[2], n->dH1, n->dH1b, p[1], nullptr, n->H1);
+  hipLaunchKernelGGL((k_gemm<true, false, EPI_F32>), dim3(p[2] / kBN, p[1] / kBM), dim3(256), 0, s,
+                     n->H1, n->dH2b, Mp, p[1], p[2], n->gW[1], nullptr, p[2], nullptr, nullptr);
+  hipLaunchKernelGGL(k_colsum, dim3((p[2] + kColW - 1) / kColW), dim3(kColW * kColS), 0, s, n->dH2, Mp, p[2], p[2],
+                     n->gB[1]);
+  hipLaunchKernelGGL((k_gemm<false, true, EPI_DRELU>), dim3(p[1] / kBN, Mp / kBM), dim3(256), 0, s,
+                     n->dH2b, n->Wb[1], p[2], p[2], p[2], n->dH1, n->dH1b, p[1], nullptr, n->H1);
   // layer 1
-  hipLaunchKernelGGL((k_gemm<true, false, EPI_F32>), dim3(p[1] / kBN, p[0] / kBM), dim3(256), 0, s, n->Xb, n->dH1b,
-                     Mp, p[0], p[1], n->gW[0], nullptr, p[1], nullptr, nullptr);
-  hipLaunchKernelGGL(k_colsum, dim3((p[1] + 255) / 256), dim3(256), 0, s, n->dH1, Mp, p[1], p[1], n->gB[0]);
+  hipLaunchKernelGGL((k_gemm<true, false, EPI_F32>), dim3(p[1] / kBN, p[0] / kBM), dim3(256), 0, s,
+                     n->Xb, n->dH1b, Mp, p[0], p[1], n->gW[0], nullptr, p[1], nullptr, nullptr);
+  hipLaunchKernelGGL(k_colsum, dim3((p[1] + kColW - 1) / kColW), dim3(kColW * kColS), 0, s, n->dH1, Mp, p[1], p[1],
+                     n->gB[0]);
   FBH_HIP_CHECK(hipGetLastError());
   return FBH_OK;
 }
@@ -491,9 +518,10 @@ int mlp_adam_dev(fbh_mlp* n, float lr, hipStream_t s) {
                                   (1.0 - std::pow((double)b1, n->adam_t)));
   for (int l = 0; l < 3; l++) {
     hipLaunchKernelGGL(k_adam, dim3((unsigned)((n->wcount[l] + 255) / 256)), dim3(256), 0, s, n->W[l], n->mW[l],
-                       n->vW[l], n->gW[l], n->Wb[l], n->wcount[l], step, b1, b2, eps);
+                       n->vW[l], n->gW[l], n->Wb[l], n->wcount[l], step, b1, b2, eps, n->WbT[l], n->p[l],
+                       n->p[l + 1]);
     hipLaunchKernelGGL(k_adam, dim3((unsigned)((n->bcount[l] + 255) / 256)), dim3(256), 0, s, n->Bv[l], n->mB[l],
-                       n->vB[l], n->gB[l], (bf16*)nullptr, n->bcount[l], step, b1, b2, eps);
+                       n->vB[l], n->gB[l], (bf16*)nullptr, n->bcount[l], step, b1, b2, eps, (bf16*)nullptr, 0, 0);
   }
   FBH_HIP_CHECK(hipGetLastError());
   return FBH_OK;

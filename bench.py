@@ -7,7 +7,9 @@ n_c ~ U{3091..4573} rows, p_c ~ U{15..19} predictors, x ~ U(0,1) with 255 cutpoi
 20 (x3-0.5)^2 + 10 x4 + 5 x5 + N(0, s_c^2), s_c ~ U(0.5, 2), w ~ U(0.5, 2); ntree 200, nskip 1000, ndpost 100.
 A step is one sweep of every chain (m tree updates per chain plus the closing pass and the sigma / latent draw). W
 warmup sweeps run untimed, then exactly K sweeps are timed; W + K covers the whole fit by default, so the kept draws of
-the timed region give the min-ESS figure. Inputs are resident in HBM before the timed region starts.
+the timed region give the min-ESS figure. Inputs are resident in HBM before the timed region starts. After the timed
+region (nothing of it enters value) the headline line also carries the metric's second half, `learned_vs_plain`: the
+learned512 comparison at 1,000 kept draws on the same chains (about a minute; --no-learned skips it).
 
 Other workloads (extra lines, not the headline):
   fwbart_n4096   one chain, n = 4,096, p = 16, 100 cutpoints, ntree 200, nskip 1000, ndpost 100
@@ -15,9 +17,10 @@ Other workloads (extra lines, not the headline):
   maxshape64     64 chains, n = 65,536, p = 32 (24 continuous + 8 integer columns), node cap 64, FX64 statistics,
                  ntree 200, nskip 500, ndpost 100 (default here: 5 warmup + 40 timed sweeps)
   learned512     the 512-chain batch twice: the plain fit, then the learned schedule -- plain sweeps, then sweeps with
-                 BART's proposal in the lock-step pipeline collecting accepted births, training of the network on the
-                 device, then sweeps with the mixture 0.2 uniform / 0.8 network up to the end (the kept draws); reports
-                 min-ESS/s of both (training time apart) and the posterior-mean agreement
+                 BART's proposal in the lock-step pipeline collecting proposed births with the exact split conditional
+                 as the training target, training of the network on the device, then sweeps with the mixture
+                 0.2 uniform / 0.8 network up to the end (the kept draws); reports min-ESS/s of both (training time
+                 apart), the network's held-out loss and the posterior-mean agreement
   predict600     predict_draw: 6 fits of the batched workload (200 trees, 100 kept draws each), 600 draws on 5,237
                  new rows; value in row-trees/s (a step is one pass over all 600 draws)
   predict_probit100  predict_draw for the 100 draws of the fpbart fit on its 24,576 rows (50 trees)
@@ -224,6 +227,35 @@ def measured_traffic(workload: str, sweeps_per_launch: float):
 
 # ------------------------------------------------------------------ learned vs plain
 def learned_vs_plain(args, chains, nskip, ndpost, cpu):
+    """Plain fit and learned schedule on the same chains and seeds; prints the learned512 line."""
+    print(json.dumps(learned_compare(args, chains, nskip, ndpost, cpu)), flush=True)
+    return 0
+
+
+def learned_summary(line):
+    """The learned-vs-plain figures the headline line carries (the learned512 line has the rest)."""
+    L, P = line["learned"], line["plain"]
+    keys = ("ess_sigma_median", "ess_f_p5_median", "ess_f_median_median", "ess_f_min_median", "sum_min_ess")
+    return {
+        "what": (f"the learned birth proposal against the plain GPU sampler on the same {line['config']['chains']} "
+                 f"chains and seeds, {line['ndpost']} kept draws each (bench.py --workload learned512 "
+                 f"--ndpost {line['ndpost']} prints the whole line)"),
+        "schedule": line["config"]["schedule"],
+        "plain_min_ess_per_sec": P["min_ess_per_sec"], "learned_min_ess_per_sec": L["min_ess_per_sec"],
+        "ratio": line["ess_per_sec_ratio_learned_over_plain"], "min_ess_note": line["min_ess_note"],
+        "posterior_mean_corr": line["posterior_mean_corr"],
+        "seconds": {"plain": P["seconds"], "learned_sampling": L["seconds_sampling"],
+                    "learned_training": L["seconds_training"]},
+        "ms_per_learned_sweep": L["ms_per_learned_sweep"],
+        "heldout_loss_nats": {"network": L["heldout_loss"], "untrained": L["heldout_loss_untrained"],
+                              "floor_entropy": L["heldout_entropy_floor"], "uniform_rule": L["heldout_uniform_rule"]},
+        "accepted_births": {"plain": line["plain_births"], "learned": L["births"]},
+        "sigma_mean_kept": line["sigma_mean_kept"],
+        "ess_per_chain_median": {"plain": {k: P[k] for k in keys}, "learned": {k: L[k] for k in keys}},
+    }
+
+
+def learned_compare(args, chains, nskip, ndpost, cpu):
     """Plain fit and learned schedule on the same chains and seeds; min-ESS/s of both over the kept draws."""
     from fastbart_amd import sampler
     from fastbart_amd.mlp import MLP
@@ -347,8 +379,7 @@ def learned_vs_plain(args, chains, nskip, ndpost, cpu):
         "roofline": None,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
 # ------------------------------------------------------------------ prediction (compile_trees / predict_draw)
@@ -574,6 +605,10 @@ def main() -> int:
     ap.add_argument("--sweeps-per-launch", type=int, default=20,
                     help="sweeps per persistent launch (the warmup and the timed sweeps split into equal launches)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="batch512: skip the learned-vs-plain comparison the headline line carries")
+    ap.add_argument("--learned-ndpost", type=int, default=1000,
+                    help="batch512: kept draws of the learned-vs-plain comparison")
     ap.add_argument("--ess-rows", type=int, default=0, help="rows per chain used for ESS (0 = all)")
     ap.add_argument("--row-split", type=int, default=None,
                     help="workgroups per chain for FX64 workloads (-1 = one workgroup per CU; default: -1 for "
@@ -743,6 +778,17 @@ def main() -> int:
     }
     if cpu:
         line["gpu_over_cpu"] = value / cpu["value"]
+    if wl == "batch512" and not args.no_learned:
+        # the metric's second half: min-ESS/s of the learned proposal against the plain sampler, on the same chains
+        # (after the timed region; nothing of it enters value)
+        try:
+            rows_keep = args.ess_rows
+            args.ess_rows = args.ess_rows or 400
+            lv = learned_compare(args, chains, NSKIP, args.learned_ndpost, None)
+            args.ess_rows = rows_keep
+            line["learned_vs_plain"] = learned_summary(lv)
+        except Exception as e:  # the headline line is printed whatever happens here
+            line["learned_vs_plain"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(line), flush=True)
     return 0
 

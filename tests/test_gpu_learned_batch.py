@@ -159,3 +159,27 @@ def test_gpu_trained_network_gains_over_alpha0():
     assert loss < uniform - 0.3 and loss > floor
     assert births_l > 1.1 * births_p
     assert z < -3
+
+
+def test_gpu_learned_pipeline_fx64_alpha0_equals_the_oracle(oracle):
+    """The pipeline's FX64 instantiation (k_lp_stage1 / stage2 / close with int64 fixed-point statistics): with
+    alpha = 0 it draws the plain proposal's generator stream, so its outputs equal the oracle's FX64 twin bit for bit,
+    alone and after plain persistent sweeps, also while it collects pairs with the conditional target."""
+    probs = [W.small_problem(n, 4, 25, 77 + n, "uniform", ntree=20) for n in (700, 1333, 2048)]
+    seeds = [5, 6, 7]
+    m, nskip, nd = 20, 8, 4
+    for first_plain in (0, 3):
+        _say("FX64 learned alpha 0 after", first_plain, "plain sweeps")
+        with sampler.BatchSampler(probs, m, nskip, nd, seeds, stats_mode=1) as b:
+            if first_plain:
+                b.run(first_plain)
+            net = MLP((513, 256, 256, 8160), max_batch=512, seed=5)
+            b.learned_setup(net, 0.0, pair_capacity=20000)
+            b.learned_target(1, seed=3)
+            b.run_learned(nskip + nd - first_plain)
+            outs = [b.fetch(c) for c in range(len(probs))]
+            assert b.learned_pairs() > 0
+        for c, pb in enumerate(probs):
+            e = oracle.fit(pb, m, nskip, nd, seeds[c], stats_mode=1)
+            for k in ("sigma", "trmean", "trees", "stats"):
+                assert same_bits(outs[c][k], e[k]), (first_plain, c, k)

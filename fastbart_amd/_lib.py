@@ -34,7 +34,7 @@ class Opts(C.Structure):
                 ("keep_trees", C.c_int), ("pgtab", C.c_void_p), ("stats_mode", C.c_int), ("node_cap", C.c_int),
                 ("device", C.c_int), ("sweeps_per_launch", C.c_int), ("block_threads", C.c_int),
                 ("tile_rows", C.c_int), ("row_split", C.c_int), ("packing", C.c_int), ("graph", C.c_int),
-                ("chains_per_cu", C.c_int), ("resident", C.c_int)]
+                ("chains_per_cu", C.c_int), ("resident", C.c_int), ("probit_redraw", C.c_int)]
 
 
 TUNE_MAX = 12
@@ -83,7 +83,7 @@ EXPORTS = ["bart_fit_cpu", "bart_wfit", "bart_set_rng", "fbh_last_status", "fbh_
            "fbh_predict_draws", "fbh_rows_upload", "fbh_rows_free", "fbh_predict_rows", "fbh_predict_timing",
            "fbh_predict_rows_device", "fbh_predict_rows_device_noalloc", "fbh_predict_last_launch", "fbh_trees_rank_form", "fbh_device_alloc", "fbh_device_free",
            "fbh_device_copy", "fbh_device_sync", "fbh_batch_row_split", "fbh_batch_configure", "fbh_batch_config",
-           "fbh_batch_set_order",
+           "fbh_batch_set_order", "fbh_batch_probit_redraw", "fbh_batch_probit_info", "fbh_debug_jump",
            "fbh_batch_autotune", "fbh_batch_autotune_cached",
            "fbh_debug_math", "fbh_debug_libm_host", "fbh_debug_copy", "fbh_batch_trace", "fbh_version",
            "fbh_mlp_create", "fbh_mlp_nparams", "fbh_mlp_params", "fbh_mlp_forward", "fbh_mlp_grad", "fbh_mlp_train",
@@ -148,6 +148,9 @@ def _declare(L):
     L.fbh_batch_row_split.argtypes = [vp]
     L.fbh_batch_configure.argtypes = [vp, i, i, i, i]
     L.fbh_batch_set_order.argtypes = [vp, vp]
+    L.fbh_batch_probit_redraw.argtypes = [vp, i, i]
+    L.fbh_batch_probit_info.argtypes = [vp, C.POINTER(i), C.POINTER(i), C.POINTER(C.c_ulonglong)]
+    L.fbh_debug_jump.argtypes = [vp, C.c_ulonglong, vp]
     L.fbh_batch_config.argtypes = [vp] + [C.POINTER(i)] * 5
     L.fbh_batch_autotune.argtypes = [vp, i, C.POINTER(Tune)]
     L.fbh_batch_autotune_cached.argtypes = [vp, i, C.c_char_p, C.POINTER(Tune)]

@@ -72,6 +72,7 @@ struct KArgs {
   unsigned long long* xbuf;  // row split: [group][parity][member][kXw] pass totals
   uint32_t* xctr;            // row split: [group] arrivals
   const int* order;          // [C] device chain run by workgroup w of the sweep kernel
+  int pr_ext;                // probit: the latents are redrawn by the k_pr_* kernels after the launch (one sweep)
 };
 
 struct Rng {
@@ -1449,7 +1450,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
         sigma = __builtin_sqrt(((double)nul + c.rss) / (2.0 * rgamma_dev(g, (double)nnu / 2.0, tb)));
       }
       if (lane == 0) a.sigma_out[c.cd.sig_off + it] = (float)sigma;
-      if (PB) {
+      if (PB && !a.pr_ext) {
         __syncthreads();
         probit_latents<RES>(n, (double)c.cd.offset, P.AF, a.af_off, a.ybin + c.cd.row_off, a.Y + c.cd.row_off, tb,
                             ufl(c.pst_[0]), ufl(c.pst_[1]), ufl(c.pst_[2]), ufl(c.pst_[3]), ufl(c.rot[0]),
@@ -1495,7 +1496,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       }
       data_pass<R, RES, FX, D, true>(c, dyn, P, dw, lane, m - 1, m, cb, true, false, keep, T, pre);
       __syncthreads();
-      if (PB) {
+      if (PB && !a.pr_ext) {
         __syncthreads();
         probit_latents<RES>(n, (double)c.cd.offset, P.AF, a.af_off, a.ybin + c.cd.row_off, a.Y + c.cd.row_off, tb,
                             ufl(c.pst_[0]), ufl(c.pst_[1]), ufl(c.pst_[2]), ufl(c.pst_[3]), ufl(c.rot[0]),
@@ -1554,6 +1555,517 @@ __global__ __launch_bounds__(256) void k_init(KArgs a, const unsigned* seeds) {
 __global__ void k_trmean(const double* TM, float* out, int n, int ndpost) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (float)TM[i] / (float)ndpost;
+}
+
+// ------------------------------------------------------------------ probit latent redraw spread over the device
+// The draws of probit_latents (:543-547, rtnorm :282-292) for batches of few chains, where one workgroup per chain
+// would leave the other CUs idle. Row i's attempts use slots S_i, S_i + 1, ... of the stream (two outputs a slot);
+// with d_i = S_i - i the rejections before row i, d is a monotone path: row i accepts at the first offset d >= d_i
+// whose attempt accepts, and d_{i+1} is that offset. Kernels after the sweep kernel (which then skips its own
+// redraw), per sweep:
+//   k_pr_rows, k_pr_band, k_pr_bandrows: per row, the expected rejections and their variance (geometric, from the
+//                row's truncation point); prefix sums (per 256 rows, over those, then within them) give the predicted
+//                offset E_i and variance V_i of every row, and its band of offsets [lo_i, lo_i + 64 nw_i), centred on
+//                E_i, half-width 4.5 sqrt(V_i) + 64;
+//   k_pr_gen:    the stream's first K slots: thread t jumps to slot 64 t with the GF(2) matrices T^(128 2^k) of the
+//                generator's state transition (T is linear over GF(2)^128) and steps through 64 slots;
+//   k_pr_decide: one wave per row decides every offset of its band (latent_accepts: device math, the host-libm
+//                images wherever rounding could matter -- each bit is the reference's decision) into 64-bit masks;
+//   k_pr_table:  per block of 64 rows and every entry offset in its first row's band, the offset at which the
+//                block's last row accepts, following the decision bits;
+//   k_pr_chain:  one wave per chain chains the blocks through those tables (the entry offset of every block) and
+//                leaves the generator state after the last row; a block whose path leaves the tables is walked;
+//   k_pr_walk:   one wave per block follows the path from its entry offset through the masks (a ballot resolves
+//                the run of rows that accept at the current offset) to each row's accepting slot; an offset outside
+//                a row's band, or a slot past K, is decided on the spot (exact; outputs past K are stepped from the
+//                last checkpoint);
+//   k_pr_value:  each row's latent from its accepting slot (latent_value).
+// The bands only decide how much is precomputed: results are the same whatever they are.
+constexpr int kJumpLevels = 20;     // thread indices below 2^20: K below 2^26 slots
+constexpr int kPrSlots = 64;        // slots per generating thread (and between checkpoints)
+
+struct PrArgs {
+  const ChainDesc* desc;
+  ChainState* state;
+  const double* AF;
+  const uint8_t* ybin;
+  double* Y;
+  LibmTables tb;
+  const uint4* jump;          // [kJumpLevels][128]: column j of T^(128 2^k), the state 128 2^k steps after unit state j
+  uint2* slots;               // [C][K] the two outputs of every slot
+  uint4* ck;                  // [C][K / 64] the generator state before slot 64 t
+  float2* ev;                 // [rows] expected rejections of the row's attempts and their variance
+  double2* btot;              // [C][nb256] their totals per 256 rows, then the prefix before each 256 rows
+  float* E;                   // [rows] predicted offset at each row
+  int* lo;                    // [rows] first offset of the row's band
+  uint8_t* nw;                // [rows] 64-offset words in the band
+  unsigned long long* mask;   // [rows][nwmax] decisions: bit b of word w = the attempt at offset lo + 64 w + b accepts
+  uint32_t* acc;              // [rows] accepting slot
+  int* tab;                   // [C][nbmax][wmax] per block of 64 rows: exit offset for each entry in its band, or -1
+  uint32_t* entry;            // [C][nbmax] the offset at each block's first row
+  unsigned* spot;             // [C] offsets the walk decided on the spot (outside the bands)
+  uint32_t K;                 // slots per chain, a multiple of 64
+  int nwmax;
+  int nbmax, wmax;            // blocks of 64 rows of the longest chain; 64 nwmax
+  int nb256;                  // blocks of 256 rows of the longest chain
+};
+
+// the generator state before slot S: its checkpoint (the last one when S is past K) stepped forward
+__device__ __forceinline__ Rng pr_state_at(const uint4* CK, uint32_t K, uint32_t S) {
+  const uint32_t q = S < K ? S / kPrSlots : K / kPrSlots - 1;
+  const uint4 c = CK[q];
+  Rng g{c.x, c.y, c.z, c.w};
+  for (uint32_t i = q * kPrSlots; i < S; i++) {
+    rng_step(g, 11u, 21u);
+    rng_step(g, 11u, 21u);
+  }
+  return g;
+}
+// the two outputs of slot k
+__device__ __forceinline__ uint2 pr_slot(const uint2* X, const uint4* CK, uint32_t K, uint32_t k) {
+  if (k < K) return X[k];
+  Rng g = pr_state_at(CK, K, k);
+  const uint32_t x1 = starstar(g.s1);
+  rng_step(g, 11u, 21u);
+  return make_uint2(x1, starstar(g.s1));
+}
+
+// expected rejections of a row's attempts and their variance: the attempt rejects with probability rho -- Phi(tau)
+// for the normal draw (tau <= 0), else one less the exponential proposal's acceptance sqrt(2 pi) lam
+// exp(lam tau - lam^2 / 2) Q(tau) -- so the rejections are geometric: rho / (1 - rho), rho / (1 - rho)^2
+__device__ __forceinline__ void pr_rejections(const LatentRow& r, double& e, double& v) {
+  double rho = 0.;
+  if (r.tau <= 0.)
+    rho = 0.5 * erfc(-r.tau * 0.70710678118654752);
+  else if (r.tau < 26.)
+    rho = 1.0 - 1.2533141373155003 * r.lam * exp(r.lam * r.tau - 0.5 * r.lam * r.lam) *
+                    erfc(r.tau * 0.70710678118654752);
+  rho = fmin(fmax(rho, 0.), 0.9);
+  const double a = 1.0 / (1.0 - rho);
+  e = rho * a;
+  v = rho * a * a;
+}
+
+// exclusive prefix sums of (e, v) over a workgroup of 256 threads; returns the workgroup's totals
+__device__ __forceinline__ double2 pr_block_scan(double e, double v, double& pe, double& pv) {
+  __shared__ double2 wsum[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double ie = e, iv = v;  // inclusive within the wave
+  for (int o = 1; o < 64; o <<= 1) {
+    const double te = __shfl_up(ie, o), tv = __shfl_up(iv, o);
+    if (lane >= o) {
+      ie += te;
+      iv += tv;
+    }
+  }
+  if (lane == 63) wsum[wave] = make_double2(ie, iv);
+  __syncthreads();
+  double be = 0., bv = 0.;
+  for (int k = 0; k < wave; k++) {
+    be += wsum[k].x;
+    bv += wsum[k].y;
+  }
+  pe = be + ie - e;
+  pv = bv + iv - v;
+  const double2 tot = make_double2(wsum[0].x + wsum[1].x + wsum[2].x + wsum[3].x,
+                                   wsum[0].y + wsum[1].y + wsum[2].y + wsum[3].y);
+  return tot;
+}
+
+// per row: expected rejections and variance; per 256 rows: their totals
+__global__ __launch_bounds__(256) void k_pr_rows(PrArgs A) {
+  const int c = blockIdx.y;
+  const ChainDesc& d = A.desc[c];
+  if ((int)blockIdx.x * 256 >= d.n) return;
+  const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  const uint64_t ro = d.row_off;
+  double e = 0., v = 0.;
+  if (i < d.n) {
+    pr_rejections(latent_row(A.AF[ro + i], A.ybin[ro + i], (double)d.offset), e, v);
+    A.ev[ro + i] = make_float2((float)e, (float)v);
+  }
+  double pe, pv;
+  const double2 tot = pr_block_scan((double)(float)e, (double)(float)v, pe, pv);
+  if (threadIdx.x == 0) A.btot[(size_t)c * A.nb256 + blockIdx.x] = tot;
+}
+
+// per chain: exclusive prefix sums of the 256-row totals
+__global__ __launch_bounds__(256) void k_pr_band(PrArgs A) {
+  const int c = blockIdx.x;
+  const int nb = (A.desc[c].n + 255) / 256;
+  double2* T = A.btot + (size_t)c * A.nb256;
+  double ce = 0., cv = 0.;
+  for (int b0 = 0; b0 < nb; b0 += 256) {
+    const int b = b0 + (int)threadIdx.x;
+    const double2 t = b < nb ? T[b] : make_double2(0., 0.);
+    double pe, pv;
+    const double2 tot = pr_block_scan(t.x, t.y, pe, pv);
+    __syncthreads();
+    if (b < nb) T[b] = make_double2(ce + pe, cv + pv);
+    ce += tot.x;
+    cv += tot.y;
+  }
+}
+
+// per row: the predicted offset (its 256 rows' offset + the prefix within them) and the band around it
+__global__ __launch_bounds__(256) void k_pr_bandrows(PrArgs A) {
+  const int c = blockIdx.y;
+  const ChainDesc& d = A.desc[c];
+  if ((int)blockIdx.x * 256 >= d.n) return;
+  const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  const uint64_t ro = d.row_off;
+  const float2 x = i < d.n ? A.ev[ro + i] : make_float2(0.f, 0.f);
+  double pe, pv;
+  (void)pr_block_scan((double)x.x, (double)x.y, pe, pv);
+  if (i >= d.n) return;
+  const double2 b = A.btot[(size_t)c * A.nb256 + blockIdx.x];
+  pe += b.x;
+  pv += b.y;
+  const double h = 4.5 * sqrt(pv) + 64.;
+  const int w = min(A.nwmax, max(3, (int)ceil(h / 32.)));
+  A.E[ro + i] = (float)pe;
+  A.lo[ro + i] = max(0, (int)floor(pe - 32. * w));
+  A.nw[ro + i] = (uint8_t)w;
+}
+
+__global__ __launch_bounds__(256) void k_pr_gen(PrArgs A) {
+  const int c = blockIdx.y;
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t nth = A.K / kPrSlots;  // generating threads of a chain
+  // levels used by any thread of the chain: the bits of nth - 1; their tables staged in LDS (read as broadcasts)
+  const int lev = nth > 1 ? 32 - __builtin_clz(nth - 1) : 0;
+  __shared__ uint4 J[kJumpLevels][128];
+  for (int k = threadIdx.x; k < lev * 128; k += 256) J[k / 128][k % 128] = A.jump[k];
+  __syncthreads();
+  if (t >= nth) return;
+  const ChainState* cs = A.state + c;
+  uint32_t s0 = cs->rng[0], s1 = cs->rng[1], s2 = cs->rng[2], s3 = cs->rng[3];
+  for (int k = 0; k < lev; k++) {
+    uint32_t o0 = 0, o1 = 0, o2 = 0, o3 = 0;
+    // columns in groups of 16, every read of a group issued before its use (one wave per SIMD: nothing else hides
+    // the LDS latency)
+#pragma unroll
+    for (int g = 0; g < 8; g++) {
+      uint4 col[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) col[q] = J[k][16 * g + q];
+      const uint32_t x = (g >> 1) == 0 ? s0 : ((g >> 1) == 1 ? s1 : ((g >> 1) == 2 ? s2 : s3));
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const uint32_t msk = 0u - ((x >> (16 * (g & 1) + q)) & 1u);
+        o0 ^= col[q].x & msk;
+        o1 ^= col[q].y & msk;
+        o2 ^= col[q].z & msk;
+        o3 ^= col[q].w & msk;
+      }
+    }
+    const bool take = (t >> k) & 1u;
+    s0 = take ? o0 : s0;
+    s1 = take ? o1 : s1;
+    s2 = take ? o2 : s2;
+    s3 = take ? o3 : s3;
+  }
+  Rng g{s0, s1, s2, s3};
+  A.ck[(size_t)c * nth + t] = make_uint4(g.s0, g.s1, g.s2, g.s3);
+  uint2* out = A.slots + (size_t)c * A.K + (size_t)t * kPrSlots;
+  for (int k = 0; k < kPrSlots; k++) {
+    const uint32_t x1 = starstar(g.s1);
+    rng_step(g, 11u, 21u);
+    const uint32_t x2 = starstar(g.s1);
+    rng_step(g, 11u, 21u);
+    out[k] = make_uint2(x1, x2);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pr_decide(PrArgs A) {
+  const int c = blockIdx.y, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const ChainDesc& d = A.desc[c];
+  const int i = (int)blockIdx.x * 4 + wave;
+  if (i >= d.n) return;
+  const uint64_t ro = d.row_off;
+  const LatentRow r = latent_row(A.AF[ro + i], A.ybin[ro + i], (double)d.offset);
+  const int lo = A.lo[ro + i], nw = A.nw[ro + i];
+  const uint2* X = A.slots + (size_t)c * A.K;
+  unsigned long long* M = A.mask + (ro + i) * (uint64_t)A.nwmax;
+  for (int w = 0; w < nw; w++) {
+    const uint32_t slot = (uint32_t)(i + lo + 64 * w + lane);
+    bool ok = false;
+    if (slot < A.K) {
+      const uint2 x = X[slot];
+      ok = latent_accepts(r, x.x, x.y, A.tb);
+    }
+    const unsigned long long b = __ballot(ok);
+    if (lane == 0) M[w] = b;
+  }
+}
+
+// A lane's view of its row in a window of 64 rows, loaded in two stages one window apart (the second stage's loads
+// depend on the first's): the row's predicted offset and band, then three mask words around the offset predicted
+// from the walk's position one window earlier, and the offsets they decide, [vlo, vhi).
+struct PrMeta {
+  float e;
+  int lo, nw;
+};
+struct PrWin {
+  unsigned long long w0, w1, w2;
+  int vlo, vhi;
+};
+__device__ __forceinline__ PrMeta pr_meta(const PrArgs& A, uint64_t ro, int n, int i0, int lane) {
+  PrMeta M{0.f, 0, 3};
+  const int i = i0 + lane;
+  if (i < n) {
+    M.e = A.E[ro + i];
+    M.lo = A.lo[ro + i];
+    M.nw = A.nw[ro + i];
+  }
+  return M;
+}
+__device__ __forceinline__ PrWin pr_window(const PrArgs& A, const PrMeta& M, uint64_t ro, int n, int i0, uint32_t rb,
+                                           float eb, int lane) {
+  PrWin W{0, 0, 0, 0, 0};
+  const int i = i0 + lane;
+  if (i < n) {
+    const float pred = (float)rb + (M.e - eb);
+    const int q = min(max((int)floorf((pred - (float)M.lo) * (1.f / 64.f)) - 1, 0), M.nw - 3);
+    const unsigned long long* P = A.mask + (ro + i) * (uint64_t)A.nwmax + q;
+    W.w0 = P[0];
+    W.w1 = P[1];
+    W.w2 = P[2];
+    W.vlo = M.lo + 64 * q;
+    W.vhi = min(W.vlo + 192, (int)A.K - i);
+  }
+  return W;
+}
+
+// row i's first accepting offset at or above `from`, decided here: 64 offsets at a time across the wave. The bound
+// only guarantees an exit (an attempt rejects with probability at most 1/2: 2^22 rejections in a row do not occur).
+__device__ uint32_t pr_spot(const PrArgs& A, const uint2* X, const uint4* CK, uint64_t ro, double off, int i,
+                            uint32_t from, int lane) {
+  const LatentRow r = latent_row(A.AF[ro + i], A.ybin[ro + i], off);
+  uint32_t o = from;
+  for (int k = 0; k < (1 << 16); k++, o += 64) {
+    const uint2 x = pr_slot(X, CK, A.K, (uint32_t)i + o + (uint32_t)lane);
+    const unsigned long long b = __ballot(latent_accepts(r, x.x, x.y, A.tb));
+    if (b) return o + (uint32_t)__builtin_ctzll(b);
+  }
+  return o;
+}
+
+// Rows [i0, i0 + reff) from offset r, with each lane's words around the path: the run of rows that accept at the
+// current offset is resolved by one ballot, then the first row that does not moves the offset to its next accepting
+// one (from its words, or decided on the spot). Writes each row's accepting slot if `write`; returns the offset at
+// which the last row accepts (the next row's starting offset).
+__device__ __forceinline__ uint32_t pr_walk_window(const PrArgs& A, const PrWin& cur, const uint2* X, const uint4* CK, uint64_t ro,
+                                   double off, int i0, int reff, uint32_t r, unsigned& spot, bool write, int lane) {
+  const unsigned long long live = reff >= 64 ? ~0ull : ((1ull << reff) - 1ull);
+  int done = 0;
+  uint32_t accv = 0;
+  while (done < reff) {
+    // every lane: its row's first accepting offset at or above r from its words, or -1 - (the first offset not known
+    // to reject) when they do not reach one
+    const int rel = (int)r - cur.vlo, lim = cur.vhi - cur.vlo;
+    int nx = -1 - (int)r;
+    if (rel >= 0 && rel < lim) {
+      const unsigned long long m0 = rel < 64 ? (cur.w0 & (~0ull << rel)) : 0ull;
+      const unsigned long long m1 = rel < 64 ? cur.w1 : (rel < 128 ? (cur.w1 & (~0ull << (rel - 64))) : 0ull);
+      const unsigned long long m2 = rel < 128 ? cur.w2 : (cur.w2 & (~0ull << (rel - 128)));
+      const int pos = m0 ? (int)__builtin_ctzll(m0)
+                         : (m1 ? 64 + (int)__builtin_ctzll(m1) : (m2 ? 128 + (int)__builtin_ctzll(m2) : 192));
+      nx = pos < lim ? cur.vlo + pos : -1 - cur.vhi;
+    }
+    const bool ok = nx == (int)r;
+    const unsigned long long notok = live & (~0ull << done) & ~(unsigned long long)__ballot(ok);
+    const int f = notok ? (int)__builtin_ctzll(notok) : reff;
+    accv = (lane >= done && lane < f) ? r : accv;
+    done = f;
+    if (done >= reff) break;
+    const int nf = __builtin_amdgcn_readlane(nx, f);
+    if (nf >= 0) {
+      r = (uint32_t)nf;
+    } else {
+      r = pr_spot(A, X, CK, ro, off, i0 + f, (uint32_t)(-1 - nf), lane);
+      spot++;
+    }
+    accv = lane == f ? r : accv;
+    done = f + 1;
+  }
+  if (write && lane < reff) A.acc[ro + i0 + lane] = (uint32_t)(i0 + lane) + accv;
+  return r;
+}
+
+// Per block of 64 rows and every entry offset in the band of its first row: the offset at which the block's last
+// row accepts, following the decision bits (-1 where the path leaves a row's band). Bands staged in LDS.
+__global__ __launch_bounds__(256) void k_pr_table(PrArgs A) {
+  const int c = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
+  const ChainDesc& d = A.desc[c];
+  const int n = d.n, r0 = 64 * b;
+  if (r0 >= n) return;
+  const int nr = min(64, n - r0), nwm = A.nwmax;
+  const uint64_t ro = d.row_off;
+  extern __shared__ unsigned long long pr_sm[];  // [64][nwmax] band words, then lo[64], nw[64]
+  int* slo = reinterpret_cast<int*>(pr_sm + 64 * nwm);
+  int* snw = slo + 64;
+  for (int k = tid; k < nr * nwm; k += 256) pr_sm[k] = A.mask[(ro + r0) * (uint64_t)nwm + k];
+  for (int k = tid; k < nr; k += 256) {
+    slo[k] = A.lo[ro + r0 + k];
+    snw[k] = A.nw[ro + r0 + k];
+  }
+  __syncthreads();
+  const int L = slo[0], Wd = 64 * snw[0];
+  int* T = A.tab + ((size_t)c * A.nbmax + b) * (size_t)A.wmax;
+  for (int u = tid; u < Wd; u += 256) {
+    int x = L + u;
+    for (int l = 0; l < nr; l++) {
+      const int lo = slo[l], nw = snw[l];
+      const int rel = x - lo;
+      if (rel < 0 || rel >= 64 * nw) {
+        x = -1;
+        break;
+      }
+      int wi = rel >> 6;
+      unsigned long long m = pr_sm[l * nwm + wi] & (~0ull << (rel & 63));
+      while (!m && ++wi < nw) m = pr_sm[l * nwm + wi];
+      if (!m) {
+        x = -1;
+        break;
+      }
+      x = lo + 64 * wi + (int)__builtin_ctzll(m);
+    }
+    T[u] = x;
+  }
+}
+
+// One wave per chain: the entry offset of every block, block after block through the tables. Blocks go in groups
+// of kPrAhead: at a group's start the table entries around each of its blocks' predicted entries (fetched during the
+// group before) go to LDS and those of the next group are fetched; the group's dependent steps then read LDS only
+// (the blocks' first-row bands are staged in LDS kPrChunk blocks at a time). A block whose entry is outside its
+// table, or whose path leaves the bands, is walked here. Leaves the generator state after the last row.
+constexpr int kPrAhead = 8;
+constexpr int kPrChunk = 1024;
+__global__ __launch_bounds__(64) void k_pr_chain(PrArgs A) {
+  const int c = blockIdx.x, lane = threadIdx.x;
+  const ChainDesc& d = A.desc[c];
+  const int n = d.n, nb = (n + 63) / 64;
+  const uint64_t ro = d.row_off;
+  const double off = (double)d.offset;
+  const uint2* X = A.slots + (size_t)c * A.K;
+  const uint4* CK = A.ck + (size_t)c * (A.K / kPrSlots);
+  const int* TB = A.tab + (size_t)c * A.nbmax * (size_t)A.wmax;
+  uint32_t* entry = A.entry + (size_t)c * A.nbmax;
+  // first row of blocks [q0, q0 + kPrChunk + 2 kPrAhead): predicted offset, band start, band width
+  __shared__ float sE[kPrChunk + 2 * kPrAhead];
+  __shared__ int sL[kPrChunk + 2 * kPrAhead], sW[kPrChunk + 2 * kPrAhead];
+  // the current group's table windows: 128 entries from swu[j]
+  __shared__ int win[kPrAhead][128];
+  __shared__ int swu[kPrAhead];
+  int q0 = 0;
+  auto stage = [&](int q) {
+    for (int k = lane; k < kPrChunk + 2 * kPrAhead; k += 64) {
+      const int b = q + k;
+      sE[k] = b < nb ? A.E[ro + 64 * b] : 0.f;
+      sL[k] = b < nb ? A.lo[ro + 64 * b] : 0;
+      sW[k] = b < nb ? 64 * (int)A.nw[ro + 64 * b] : 0;
+    }
+    wave_sync();
+  };
+  // lane k: entries u0 + 2 k, u0 + 2 k + 1 of block b's table
+  auto fetch = [&](int b, int u0, int W, int& t0, int& t1) {
+    const int u = u0 + 2 * lane;
+    t0 = (b < nb && u >= 0 && u < W) ? TB[(size_t)b * A.wmax + u] : -2;
+    t1 = (b < nb && u + 1 >= 0 && u + 1 < W) ? TB[(size_t)b * A.wmax + u + 1] : -2;
+  };
+  stage(0);
+  int t0[kPrAhead], t1[kPrAhead], u0v[kPrAhead];
+#pragma unroll
+  for (int j = 0; j < kPrAhead; j++) {
+    u0v[j] = (int)rintf(sE[j] - sE[0]) - sL[j] - 64;
+    fetch(j, u0v[j], sW[j], t0[j], t1[j]);
+  }
+  uint32_t x = 0;
+  unsigned spot = 0;
+  for (int b0 = 0; b0 < nb; b0 += kPrAhead) {
+    if (b0 >= q0 + kPrChunk) {
+      q0 = b0;
+      stage(q0);
+    }
+    // this group's windows to LDS (fetched a group ago), the next group's fetched from here
+#pragma unroll
+    for (int j = 0; j < kPrAhead; j++) {
+      win[j][2 * lane] = t0[j];
+      win[j][2 * lane + 1] = t1[j];
+      if (lane == j) swu[j] = u0v[j];
+    }
+    wave_sync();
+    const float eb0 = sE[b0 - q0];
+#pragma unroll
+    for (int j = 0; j < kPrAhead; j++) {
+      const int bn = b0 + kPrAhead + j, k = bn - q0;
+      if (bn < nb) {
+        u0v[j] = (int)x + (int)rintf(sE[k] - eb0) - sL[k] - 64;
+        fetch(bn, u0v[j], sW[k], t0[j], t1[j]);
+      }
+    }
+    // the group's steps: a plain loop (one copy of the walk below keeps the hot code small); lane j holds block
+    // b0 + j's band start, band width and window start, so that a step's only LDS read is its table entry
+    const int gL = lane < kPrAhead ? sL[b0 - q0 + lane] : 0, gW = lane < kPrAhead ? sW[b0 - q0 + lane] : 0;
+    const int gU = lane < kPrAhead ? swu[lane] : 0;
+#pragma unroll 1
+    for (int j = 0; j < kPrAhead; j++) {
+      const int b = b0 + j, k = b - q0;
+      if (b < nb) {
+        const int Lb = __builtin_amdgcn_readlane(gL, j), Wb = __builtin_amdgcn_readlane(gW, j);
+        if (lane == 0) entry[b] = x;
+        const int u = (int)x - Lb;
+        int v = -1;
+        if (u >= 0 && u < Wb) {
+          const int kk = u - __builtin_amdgcn_readlane(gU, j);
+          v = (kk >= 0 && kk < 128) ? win[j][kk] : TB[(size_t)b * A.wmax + u];
+        }
+        if (v >= 0) {
+          x = (uint32_t)v;
+        } else {
+          const int i0 = 64 * b;
+          const PrWin cur = pr_window(A, pr_meta(A, ro, n, i0, lane), ro, n, i0, x, sE[k], lane);
+          x = pr_walk_window(A, cur, X, CK, ro, off, i0, min(64, n - i0), x, spot, false, lane);
+        }
+      }
+    }
+    wave_sync();
+  }
+  if (lane == 0) {
+    const Rng g = pr_state_at(CK, A.K, (uint32_t)n + x);
+    ChainState* cs = A.state + c;
+    cs->rng[0] = g.s0;
+    cs->rng[1] = g.s1;
+    cs->rng[2] = g.s2;
+    cs->rng[3] = g.s3;
+  }
+}
+
+// One wave per block of 64 rows: each row's accepting slot, from the block's entry offset.
+__global__ __launch_bounds__(64) void k_pr_walk(PrArgs A) {
+  const int c = blockIdx.y, b = blockIdx.x, lane = threadIdx.x;
+  const ChainDesc& d = A.desc[c];
+  const int n = d.n, i0 = 64 * b;
+  if (i0 >= n) return;
+  const uint64_t ro = d.row_off;
+  const uint32_t r = A.entry[(size_t)c * A.nbmax + b];
+  const PrWin cur = pr_window(A, pr_meta(A, ro, n, i0, lane), ro, n, i0, r, A.E[ro + i0], lane);
+  unsigned spot = 0;
+  (void)pr_walk_window(A, cur, A.slots + (size_t)c * A.K, A.ck + (size_t)c * (A.K / kPrSlots), ro,
+                       (double)d.offset, i0, min(64, n - i0), r, spot, true, lane);
+  if (lane == 0 && spot) atomicAdd(A.spot + c, spot);
+}
+
+__global__ __launch_bounds__(256) void k_pr_value(PrArgs A) {
+  const int c = blockIdx.y;
+  const ChainDesc& d = A.desc[c];
+  const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (i >= d.n) return;
+  const uint64_t ro = d.row_off;
+  const uint2 x = pr_slot(A.slots + (size_t)c * A.K, A.ck + (size_t)c * (A.K / kPrSlots), A.K, A.acc[ro + i]);
+  const LatentRow r = latent_row(A.AF[ro + i], A.ybin[ro + i], (double)d.offset);
+  A.Y[ro + i] = latent_value(r, x.x, x.y, A.tb);
 }
 
 // ------------------------------------------------------------------ launch variants
@@ -1829,7 +2341,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
   if (wave > 0) {
     if (j > 0) lp_afx_apply(c, P, AFX, j - 1, dw, lane);
     TileIn<kRowsStreamL> T;
-    data_pass<kRowsStreamL, false, FX, false, false>(c, dyn, P, dw, lane, j - 1, j, 0, j > 0, true, false, T, false);
+    data_pass<kRowsStreamL, false, FX, true, false>(c, dyn, P, dw, lane, j - 1, j, 0, j > 0, true, false, T, false);
   }
   __syncthreads();
   if (wave == 0) ctl_combine<FX>(c, 0, false, 0, lane);
@@ -2487,7 +2999,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
   const int cmode = c.cmode[0];
   if (cmode == 1 && wave > 0) {
     TileIn<kRowsStreamL> T;
-    data_pass<kRowsStreamL, false, FX, false, false>(c, dyn, P, dw, lane, j - 1, j, 0, false, true, false, T, false);
+    data_pass<kRowsStreamL, false, FX, true, false>(c, dyn, P, dw, lane, j - 1, j, 0, false, true, false, T, false);
   }
   __syncthreads();
   if (wave == 0) {
@@ -2627,14 +3139,14 @@ __global__ __launch_bounds__(kBlock) void k_lp_close(LpArgs A) {
   img_copy(&c, A.img + chain, tid);
   __syncthreads();
   const RowPtrs P = lp_rows(a, c.cd);
-  const int m = a.m, n = c.cd.n;
+  const int m = a.m;
   LpState* S = A.st + chain;
   const int it = S->it;
   const bool keep = it >= a.nskip;
   if (wave > 0) {
     lp_afx_apply(c, P, A.AFX + c.cd.row_off, m - 1, dw, lane);
     TileIn<kRowsStreamL> T;
-    data_pass<kRowsStreamL, false, FX, false, true>(c, dyn, P, dw, lane, m - 1, m, 0, true, false, keep, T, false);
+    data_pass<kRowsStreamL, false, FX, true, true>(c, dyn, P, dw, lane, m - 1, m, 0, true, false, keep, T, false);
   }
   __syncthreads();
   if (wave == 0) {
@@ -2806,6 +3318,23 @@ struct fbh_batch {
   float keep = 1.f;
   unsigned long long tseed = 0;
   std::vector<hipEvent_t> lev;  // start/stop pairs per learned sweep
+  // probit latent redraw: 1 in the sweep kernel, 2 by the k_pr_* kernels after every sweep (buffers made on first use)
+  int pr_mode = 1;
+  uint32_t pr_K = 0;
+  int pr_nwmax = 0;
+  uint4* d_jump = nullptr;
+  uint2* d_slots = nullptr;
+  uint4* d_ck = nullptr;
+  float2* d_ev = nullptr;
+  double2* d_btot = nullptr;
+  float* d_E = nullptr;
+  int* d_lo = nullptr;
+  uint8_t* d_nw = nullptr;
+  unsigned long long* d_mask = nullptr;
+  uint32_t* d_acc = nullptr;
+  unsigned* d_spot = nullptr;
+  int* d_tab = nullptr;
+  uint32_t* d_entry = nullptr;
 };
 
 static void free_batch(fbh_batch* b) {
@@ -2814,7 +3343,9 @@ static void free_batch(fbh_batch* b) {
   void* ptrs[] = {b->d_desc, b->d_state, b->d_trees, b->d_q,   b->d_Y,   b->d_W,   b->d_yin,    b->d_AF,
                   b->d_TM,   b->d_ybin,  b->d_leaf,  b->d_sigma, b->d_tout, b->d_PG, b->d_trm, b->d_trace,
                   b->d_img,  b->d_lps,   b->d_afx,   b->d_X,     b->d_rg,   b->d_px, b->d_prg, b->d_ptg,
-                  b->d_pcount, b->d_idx, b->d_xbuf, b->d_xctr, b->d_order, b->d_pent, b->d_pq};
+                  b->d_pcount, b->d_idx, b->d_xbuf, b->d_xctr, b->d_order, b->d_pent, b->d_pq,
+                  b->d_jump,  b->d_slots, b->d_ck,    b->d_E,     b->d_lo,   b->d_nw, b->d_mask, b->d_acc,
+                  b->d_spot,  b->d_ev,    b->d_tab,   b->d_entry, b->d_btot};
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto g : b->gexecs) (void)hipGraphExecDestroy(g);
   for (auto g : b->graphs) (void)hipGraphDestroy(g);
@@ -2838,6 +3369,154 @@ static int dalloc(T** p, size_t count) {
     return FBH_ENOMEM;
   }
   return FBH_OK;
+}
+
+// ---- probit redraw spread over the device (k_pr_*)
+constexpr int kPrAutoChains = 16;  // automatic choice: the k_pr_* kernels for batches of at most this many chains
+// the generator's state transition applied to a host state (rng_step)
+static void host_xo_step(uint32_t s[4]) {
+  const uint32_t t = s[1] << 9;
+  s[2] ^= s[0];
+  s[3] ^= s[1];
+  s[1] ^= s[2];
+  s[0] ^= s[3];
+  s[2] ^= t;
+  s[3] = (s[3] << 11) | (s[3] >> 21);
+}
+// out[k][j] = T^(128 2^k) e_j: 128 steps from each unit state, then repeated squaring (column j of M M is M applied to
+// column j of M)
+static void host_jump_tables(std::vector<uint32_t>& out) {
+  out.assign((size_t)kJumpLevels * 128 * 4, 0u);
+  for (int j = 0; j < 128; j++) {
+    uint32_t st[4] = {0u, 0u, 0u, 0u};
+    st[j >> 5] = 1u << (j & 31);
+    for (int k = 0; k < 2 * kPrSlots; k++) host_xo_step(st);
+    std::memcpy(&out[(size_t)j * 4], st, 16);
+  }
+  for (int L = 1; L < kJumpLevels; L++) {
+    const uint32_t* P = &out[(size_t)(L - 1) * 512];
+    for (int j = 0; j < 128; j++) {
+      uint32_t o[4] = {0u, 0u, 0u, 0u};
+      for (int bit = 0; bit < 128; bit++)
+        if ((P[4 * j + (bit >> 5)] >> (bit & 31)) & 1u)
+          for (int w = 0; w < 4; w++) o[w] ^= P[4 * bit + w];
+      std::memcpy(&out[((size_t)L * 128 + j) * 4], o, 16);
+    }
+  }
+}
+// the state 64 blocks slots (128 blocks generator steps) after s, through the tables
+static void host_jump(const std::vector<uint32_t>& J, const uint32_t s[4], uint64_t blocks, uint32_t out[4]) {
+  uint32_t st[4] = {s[0], s[1], s[2], s[3]};
+  for (int k = 0; k < kJumpLevels; k++) {
+    if (!((blocks >> k) & 1u)) continue;
+    uint32_t o[4] = {0u, 0u, 0u, 0u};
+    for (int bit = 0; bit < 128; bit++)
+      if ((st[bit >> 5] >> (bit & 31)) & 1u)
+        for (int w = 0; w < 4; w++) o[w] ^= J[((size_t)k * 128 + bit) * 4 + w];
+    std::memcpy(st, o, 16);
+  }
+  std::memcpy(out, st, 16);
+}
+
+static int pr_blocks(const fbh_batch* b) { return (std::max(b->nmax, 1) + 63) / 64; }
+static size_t pr_table_lds(int nwmax) { return (size_t)64 * nwmax * 8 + 64 * 8; }
+
+// buffers of the device-wide redraw; K = 0: room for twice the longest chain's rows and its widest band (an attempt
+// rejects with probability at most 1/2, so a chain uses about n + E_n <= 2 n slots); a smaller K (a multiple of 64)
+// sends the slots past it to the walk's own stepping
+static int pr_alloc(fbh_batch* b, uint32_t K) {
+  const int nwmax = std::min(255, (int)std::ceil((9.0 * std::sqrt(2.0 * std::max(b->nmax, 1)) + 128.0) / 64.0) + 1);
+  const uint64_t Kd = ((2ull * (uint64_t)b->nmax + 64ull * (uint64_t)nwmax + 4096ull) + 63ull) / 64ull * 64ull;
+  if (K == 0) K = (uint32_t)Kd;
+  if (K % kPrSlots || K < (uint32_t)kPrSlots || (uint64_t)K / kPrSlots > (1ull << kJumpLevels)) {
+    set_error(FBH_EINVAL, "probit redraw: slots must be a positive multiple of 64 below 2^26");
+    return FBH_EINVAL;
+  }
+  int rc = FBH_OK;
+  if (!b->d_jump) {
+    std::vector<uint32_t> J;
+    host_jump_tables(J);
+    if ((rc = dalloc(&b->d_jump, (size_t)kJumpLevels * 128))) return rc;
+    if (hipMemcpy(b->d_jump, J.data(), J.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+      set_error(FBH_EHIP, "probit redraw: hipMemcpy failed");
+      return FBH_EHIP;
+    }
+  }
+  if (!b->d_E) {
+    const size_t ro = b->rows_total;
+    if ((rc = dalloc(&b->d_ev, ro)) || (rc = dalloc(&b->d_btot, (size_t)b->C * ((b->nmax + 255) / 256 + 1))) || (rc = dalloc(&b->d_E, ro)) || (rc = dalloc(&b->d_lo, ro)) || (rc = dalloc(&b->d_nw, ro)) ||
+        (rc = dalloc(&b->d_mask, ro * (size_t)nwmax)) || (rc = dalloc(&b->d_acc, ro)) ||
+        (rc = dalloc(&b->d_tab, (size_t)b->C * pr_blocks(b) * 64 * (size_t)nwmax)) ||
+        (rc = dalloc(&b->d_entry, (size_t)b->C * pr_blocks(b))) ||
+        (rc = dalloc(&b->d_spot, (size_t)b->C)))
+      return rc;
+    if (hipMemset(b->d_spot, 0, sizeof(unsigned) * b->C) != hipSuccess) {
+      set_error(FBH_EHIP, "probit redraw: hipMemset failed");
+      return FBH_EHIP;
+    }
+    b->pr_nwmax = nwmax;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(k_pr_table), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)pr_table_lds(nwmax)) != hipSuccess) {
+      set_error(FBH_EHIP, "probit redraw: hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
+      return FBH_EHIP;
+    }
+  }
+  if (K != b->pr_K) {
+    (void)hipFree(b->d_slots);
+    (void)hipFree(b->d_ck);
+    b->d_slots = nullptr;
+    b->d_ck = nullptr;
+    b->pr_K = 0;
+    if ((rc = dalloc(&b->d_slots, (size_t)b->C * K)) || (rc = dalloc(&b->d_ck, (size_t)b->C * (K / kPrSlots))))
+      return rc;
+    b->pr_K = K;
+  }
+  return FBH_OK;
+}
+
+static PrArgs make_pr_args(const fbh_batch* b) {
+  PrArgs A;
+  A.desc = b->d_desc;
+  A.state = b->d_state;
+  A.AF = b->d_AF;
+  A.ybin = b->d_ybin;
+  A.Y = b->d_Y;
+  A.tb = b->tb;
+  A.jump = b->d_jump;
+  A.slots = b->d_slots;
+  A.ck = b->d_ck;
+  A.ev = b->d_ev;
+  A.btot = b->d_btot;
+  A.nb256 = (std::max(b->nmax, 1) + 255) / 256;
+  A.E = b->d_E;
+  A.lo = b->d_lo;
+  A.nw = b->d_nw;
+  A.mask = b->d_mask;
+  A.acc = b->d_acc;
+  A.spot = b->d_spot;
+  A.K = b->pr_K;
+  A.nwmax = b->pr_nwmax;
+  A.tab = b->d_tab;
+  A.entry = b->d_entry;
+  A.nbmax = pr_blocks(b);
+  A.wmax = 64 * b->pr_nwmax;
+  return A;
+}
+
+// one sweep's redraw for every chain, on the batch's stream (after the sweep kernel has left the fit in HBM)
+static hipError_t pr_launch(const fbh_batch* b) {
+  const PrArgs A = make_pr_args(b);
+  const unsigned C = (unsigned)b->C, nm = (unsigned)std::max(b->nmax, 1);
+  hipLaunchKernelGGL(k_pr_rows, dim3((nm + 255) / 256, C), dim3(256), 0, b->stream, A);
+  hipLaunchKernelGGL(k_pr_band, dim3(C), dim3(256), 0, b->stream, A);
+  hipLaunchKernelGGL(k_pr_bandrows, dim3((nm + 255) / 256, C), dim3(256), 0, b->stream, A);
+  hipLaunchKernelGGL(k_pr_gen, dim3((A.K / kPrSlots + 255) / 256, C), dim3(256), 0, b->stream, A);
+  hipLaunchKernelGGL(k_pr_decide, dim3((nm + 3) / 4, C), dim3(256), 0, b->stream, A);
+  hipLaunchKernelGGL(k_pr_table, dim3((unsigned)A.nbmax, C), dim3(256), pr_table_lds(A.nwmax), b->stream, A);
+  hipLaunchKernelGGL(k_pr_chain, dim3(C), dim3(64), 0, b->stream, A);
+  hipLaunchKernelGGL(k_pr_walk, dim3((unsigned)A.nbmax, C), dim3(64), 0, b->stream, A);
+  hipLaunchKernelGGL(k_pr_value, dim3((nm + 255) / 256, C), dim3(256), 0, b->stream, A);
+  return hipGetLastError();
 }
 
 static KArgs make_args(const fbh_batch* b) {
@@ -2871,6 +3550,7 @@ static KArgs make_args(const fbh_batch* b) {
   a.xbuf = b->d_xbuf;
   a.xctr = b->d_xctr;
   a.order = b->d_order;
+  a.pr_ext = 0;
   return a;
 }
 
@@ -2951,9 +3631,9 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
       (o->block_threads != 0 && o->block_threads != kBlock) ||
       (o->tile_rows != 0 && o->tile_rows != Geo<kRowsStream>::kTile && o->tile_rows != Geo<kRowsRes>::kTile) ||
       o->packing < 0 || o->packing > 2 || o->chains_per_cu < 0 || o->chains_per_cu > 2 || o->resident < 0 ||
-      o->resident > 2) {
+      o->resident > 2 || o->probit_redraw < 0 || o->probit_redraw > 2) {
     set_error(FBH_EINVAL, "fbh_batch_create: bad options (m, nskip, ndpost, pgtab, stats_mode, node_cap, "
-                          "block_threads, tile_rows, packing, chains_per_cu or resident)");
+                          "block_threads, tile_rows, packing, chains_per_cu, resident or probit_redraw)");
     return FBH_EINVAL;
   }
   int W = o->row_split > 1 ? o->row_split : 1;
@@ -3020,6 +3700,7 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
   b->nskip = o->nskip;
   b->ndpost = o->ndpost;
   b->probit = o->probit ? 1 : 0;
+  b->pr_mode = o->probit_redraw ? o->probit_redraw : (b->C <= kPrAutoChains ? 2 : 1);
   b->keep_trees = o->keep_trees ? 1 : 0;
   b->stats_mode = o->stats_mode;
   b->node_cap = o->node_cap ? o->node_cap : kNodes;
@@ -3234,8 +3915,15 @@ extern "C" int fbh_batch_run(fbh_batch* b, int nsweeps) {
   const int end = std::min(total, b->it + nsweeps);
   if (b->it >= end) return FBH_OK;
   // the launches of this call: plain, each between a pair of timing events; or, with graph = 1, captured into one
-  // graph of kernel nodes in stream order that is launched once between a pair of timing events
-  const int nl = (end - b->it + b->sweeps_per_launch - 1) / b->sweeps_per_launch;
+  // graph of kernel nodes in stream order that is launched once between a pair of timing events. With the probit
+  // redraw spread over the device a launch is one sweep, followed by the k_pr_* kernels.
+  const bool par = b->probit && b->pr_mode == 2;
+  if (par) {
+    const int rc = pr_alloc(b, b->pr_K);
+    if (rc) return rc;
+  }
+  const int per = par ? 1 : b->sweeps_per_launch;
+  const int nl = (end - b->it + per - 1) / per;
   const int npair = b->graph ? 1 : nl;
   const size_t ev0 = b->ev.size();
   for (int k = 0; k < npair; k++) {
@@ -3252,14 +3940,16 @@ extern "C" int fbh_batch_run(fbh_batch* b, int nsweeps) {
   hipGraph_t g = nullptr;
   if (b->graph) e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
   KArgs a = make_args(b);
+  a.pr_ext = par ? 1 : 0;
   for (int k = 0, it = b->it; k < nl && e == hipSuccess; k++) {
     a.it0 = it;
-    a.it1 = std::min(end, it + b->sweeps_per_launch);
+    a.it1 = std::min(end, it + per);
     void* args[] = {&a};
     if (!b->graph) e = hipEventRecord(b->ev[ev0 + k].e0, b->stream);
     if (e == hipSuccess)
       e = hipLaunchKernel(reinterpret_cast<const void*>(b->var.fn), dim3(b->C), dim3(kBlock), args, b->dyn_bytes,
                           b->stream);
+    if (e == hipSuccess && par) e = pr_launch(b);
     if (e == hipSuccess && !b->graph) e = hipEventRecord(b->ev[ev0 + k].e1, b->stream);
     it = a.it1;
   }
@@ -3294,6 +3984,56 @@ extern "C" int fbh_batch_configure(fbh_batch* b, int packing, int tile_rows, int
   }
   const bool dense = chains_per_cu == 0 ? b->var.dense : chains_per_cu == 2;
   return configure(b, packing, pick_kind(b, tile_rows, resident, dense), dense);
+}
+
+extern "C" int fbh_batch_probit_redraw(fbh_batch* b, int mode, int slots) {
+  if (!b || !b->probit || mode < 0 || mode > 2 || slots < 0) {
+    set_error(FBH_EINVAL, "fbh_batch_probit_redraw: bad argument (a probit batch, mode 0..2, slots >= 0)");
+    return FBH_EINVAL;
+  }
+  FBH_HIP_CHECK(hipSetDevice(b->device));
+  const int m = mode ? mode : (b->C <= kPrAutoChains ? 2 : 1);
+  if (m == 2) {
+    FBH_HIP_CHECK(hipStreamSynchronize(b->stream));  // the slot buffers may be replaced
+    const int rc = pr_alloc(b, (uint32_t)slots);
+    if (rc) return rc;
+  }
+  b->pr_mode = m;
+  return FBH_OK;
+}
+
+extern "C" int fbh_batch_probit_info(fbh_batch* b, int* mode, int* slots, unsigned long long* spot) {
+  if (!b) {
+    set_error(FBH_EINVAL, "fbh_batch_probit_info: null batch");
+    return FBH_EINVAL;
+  }
+  FBH_HIP_CHECK(hipSetDevice(b->device));
+  if (mode) *mode = b->probit ? b->pr_mode : 0;
+  if (slots) *slots = (int)b->pr_K;
+  if (spot) {
+    *spot = 0;
+    if (b->d_spot) {
+      std::vector<unsigned> h((size_t)b->C);
+      FBH_HIP_CHECK(hipStreamSynchronize(b->stream));
+      FBH_HIP_CHECK(hipMemcpy(h.data(), b->d_spot, sizeof(unsigned) * b->C, hipMemcpyDeviceToHost));
+      for (unsigned v : h) *spot += v;
+    }
+  }
+  return FBH_OK;
+}
+
+extern "C" int fbh_debug_jump(const uint32_t* state, unsigned long long blocks, uint32_t* out) {
+  if (!state || !out || blocks >= (1ull << kJumpLevels)) {
+    set_error(FBH_EINVAL, "fbh_debug_jump: bad argument");
+    return FBH_EINVAL;
+  }
+  static const std::vector<uint32_t> J = [] {
+    std::vector<uint32_t> v;
+    host_jump_tables(v);
+    return v;
+  }();
+  host_jump(J, state, blocks, out);
+  return FBH_OK;
 }
 
 extern "C" int fbh_batch_set_order(fbh_batch* b, const int* order) {

@@ -41,7 +41,7 @@ class BatchSampler:
 
     def __init__(self, problems, m, nskip, ndpost, seeds, probit=0, keep_trees=1, stats_mode=0, node_cap=0,
                  device=0, sweeps_per_launch=0, pgtab=None, tile_rows=0, row_split=0, packing=0, graph=0, chains_per_cu=0,
-                 resident=0):
+                 resident=0, probit_redraw=0):
         L = lib()
         self.problems = problems
         self.C = len(problems)
@@ -54,7 +54,7 @@ class BatchSampler:
         pg = np.ascontiguousarray(pgtab if pgtab is not None else problems[0]["pgtab"], dtype=np.float32)
         self._keep.append(pg)
         o = Opts(m, nskip, ndpost, probit, keep_trees, ptr(pg), stats_mode, node_cap, device, sweeps_per_launch, 0,
-                 tile_rows, row_split, packing, int(bool(graph)), chains_per_cu, resident)
+                 tile_rows, row_split, packing, int(bool(graph)), chains_per_cu, resident, probit_redraw)
         h = C.c_void_p()
         check(L.fbh_batch_create(arr, self.C, C.byref(o), C.byref(h)), "fbh_batch_create")
         self.h = h
@@ -71,6 +71,20 @@ class BatchSampler:
         automatic) and the build (one or two chains per CU; 0 keeps it) between runs. Results do not depend on them."""
         check(lib().fbh_batch_configure(self.h, int(packing), int(tile_rows), int(resident), int(chains_per_cu)),
               "fbh_batch_configure")
+
+    def probit_redraw(self, mode=0, slots=0):
+        """Probit batches: the latent redraw from the next run on -- 1 inside the sweep kernel, 2 by kernels spread over
+        the device, 0 automatic; slots (a multiple of 64) shrinks mode 2's stream buffer. Results do not depend on
+        either."""
+        check(lib().fbh_batch_probit_redraw(self.h, int(mode), int(slots)), "fbh_batch_probit_redraw")
+
+    @property
+    def probit_info(self):
+        """The redraw in effect (0 for wbart), mode 2's stream slots per chain, and the attempts it decided outside its
+        precomputed bands so far (all chains)."""
+        md, sl, sp = C.c_int(), C.c_int(), C.c_ulonglong()
+        check(lib().fbh_batch_probit_info(self.h, C.byref(md), C.byref(sl), C.byref(sp)), "fbh_batch_probit_info")
+        return dict(mode=md.value, slots=sl.value, spot=sp.value)
 
     def set_order(self, order):
         """Run logical chain order[w] on workgroup w (a permutation of the chains)."""

@@ -103,6 +103,11 @@ typedef struct fbh_opts {
   int resident;             /* the running fit during a launch: 1 = streamed from HBM, 2 = resident in LDS (needs the
                                longest chain's 8 n bytes beside the tile within 80 KiB), 0 = automatic (in LDS when
                                it fits: 1024-row tiles for the two-per-CU build, 1536 otherwise) */
+  int probit_redraw;        /* probit batches, the latent redraw after every sweep: 1 = inside the sweep kernel (each
+                               chain's workgroup walks its rows in windows of 64), 2 = by kernels spread over the
+                               device (bands of precomputed decisions, one wave per chain walking them; a launch is
+                               then one sweep), 0 = automatic (2 for batches of at most 16 chains). Results do not
+                               depend on it. */
 } fbh_opts;
 
 typedef struct fbh_chain_out {
@@ -144,6 +149,14 @@ int fbh_batch_configure(fbh_batch* b, int packing, int tile_rows, int resident, 
 /* A packing of the caller's: logical chain order[w] runs on workgroup w (on workgroups w W .. w W + W - 1 with a row
  * split of W); order is a permutation of 0..nchains-1. Waits for the batch's queued work; results do not depend on it. */
 int fbh_batch_set_order(fbh_batch* b, const int* order);
+/* Probit batches: the latent redraw (mode 1, 2, 0 = automatic, as fbh_opts.probit_redraw) from the next run on;
+ * slots = 0 for the default stream buffer of mode 2 (twice the longest chain's rows and its widest band), or a
+ * smaller multiple of 64: slots past it are then generated by the walk itself (slower; the results are the same).
+ * Waits for the batch's queued work. FBH_EINVAL for a wbart batch. */
+int fbh_batch_probit_redraw(fbh_batch* b, int mode, int slots);
+/* The redraw in effect (mode 0 for a wbart batch), the mode-2 stream buffer's slots per chain (0 before its first
+ * use), and the attempts mode 2 has decided outside the precomputed bands so far, summed over chains (synchronises). */
+int fbh_batch_probit_info(fbh_batch* b, int* mode, int* slots, unsigned long long* spot);
 /* the configuration in effect (packing -1 after fbh_batch_set_order; resident 1 streamed, 2 in LDS) */
 int fbh_batch_config(const fbh_batch* b, int* packing, int* tile_rows, int* resident, int* chains_per_cu, int* graph);
 /* Time launch configurations on the batch's next sweeps and keep the fastest: first build x tile kind (1024-row tiles
@@ -371,6 +384,9 @@ int fbh_batch_trace(fbh_batch* b, int cap, void* host_out, int chain);
 /* one launch of a plain copy kernel over two fresh device buffers of mib MiB (1..16384); its duration in ms goes to
  * *ms. Used to calibrate hardware byte counters (FETCH_SIZE / WRITE_SIZE) against a transfer of known size. */
 int fbh_debug_copy(long mib, int device, double* ms);
+/* Host only: the generator state 128 blocks steps (64 blocks attempt slots) after state[4], through the jump tables
+ * of the device-wide probit redraw (the powers T^(128 2^k) of the state transition); blocks < 2^20. */
+int fbh_debug_jump(const uint32_t* state, unsigned long long blocks, uint32_t* out);
 /* the exact split conditional of fbh_batch_learned_target kind 1 at one node given on the host (the device function
  * the pairs use): n rows with cutpoint ranks q [p][n], membership member[n] (1 = the row is in the node), weights w and
  * partial residuals r; ranges [p] = L | U << 8 of the node's allowed cuts (U < L: none); the node's depth and

@@ -87,11 +87,15 @@ def test_gpu_state_after_partial_run_and_resume(oracle):
     assert same_bits(out["trees"].reshape(-1), ch.trees_out)
 
 
-def test_gpu_probit_latents_match_oracle(oracle):
+@pytest.mark.parametrize("redraw", [1, 2])
+def test_gpu_probit_latents_match_oracle(oracle, redraw):
+    """Latents, generator state and kept trees of a probit chain; redraw 1 walks the rows inside the sweep kernel,
+    2 with the kernels spread over the device."""
     pb = W.small_problem(700, 3, 12, 41, probit=True, ntree=12)
     m, nskip, nd = 12, 10, 5
     ch = oracle.chain(pb, m, nskip, nd, 3, probit=1)
-    with sampler.BatchSampler([pb], m, nskip, nd, [3], probit=1) as b:
+    with sampler.BatchSampler([pb], m, nskip, nd, [3], probit=1, probit_redraw=redraw) as b:
+        assert b.probit_info["mode"] == redraw
         s0 = b.state(0)
         assert same_bits(s0["latent"], ch.state()["latent"])     # initial latents
         b.run(nskip + nd)
@@ -145,15 +149,16 @@ def test_gpu_maxshape_chain_fx64_matches_oracle(oracle):
 
 
 @pytest.mark.parametrize("tile_rows", [0, 1024])
-def test_gpu_probit_windows_match_oracle(oracle, tile_rows):
+@pytest.mark.parametrize("redraw", [1, 2])
+def test_gpu_probit_windows_match_oracle(oracle, tile_rows, redraw):
     """Latent redraw in windows of 64 rows: chains whose n is not a multiple of the window, resident and streamed
-    running fit, latents and generator state after every launch against the oracle."""
+    running fit, latents and generator state after every launch against the oracle (both redraws)."""
     probs = [W.small_problem(n, 3, 15, 91 + n, probit=True, ntree=10) for n in (64, 130, 2999)]
     m, nskip, nd = 10, 6, 3
     chs = [oracle.chain(pb, m, nskip, nd, 21 + c, probit=1) for c, pb in enumerate(probs)]
-    _say("probit windows, tile_rows", tile_rows)
+    _say("probit windows, tile_rows", tile_rows, "redraw", redraw)
     with sampler.BatchSampler(probs, m, nskip, nd, [21, 22, 23], probit=1, tile_rows=tile_rows,
-                              sweeps_per_launch=3) as b:
+                              sweeps_per_launch=3, probit_redraw=redraw) as b:
         for k in range(3):
             b.run(3)
             for c, ch in enumerate(chs):
@@ -167,15 +172,21 @@ def test_gpu_probit_windows_match_oracle(oracle, tile_rows):
         assert same_bits(outs[c]["trmean"], ch.trmean()), c
 
 
-def test_gpu_probit_workload_chain_matches_oracle(oracle):
-    """The probit workload's chain (n = 24,576, p = 3, 50 trees), a few sweeps."""
+@pytest.mark.parametrize("redraw", [1, 2])
+def test_gpu_probit_workload_chain_matches_oracle(oracle, redraw):
+    """The probit workload's chain (n = 24,576, p = 3, 50 trees), a few sweeps, with either redraw."""
     pb = W.workload_fpbart_n24576()
     m, nskip, nd = 50, 3, 2
-    _say("probit workload chain")
-    with sampler.BatchSampler([pb], m, nskip, nd, [1], probit=1) as b:
+    _say("probit workload chain, redraw", redraw)
+    with sampler.BatchSampler([pb], m, nskip, nd, [1], probit=1, probit_redraw=redraw) as b:
         b.run(nskip + nd)
         s = b.state(0)
         out = b.fetch(0)
+        info = b.probit_info
+    if redraw == 2:
+        # the bands hold nearly every row's path: attempts decided on the spot are rare
+        print("attempts decided outside the bands:", info["spot"], "of", 24576 * (nskip + nd), "rows", flush=True)
+        assert info["spot"] < 0.01 * 24576 * (nskip + nd)
     ch = oracle.chain(pb, m, nskip, nd, 1, probit=1)
     ch.sweeps(nskip + nd)
     e = ch.state()
@@ -185,20 +196,31 @@ def test_gpu_probit_workload_chain_matches_oracle(oracle):
     assert same_bits(out["trmean"], ch.trmean())
 
 
-@pytest.mark.parametrize("name", ["fwbart_n4096", "fpbart_n24576"])
+_FULL = {}
+
+
+@pytest.mark.parametrize("name", ["fwbart_n4096", "fpbart_n24576", "fpbart_n24576_in_kernel"])
 def test_gpu_workload_chain_full_length_matches_oracle(oracle, name):
     """The single-chain workloads as configured: all 1,100 sweeps (nskip 1000, ndpost 100), every output bit for bit
-    against the oracle (sigma trace, posterior-mean fit, the 100 kept draws' tree words, move counts)."""
+    against the oracle (sigma trace, posterior-mean fit, the 100 kept draws' tree words, move counts); the probit
+    chain with the automatic redraw (spread over the device) and with the one inside the sweep kernel."""
+    redraw = 0
     if name == "fwbart_n4096":
         pb, m, probit = W.workload_fwbart_n4096(), 200, 0
     else:
         pb, m, probit = W.workload_fpbart_n24576(), 50, 1
+        redraw = 1 if name.endswith("in_kernel") else 0
     nskip, nd = 1000, 100
     _say("full-length chain", name)
-    with sampler.BatchSampler([pb], m, nskip, nd, [1], probit=probit) as b:
+    with sampler.BatchSampler([pb], m, nskip, nd, [1], probit=probit, probit_redraw=redraw) as b:
+        if probit:
+            assert b.probit_info["mode"] == (redraw or 2)
         b.run(nskip + nd)
         out = b.fetch(0)
-    e = oracle.fit(pb, m, nskip, nd, 1, probit=probit)
+    key = name.replace("_in_kernel", "")
+    if key not in _FULL:
+        _FULL[key] = oracle.fit(pb, m, nskip, nd, 1, probit=probit)
+    e = _FULL[key]
     for k in ("sigma", "trmean", "trees", "stats"):
         assert same_bits(out[k], e[k]), k
 

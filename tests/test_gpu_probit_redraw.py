@@ -1,0 +1,119 @@
+"""The probit latent redraw spread over the device (k_pr_band / k_pr_gen / k_pr_decide / k_pr_walk / k_pr_value)
+against the oracle's sequential redraw (src/bart_kernel.cpp:543-547, rtnorm :282-292): a stream buffer shorter than
+the walk needs (slots past it stepped by the walk itself), switches between the two redraws at run boundaries, the
+automatic choice on a larger batch, and hipGraph capture."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import same_bits
+from fastbart_amd import sampler
+from fastbart_amd import workloads as W
+from fastbart_amd._lib import FbhError
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _check(b, chs, tag):
+    for c, ch in enumerate(chs):
+        s, e = b.state(c), ch.state()
+        assert np.array_equal(s["rng"], e["rng"]), (tag, c)
+        assert same_bits(s["latent"], e["latent"]), (tag, c)
+
+
+@pytest.mark.parametrize("slots", [64, 1024])
+def test_gpu_probit_redraw_short_stream_matches_oracle(oracle, slots):
+    """With the stream buffer cut to `slots` slots, the rows past it are decided and valued from outputs the walk
+    steps itself from the last checkpoint; nothing changes in the results."""
+    probs = [W.small_problem(n, 3, 15, 300 + n, probit=True, ntree=8) for n in (700, 1333)]
+    m, nskip, nd = 8, 4, 2
+    chs = [oracle.chain(pb, m, nskip, nd, 31 + c, probit=1) for c, pb in enumerate(probs)]
+    _say("short stream buffer", slots)
+    with sampler.BatchSampler(probs, m, nskip, nd, [31, 32], probit=1, probit_redraw=2) as b:
+        b.probit_redraw(2, slots)
+        for k in range(3):
+            b.run(2)
+            for ch in chs:
+                ch.sweeps(2)
+            _check(b, chs, k)
+        info = b.probit_info
+        outs = [b.fetch(c) for c in range(2)]
+    assert info["slots"] == slots and info["spot"] > 0
+    for c, ch in enumerate(chs):
+        assert same_bits(outs[c]["trees"].reshape(-1), ch.trees_out), c
+        assert same_bits(outs[c]["trmean"], ch.trmean()), c
+
+
+def test_gpu_probit_redraw_switches_between_runs(oracle):
+    """Either redraw may follow the other at any run boundary (the generator state and latents carry over)."""
+    probs = [W.small_problem(n, 4, 20, 400 + n, probit=True, ntree=10) for n in (513, 2048, 4100)]
+    m, nskip, nd = 10, 6, 4
+    chs = [oracle.chain(pb, m, nskip, nd, 41 + c, probit=1) for c, pb in enumerate(probs)]
+    _say("redraw switches")
+    with sampler.BatchSampler(probs, m, nskip, nd, [41, 42, 43], probit=1, probit_redraw=1,
+                              sweeps_per_launch=2) as b:
+        for k, mode in enumerate((1, 2, 1, 2, 2)):
+            b.probit_redraw(mode)
+            assert b.probit_info["mode"] == mode
+            b.run(2)
+            for ch in chs:
+                ch.sweeps(2)
+            _check(b, chs, k)
+        outs = [b.fetch(c) for c in range(3)]
+    for c, ch in enumerate(chs):
+        assert same_bits(outs[c]["trees"].reshape(-1), ch.trees_out), c
+
+
+def test_gpu_probit_redraw_automatic_choice_on_a_larger_batch(oracle):
+    """Automatic: the sweep kernel's own redraw for more than 16 chains; forced to the device-wide one, a batch of 20
+    chains of different lengths still matches the oracle chain by chain."""
+    probs = [W.small_problem(200 + 97 * k, 3, 12, 500 + k, probit=True, ntree=6) for k in range(20)]
+    m, nskip, nd = 6, 3, 2
+    idx = [0, 7, 19]
+    chs = {c: oracle.chain(probs[c], m, nskip, nd, 51 + c, probit=1) for c in idx}
+    _say("automatic choice, 20 chains")
+    with sampler.BatchSampler(probs, m, nskip, nd, [51 + c for c in range(20)], probit=1) as b:
+        assert b.probit_info["mode"] == 1
+        b.run(2)
+        b.probit_redraw(2)
+        b.run(3)
+        for c, ch in chs.items():
+            ch.sweeps(5)
+            s, e = b.state(c), ch.state()
+            assert np.array_equal(s["rng"], e["rng"]), c
+            assert same_bits(s["latent"], e["latent"]), c
+
+
+def test_gpu_probit_redraw_in_a_graph(oracle):
+    """Runs captured as hipGraphs (the sweep kernel and the five redraw kernels of every sweep), with eager state
+    copies between them, against the oracle after every run."""
+    pb = W.small_problem(1500, 3, 15, 611, probit=True, ntree=10)
+    m, nskip, nd = 10, 4, 4
+    ch = oracle.chain(pb, m, nskip, nd, 61, probit=1)
+    _say("redraw in a graph")
+    with sampler.BatchSampler([pb], m, nskip, nd, [61], probit=1, probit_redraw=2, graph=1) as b:
+        for k in range(4):
+            b.run(2)
+            ch.sweeps(2)
+            _check(b, [ch], k)
+        out = b.fetch(0)
+    assert same_bits(out["trees"].reshape(-1), ch.trees_out)
+
+
+def test_gpu_probit_redraw_rejects_bad_requests():
+    pb = W.small_problem(300, 3, 10, 71, ntree=5)
+    with sampler.BatchSampler([pb], 5, 1, 1, [1]) as b:
+        assert b.probit_info["mode"] == 0
+        with pytest.raises(FbhError):
+            b.probit_redraw(2)
+    pbp = W.small_problem(300, 3, 10, 72, probit=True, ntree=5)
+    with sampler.BatchSampler([pbp], 5, 1, 1, [1], probit=1) as b:
+        for mode, slots in ((3, 0), (-1, 0), (2, 100), (2, -64)):
+            with pytest.raises(FbhError):
+                b.probit_redraw(mode, slots)

@@ -709,6 +709,7 @@ def main() -> int:
     t_w = time.perf_counter() - t_w
     log(f"warmup {W_} sweeps: {t_w:.2f} s")
     launch_cfg = b.config
+    redraw = b.probit_info["mode"] if probit else 0
     avg0, nl0, _ = b.timing()
     t1 = time.perf_counter()
     b.run(K)
@@ -762,7 +763,9 @@ def main() -> int:
                    "fit_in_lds": launch_cfg["resident"] == 2,
                    "chains_per_cu": launch_cfg["chains_per_cu"], "graph": bool(launch_cfg["graph"]),
                    "autotuned": tuned is not None,
-                   "tune_from_cache": bool(tuned and tuned["from_cache"])},
+                   "tune_from_cache": bool(tuned and tuned["from_cache"]),
+                   "probit_redraw": ({1: "in the sweep kernel", 2: "spread over the device"}[redraw]
+                                     if probit else None)},
         "warmup_requested": args.warmup,
         "warmup_note": (f"{W_} warmup sweeps run: the autotuner's trials take up to {TUNE_SWEEPS_MAX} of them"
                         if W_ != (args.warmup if args.warmup is not None else w_def) else None),
@@ -771,7 +774,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": TRAFFIC_SOURCE.format(wl) if traffic is not None else None,
-                     "kernel": "k_sweeps", "kernel_avg_ms": ms_launch, "launches": nlt,
+                     "kernel": "k_sweeps" if redraw != 2 else "k_sweeps + k_pr_* (the latent redraw)",
+                     "kernel_avg_ms": ms_launch, "launches": nlt,
                      "sweeps_per_launch": sweeps_launch, "bytes_per_tree_update_row": 37,
                      "algorithmic_bytes_per_launch": bytes_per_sweep * sweeps_launch},
         "cpu_baseline": cpu,

@@ -1510,7 +1510,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
 }
 
 // Chain initialisation: root-only trees, generator seed + 16 discarded outputs, the initial probit latents
-// (src/bart_kernel.cpp:367-386).
+// (src/bart_kernel.cpp:367-386) -- those by one thread here, or (pr_ext) by the k_pr_* kernels after this one, as
+// the redraw of a zero running fit.
 __global__ __launch_bounds__(256) void k_init(KArgs a, const unsigned* seeds) {
   const int tid = threadIdx.x;
   const int chain = blockIdx.x;
@@ -1532,7 +1533,7 @@ __global__ __launch_bounds__(256) void k_init(KArgs a, const unsigned* seeds) {
     const uint32_t seed = seeds[chain];
     Rng g{seed ^ 0x9E3779B9u, seed * 2654435761u + 1u, 0x6A09E667u ^ (seed << 7), 0xBB67AE85u + seed};
     for (int i = 0; i < 16; i++) rng_next(g);
-    if (a.probit) {
+    if (a.probit && !a.pr_ext) {
       const double off = (double)cd.offset;
       const uint8_t* YB = a.ybin + cd.row_off;
       double* Y = a.Y + cd.row_off;
@@ -3888,10 +3889,24 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     // rows past n in the row pools are read (never used) by the last quad of rows of a tile: keep them zero
     if (e == hipSuccess) e = hipMemsetAsync(b->d_AF, 0, sizeof(double) * ro, b->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b->d_Y, 0, sizeof(double) * ro, b->stream);
+    // the initial probit latents (:384-386) are a redraw with the running fit at zero: batches that redraw with the
+    // k_pr_* kernels draw them there too
+    const bool par = b->probit && b->pr_mode == 2;
     if (e == hipSuccess) {
       KArgs a = make_args(b);
+      a.pr_ext = par ? 1 : 0;
       hipLaunchKernelGGL(k_init, dim3(nchains), dim3(256), 0, b->stream, a, d_seeds);
       e = hipGetLastError();
+    }
+    if (e == hipSuccess && par) {
+      const int rc = pr_alloc(b, 0);
+      if (rc) {
+        (void)hipStreamSynchronize(b->stream);
+        (void)hipFree(d_seeds);
+        free_batch(b);
+        return rc;
+      }
+      e = pr_launch(b);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     (void)hipFree(d_seeds);

@@ -1,7 +1,8 @@
-"""The probit latent redraw spread over the device (k_pr_band / k_pr_gen / k_pr_decide / k_pr_walk / k_pr_value)
-against the oracle's sequential redraw (src/bart_kernel.cpp:543-547, rtnorm :282-292): a stream buffer shorter than
-the walk needs (slots past it stepped by the walk itself), switches between the two redraws at run boundaries, the
-automatic choice on a larger batch, and hipGraph capture."""
+"""The probit latent redraw spread over the device (the k_pr_* kernels: predicted bands, jump-ahead generation, band
+decisions, per-block exit tables, the chained walk, the values) against the oracle's sequential redraw
+(src/bart_kernel.cpp:543-547, rtnorm :282-292): a stream buffer shorter than the walk needs (slots past it stepped by
+the walk itself), switches between the two redraws at run boundaries, the automatic choice on a larger batch,
+hipGraph capture, and a chain of 65,536 rows."""
 import sys
 
 import numpy as np
@@ -91,8 +92,8 @@ def test_gpu_probit_redraw_automatic_choice_on_a_larger_batch(oracle):
 
 
 def test_gpu_probit_redraw_in_a_graph(oracle):
-    """Runs captured as hipGraphs (the sweep kernel and the five redraw kernels of every sweep), with eager state
-    copies between them, against the oracle after every run."""
+    """Runs captured as hipGraphs (the sweep kernel and the redraw kernels of every sweep), with eager state copies
+    between them, against the oracle after every run."""
     pb = W.small_problem(1500, 3, 15, 611, probit=True, ntree=10)
     m, nskip, nd = 10, 4, 4
     ch = oracle.chain(pb, m, nskip, nd, 61, probit=1)
@@ -117,3 +118,22 @@ def test_gpu_probit_redraw_rejects_bad_requests():
         for mode, slots in ((3, 0), (-1, 0), (2, 100), (2, -64)):
             with pytest.raises(FbhError):
                 b.probit_redraw(mode, slots)
+
+
+def test_gpu_probit_redraw_large_chain_matches_oracle(oracle):
+    """A probit chain of 65,536 rows (the largest rows per chain the workloads use; wider bands, more generator jump
+    levels and block tables than the 24,576-row chain), a few sweeps, latents and generator state against the
+    oracle."""
+    pb = W.small_problem(65536, 4, 30, 811, probit=True, ntree=20)
+    m, nskip, nd = 20, 2, 1
+    ch = oracle.chain(pb, m, nskip, nd, 81, probit=1)
+    _say("large probit chain, n = 65536")
+    with sampler.BatchSampler([pb], m, nskip, nd, [81], probit=1, probit_redraw=2) as b:
+        _check(b, [ch], "initial")
+        for k in range(3):
+            b.run(1)
+            ch.sweeps(1)
+            _check(b, [ch], k)
+        info = b.probit_info
+    _say("attempts decided outside the bands:", info["spot"])
+    assert info["spot"] < 0.001 * 65536 * 4

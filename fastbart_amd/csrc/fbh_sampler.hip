@@ -1604,11 +1604,14 @@ struct PrArgs {
   uint32_t* acc;              // [rows] accepting slot
   int* tab;                   // [C][nbmax][wmax] per block of 64 rows: exit offset for each entry in its band, or -1
   uint32_t* entry;            // [C][nbmax] the offset at each block's first row
+  int* gtab;                  // [C][ngmax][wmax] per group of 8 blocks: the block tables composed
+  uint32_t* gentry;           // [C][ngmax] the offset at each group's first row
   unsigned* spot;             // [C] offsets the walk decided on the spot (outside the bands)
   uint32_t K;                 // slots per chain, a multiple of 64
   int nwmax;
   int nbmax, wmax;            // blocks of 64 rows of the longest chain; 64 nwmax
   int nb256;                  // blocks of 256 rows of the longest chain
+  int ngmax;                  // groups of 8 blocks of the longest chain
 };
 
 // the generator state before slot S: its checkpoint (the last one when S is past K) stepped forward
@@ -1936,99 +1939,153 @@ __global__ __launch_bounds__(256) void k_pr_table(PrArgs A) {
   }
 }
 
-// One wave per chain: the entry offset of every block, block after block through the tables. Blocks go in groups
-// of kPrAhead: at a group's start the table entries around each of its blocks' predicted entries (fetched during the
-// group before) go to LDS and those of the next group are fetched; the group's dependent steps then read LDS only
-// (the blocks' first-row bands are staged in LDS kPrChunk blocks at a time). A block whose entry is outside its
-// table, or whose path leaves the bands, is walked here. Leaves the generator state after the last row.
-constexpr int kPrAhead = 8;
-constexpr int kPrChunk = 1024;
+// Blocks [b0, b1) from entry offset x through the block tables, one dependent lookup a block (every block's band
+// start and width fetched first, lane j holding block b0 + j's); a block whose entry is outside its table, or whose
+// path leaves the bands, is walked row by row. Stores each block's entry offset if `entry`; returns the exit.
+__device__ __forceinline__ uint32_t pr_blocks_walk(const PrArgs& A, int c, int n, uint64_t ro, double off,
+                                                   const uint2* X, const uint4* CK, int b0, int b1, uint32_t x,
+                                                   uint32_t* entry, unsigned& spot, int lane) {
+  const int* TB = A.tab + (size_t)c * A.nbmax * (size_t)A.wmax;
+  const int bl = b0 + lane;
+  const int mL = lane < b1 - b0 ? A.lo[ro + 64 * bl] : 0, mW = lane < b1 - b0 ? 64 * (int)A.nw[ro + 64 * bl] : 0;
+  for (int b = b0; b < b1; b++) {
+    if (entry && lane == 0) entry[b] = x;
+    const int t = (int)x - __builtin_amdgcn_readlane(mL, b - b0);
+    const int v = (t >= 0 && t < __builtin_amdgcn_readlane(mW, b - b0)) ? TB[(size_t)b * A.wmax + t] : -1;
+    if (v >= 0) {
+      x = (uint32_t)v;
+    } else {
+      const int i0 = 64 * b;
+      const PrWin cur = pr_window(A, pr_meta(A, ro, n, i0, lane), ro, n, i0, x, A.E[ro + i0], lane);
+      x = pr_walk_window(A, cur, X, CK, ro, off, i0, min(64, n - i0), x, spot, false, lane);
+    }
+  }
+  return x;
+}
+
+// Per group of kPrGroup blocks and every entry offset in its first block's band: the exit offset of the group, the
+// block tables composed (-1 where a path leaves them).
+constexpr int kPrGroup = 8;
+__global__ __launch_bounds__(256) void k_pr_compose(PrArgs A) {
+  const int c = blockIdx.y, g = blockIdx.x, tid = threadIdx.x;
+  const ChainDesc& d = A.desc[c];
+  const int n = d.n, nb = (n + 63) / 64, b0 = kPrGroup * g;
+  if (b0 >= nb) return;
+  const int b1 = min(nb, b0 + kPrGroup);
+  const uint64_t ro = d.row_off;
+  __shared__ int sL[kPrGroup], sW[kPrGroup];
+  if (tid < b1 - b0) {
+    sL[tid] = A.lo[ro + 64 * (b0 + tid)];
+    sW[tid] = 64 * (int)A.nw[ro + 64 * (b0 + tid)];
+  }
+  __syncthreads();
+  const int* TB = A.tab + (size_t)c * A.nbmax * (size_t)A.wmax;
+  int* G = A.gtab + ((size_t)c * A.ngmax + g) * (size_t)A.wmax;
+  for (int u = tid; u < sW[0]; u += 256) {
+    int x = sL[0] + u;
+    for (int b = b0; b < b1 && x >= 0; b++) {
+      const int t = x - sL[b - b0];
+      x = (t >= 0 && t < sW[b - b0]) ? TB[(size_t)b * A.wmax + t] : -1;
+    }
+    G[u] = x;
+  }
+}
+
+// One wave per chain: the entry offset of every group of blocks, group after group through the group tables. Groups
+// go kPrAhead at a time: at their start the table entries around each one's predicted entry (fetched during the
+// previous kPrAhead) go to LDS and those of the next kPrAhead are fetched; the dependent steps then read LDS only
+// (the groups' first-row bands staged in LDS kPrChunk groups at a time). An entry outside the fetched window is
+// looked up in HBM; a group whose path leaves the tables is walked block by block. Leaves the generator state after
+// the last row.
+constexpr int kPrAhead = 4;
+constexpr int kPrWin = 256;   // table entries fetched around a group's predicted entry (4 a lane)
+constexpr int kPrChunk = 256;
 __global__ __launch_bounds__(64) void k_pr_chain(PrArgs A) {
   const int c = blockIdx.x, lane = threadIdx.x;
   const ChainDesc& d = A.desc[c];
-  const int n = d.n, nb = (n + 63) / 64;
+  const int n = d.n, nb = (n + 63) / 64, ng = (nb + kPrGroup - 1) / kPrGroup;
   const uint64_t ro = d.row_off;
   const double off = (double)d.offset;
   const uint2* X = A.slots + (size_t)c * A.K;
   const uint4* CK = A.ck + (size_t)c * (A.K / kPrSlots);
-  const int* TB = A.tab + (size_t)c * A.nbmax * (size_t)A.wmax;
-  uint32_t* entry = A.entry + (size_t)c * A.nbmax;
-  // first row of blocks [q0, q0 + kPrChunk + 2 kPrAhead): predicted offset, band start, band width
+  const int* GT = A.gtab + (size_t)c * A.ngmax * (size_t)A.wmax;
+  uint32_t* gentry = A.gentry + (size_t)c * A.ngmax;
+  // first row of groups [q0, q0 + kPrChunk + 2 kPrAhead): predicted offset, band start, band width
   __shared__ float sE[kPrChunk + 2 * kPrAhead];
   __shared__ int sL[kPrChunk + 2 * kPrAhead], sW[kPrChunk + 2 * kPrAhead];
-  // the current group's table windows: 128 entries from swu[j]
-  __shared__ int win[kPrAhead][128];
+  // the current groups' table windows: kPrWin entries from swu[j]
+  __shared__ __align__(16) int win[kPrAhead][kPrWin];
   __shared__ int swu[kPrAhead];
   int q0 = 0;
   auto stage = [&](int q) {
     for (int k = lane; k < kPrChunk + 2 * kPrAhead; k += 64) {
-      const int b = q + k;
-      sE[k] = b < nb ? A.E[ro + 64 * b] : 0.f;
-      sL[k] = b < nb ? A.lo[ro + 64 * b] : 0;
-      sW[k] = b < nb ? 64 * (int)A.nw[ro + 64 * b] : 0;
+      const int gi = q + k;
+      const uint64_t r = ro + (uint64_t)64 * kPrGroup * gi;
+      sE[k] = gi < ng ? A.E[r] : 0.f;
+      sL[k] = gi < ng ? A.lo[r] : 0;
+      sW[k] = gi < ng ? 64 * (int)A.nw[r] : 0;
     }
     wave_sync();
   };
-  // lane k: entries u0 + 2 k, u0 + 2 k + 1 of block b's table
-  auto fetch = [&](int b, int u0, int W, int& t0, int& t1) {
-    const int u = u0 + 2 * lane;
-    t0 = (b < nb && u >= 0 && u < W) ? TB[(size_t)b * A.wmax + u] : -2;
-    t1 = (b < nb && u + 1 >= 0 && u + 1 < W) ? TB[(size_t)b * A.wmax + u + 1] : -2;
+  // lane k: entries u0 + 4 k .. u0 + 4 k + 3 of group gi's table
+  auto fetch = [&](int gi, int u0, int W, int4& t) {
+    const int u = u0 + 4 * lane;
+    const int* T = GT + (size_t)gi * A.wmax;
+    t.x = (gi < ng && u >= 0 && u < W) ? T[u] : -2;
+    t.y = (gi < ng && u + 1 >= 0 && u + 1 < W) ? T[u + 1] : -2;
+    t.z = (gi < ng && u + 2 >= 0 && u + 2 < W) ? T[u + 2] : -2;
+    t.w = (gi < ng && u + 3 >= 0 && u + 3 < W) ? T[u + 3] : -2;
   };
   stage(0);
-  int t0[kPrAhead], t1[kPrAhead], u0v[kPrAhead];
+  int4 tq[kPrAhead];
+  int u0v[kPrAhead];
 #pragma unroll
   for (int j = 0; j < kPrAhead; j++) {
-    u0v[j] = (int)rintf(sE[j] - sE[0]) - sL[j] - 64;
-    fetch(j, u0v[j], sW[j], t0[j], t1[j]);
+    u0v[j] = (int)rintf(sE[j] - sE[0]) - sL[j] - kPrWin / 2;
+    fetch(j, u0v[j], sW[j], tq[j]);
   }
   uint32_t x = 0;
   unsigned spot = 0;
-  for (int b0 = 0; b0 < nb; b0 += kPrAhead) {
-    if (b0 >= q0 + kPrChunk) {
-      q0 = b0;
+  for (int g0 = 0; g0 < ng; g0 += kPrAhead) {
+    if (g0 >= q0 + kPrChunk) {
+      q0 = g0;
       stage(q0);
     }
-    // this group's windows to LDS (fetched a group ago), the next group's fetched from here
+    // these groups' windows to LDS (fetched kPrAhead groups ago), the next ones fetched from here
 #pragma unroll
     for (int j = 0; j < kPrAhead; j++) {
-      win[j][2 * lane] = t0[j];
-      win[j][2 * lane + 1] = t1[j];
+      *reinterpret_cast<int4*>(&win[j][4 * lane]) = tq[j];
       if (lane == j) swu[j] = u0v[j];
     }
     wave_sync();
-    const float eb0 = sE[b0 - q0];
+    const float eg0 = sE[g0 - q0];
 #pragma unroll
     for (int j = 0; j < kPrAhead; j++) {
-      const int bn = b0 + kPrAhead + j, k = bn - q0;
-      if (bn < nb) {
-        u0v[j] = (int)x + (int)rintf(sE[k] - eb0) - sL[k] - 64;
-        fetch(bn, u0v[j], sW[k], t0[j], t1[j]);
+      const int gn = g0 + kPrAhead + j, k = gn - q0;
+      if (gn < ng) {
+        u0v[j] = (int)x + (int)rintf(sE[k] - eg0) - sL[k] - kPrWin / 2;
+        fetch(gn, u0v[j], sW[k], tq[j]);
       }
     }
-    // the group's steps: a plain loop (one copy of the walk below keeps the hot code small); lane j holds block
-    // b0 + j's band start, band width and window start, so that a step's only LDS read is its table entry
-    const int gL = lane < kPrAhead ? sL[b0 - q0 + lane] : 0, gW = lane < kPrAhead ? sW[b0 - q0 + lane] : 0;
+    // lane j holds group g0 + j's band start, band width and window start
+    const int gL = lane < kPrAhead ? sL[g0 - q0 + lane] : 0, gW = lane < kPrAhead ? sW[g0 - q0 + lane] : 0;
     const int gU = lane < kPrAhead ? swu[lane] : 0;
 #pragma unroll 1
     for (int j = 0; j < kPrAhead; j++) {
-      const int b = b0 + j, k = b - q0;
-      if (b < nb) {
-        const int Lb = __builtin_amdgcn_readlane(gL, j), Wb = __builtin_amdgcn_readlane(gW, j);
-        if (lane == 0) entry[b] = x;
-        const int u = (int)x - Lb;
+      const int gi = g0 + j;
+      if (gi < ng) {
+        if (lane == 0) gentry[gi] = x;
+        const int u = (int)x - __builtin_amdgcn_readlane(gL, j);
         int v = -1;
-        if (u >= 0 && u < Wb) {
+        if (u >= 0 && u < __builtin_amdgcn_readlane(gW, j)) {
           const int kk = u - __builtin_amdgcn_readlane(gU, j);
-          v = (kk >= 0 && kk < 128) ? win[j][kk] : TB[(size_t)b * A.wmax + u];
+          v = (kk >= 0 && kk < kPrWin) ? win[j][kk] : GT[(size_t)gi * A.wmax + u];
         }
-        if (v >= 0) {
+        if (v >= 0)
           x = (uint32_t)v;
-        } else {
-          const int i0 = 64 * b;
-          const PrWin cur = pr_window(A, pr_meta(A, ro, n, i0, lane), ro, n, i0, x, sE[k], lane);
-          x = pr_walk_window(A, cur, X, CK, ro, off, i0, min(64, n - i0), x, spot, false, lane);
-        }
+        else
+          x = pr_blocks_walk(A, c, n, ro, off, X, CK, kPrGroup * gi, min(nb, kPrGroup * gi + kPrGroup), x, nullptr,
+                             spot, lane);
       }
     }
     wave_sync();
@@ -2041,6 +2098,19 @@ __global__ __launch_bounds__(64) void k_pr_chain(PrArgs A) {
     cs->rng[2] = g.s2;
     cs->rng[3] = g.s3;
   }
+}
+
+// One wave per group: the entry offset of each of its blocks, from the group's.
+__global__ __launch_bounds__(64) void k_pr_expand(PrArgs A) {
+  const int c = blockIdx.y, g = blockIdx.x, lane = threadIdx.x;
+  const ChainDesc& d = A.desc[c];
+  const int n = d.n, nb = (n + 63) / 64, b0 = kPrGroup * g;
+  if (b0 >= nb) return;
+  const uint64_t ro = d.row_off;
+  unsigned spot = 0;
+  (void)pr_blocks_walk(A, c, n, ro, (double)d.offset, A.slots + (size_t)c * A.K,
+                       A.ck + (size_t)c * (A.K / kPrSlots), b0, min(nb, b0 + kPrGroup),
+                       A.gentry[(size_t)c * A.ngmax + g], A.entry + (size_t)c * A.nbmax, spot, lane);
 }
 
 // One wave per block of 64 rows: each row's accepting slot, from the block's entry offset.
@@ -3336,6 +3406,8 @@ struct fbh_batch {
   unsigned* d_spot = nullptr;
   int* d_tab = nullptr;
   uint32_t* d_entry = nullptr;
+  int* d_gtab = nullptr;
+  uint32_t* d_gentry = nullptr;
 };
 
 static void free_batch(fbh_batch* b) {
@@ -3346,7 +3418,7 @@ static void free_batch(fbh_batch* b) {
                   b->d_img,  b->d_lps,   b->d_afx,   b->d_X,     b->d_rg,   b->d_px, b->d_prg, b->d_ptg,
                   b->d_pcount, b->d_idx, b->d_xbuf, b->d_xctr, b->d_order, b->d_pent, b->d_pq,
                   b->d_jump,  b->d_slots, b->d_ck,    b->d_E,     b->d_lo,   b->d_nw, b->d_mask, b->d_acc,
-                  b->d_spot,  b->d_ev,    b->d_tab,   b->d_entry, b->d_btot};
+                  b->d_spot,  b->d_ev,    b->d_tab,   b->d_entry, b->d_btot, b->d_gtab, b->d_gentry};
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto g : b->gexecs) (void)hipGraphExecDestroy(g);
   for (auto g : b->graphs) (void)hipGraphDestroy(g);
@@ -3420,6 +3492,7 @@ static void host_jump(const std::vector<uint32_t>& J, const uint32_t s[4], uint6
 }
 
 static int pr_blocks(const fbh_batch* b) { return (std::max(b->nmax, 1) + 63) / 64; }
+static int pr_groups(const fbh_batch* b) { return (pr_blocks(b) + kPrGroup - 1) / kPrGroup; }
 static size_t pr_table_lds(int nwmax) { return (size_t)64 * nwmax * 8 + 64 * 8; }
 
 // buffers of the device-wide redraw; K = 0: room for twice the longest chain's rows and its widest band (an attempt
@@ -3449,6 +3522,8 @@ static int pr_alloc(fbh_batch* b, uint32_t K) {
         (rc = dalloc(&b->d_mask, ro * (size_t)nwmax)) || (rc = dalloc(&b->d_acc, ro)) ||
         (rc = dalloc(&b->d_tab, (size_t)b->C * pr_blocks(b) * 64 * (size_t)nwmax)) ||
         (rc = dalloc(&b->d_entry, (size_t)b->C * pr_blocks(b))) ||
+        (rc = dalloc(&b->d_gtab, (size_t)b->C * pr_groups(b) * 64 * (size_t)nwmax)) ||
+        (rc = dalloc(&b->d_gentry, (size_t)b->C * pr_groups(b))) ||
         (rc = dalloc(&b->d_spot, (size_t)b->C)))
       return rc;
     if (hipMemset(b->d_spot, 0, sizeof(unsigned) * b->C) != hipSuccess) {
@@ -3500,6 +3575,9 @@ static PrArgs make_pr_args(const fbh_batch* b) {
   A.tab = b->d_tab;
   A.entry = b->d_entry;
   A.nbmax = pr_blocks(b);
+  A.gtab = b->d_gtab;
+  A.gentry = b->d_gentry;
+  A.ngmax = pr_groups(b);
   A.wmax = 64 * b->pr_nwmax;
   return A;
 }
@@ -3514,7 +3592,9 @@ static hipError_t pr_launch(const fbh_batch* b) {
   hipLaunchKernelGGL(k_pr_gen, dim3((A.K / kPrSlots + 255) / 256, C), dim3(256), 0, b->stream, A);
   hipLaunchKernelGGL(k_pr_decide, dim3((nm + 3) / 4, C), dim3(256), 0, b->stream, A);
   hipLaunchKernelGGL(k_pr_table, dim3((unsigned)A.nbmax, C), dim3(256), pr_table_lds(A.nwmax), b->stream, A);
+  hipLaunchKernelGGL(k_pr_compose, dim3((unsigned)A.ngmax, C), dim3(256), 0, b->stream, A);
   hipLaunchKernelGGL(k_pr_chain, dim3(C), dim3(64), 0, b->stream, A);
+  hipLaunchKernelGGL(k_pr_expand, dim3((unsigned)A.ngmax, C), dim3(64), 0, b->stream, A);
   hipLaunchKernelGGL(k_pr_walk, dim3((unsigned)A.nbmax, C), dim3(64), 0, b->stream, A);
   hipLaunchKernelGGL(k_pr_value, dim3((nm + 255) / 256, C), dim3(256), 0, b->stream, A);
   return hipGetLastError();

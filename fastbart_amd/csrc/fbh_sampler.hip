@@ -1574,8 +1574,10 @@ __global__ void k_trmean(const double* TM, float* out, int n, int ndpost) {
 //                images wherever rounding could matter -- each bit is the reference's decision) into 64-bit masks;
 //   k_pr_table:  per block of 64 rows and every entry offset in its first row's band, the offset at which the
 //                block's last row accepts, following the decision bits;
-//   k_pr_chain:  one wave per chain chains the blocks through those tables (the entry offset of every block) and
-//                leaves the generator state after the last row; a block whose path leaves the tables is walked;
+//   k_pr_compose, k_pr_chain, k_pr_expand: those tables composed per group of 8 blocks; one wave per chain chains
+//                the groups (the entry offset of every group) and leaves the generator state after the last row (a
+//                group whose path leaves the tables is walked block by block); one wave per group then gives each
+//                of its blocks' entry offsets;
 //   k_pr_walk:   one wave per block follows the path from its entry offset through the masks (a ballot resolves
 //                the run of rows that accept at the current offset) to each row's accepting slot; an offset outside
 //                a row's band, or a slot past K, is decided on the spot (exact; outputs past K are stepped from the

@@ -137,3 +137,23 @@ def test_gpu_probit_redraw_large_chain_matches_oracle(oracle):
         info = b.probit_info
     _say("attempts decided outside the bands:", info["spot"])
     assert info["spot"] < 0.001 * 65536 * 4
+
+
+@pytest.mark.parametrize("offset", [-2.5, 3.0])
+@pytest.mark.parametrize("redraw", [1, 2])
+def test_gpu_probit_redraw_far_offsets_match_oracle(oracle, offset, redraw):
+    """A binary offset far from the data (most attempts then truncate beyond the mean: rtnorm's exponential branch,
+    :286-291, with its exact-image fallback) -- both redraws against the oracle."""
+    pb = dict(W.small_problem(3000, 3, 15, 911, probit=True, ntree=10))
+    pb["offset"] = float(np.float32(offset))
+    m, nskip, nd = 10, 3, 2
+    ch = oracle.chain(pb, m, nskip, nd, 91, probit=1)
+    _say("far offset", offset, "redraw", redraw)
+    with sampler.BatchSampler([pb], m, nskip, nd, [91], probit=1, probit_redraw=redraw) as b:
+        _check(b, [ch], "initial")
+        for k in range(nskip + nd):
+            b.run(1)
+            ch.sweeps(1)
+            _check(b, [ch], k)
+        out = b.fetch(0)
+    assert same_bits(out["trees"].reshape(-1), ch.trees_out)

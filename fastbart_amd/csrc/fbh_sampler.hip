@@ -1805,9 +1805,9 @@ __global__ __launch_bounds__(256) void k_pr_decide(PrArgs A) {
   }
 }
 
-// A lane's view of its row in a window of 64 rows, loaded in two stages one window apart (the second stage's loads
-// depend on the first's): the row's predicted offset and band, then three mask words around the offset predicted
-// from the walk's position one window earlier, and the offsets they decide, [vlo, vhi).
+// A lane's view of its row in a window of 64 rows: the row's predicted offset and band (pr_meta), then three mask
+// words around the offset predicted for the row from a known offset rb at the row whose predicted offset is eb
+// (pr_window), and the offsets they decide, [vlo, vhi).
 struct PrMeta {
   float e;
   int lo, nw;
@@ -1861,8 +1861,9 @@ __device__ uint32_t pr_spot(const PrArgs& A, const uint2* X, const uint4* CK, ui
 // current offset is resolved by one ballot, then the first row that does not moves the offset to its next accepting
 // one (from its words, or decided on the spot). Writes each row's accepting slot if `write`; returns the offset at
 // which the last row accepts (the next row's starting offset).
-__device__ __forceinline__ uint32_t pr_walk_window(const PrArgs& A, const PrWin& cur, const uint2* X, const uint4* CK, uint64_t ro,
-                                   double off, int i0, int reff, uint32_t r, unsigned& spot, bool write, int lane) {
+__device__ __forceinline__ uint32_t pr_walk_window(const PrArgs& A, const PrWin& cur, const uint2* X, const uint4* CK,
+                                                   uint64_t ro, double off, int i0, int reff, uint32_t r,
+                                                   unsigned& spot, bool write, int lane) {
   const unsigned long long live = reff >= 64 ? ~0ull : ((1ull << reff) - 1ull);
   int done = 0;
   uint32_t accv = 0;

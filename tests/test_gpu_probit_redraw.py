@@ -157,3 +157,19 @@ def test_gpu_probit_redraw_far_offsets_match_oracle(oracle, offset, redraw):
             _check(b, [ch], k)
         out = b.fetch(0)
     assert same_bits(out["trees"].reshape(-1), ch.trees_out)
+
+
+@pytest.mark.parametrize("redraw", [1, 2])
+def test_gpu_probit_redraw_tiny_chains_match_oracle(oracle, redraw):
+    """Chains shorter than one 64-row window (one partial block and group), beside a longer one, against the oracle."""
+    probs = [W.small_problem(n, 2, 8, 950 + n, probit=True, ntree=4) for n in (7, 33, 63, 200)]
+    m, nskip, nd = 4, 3, 2
+    chs = [oracle.chain(pb, m, nskip, nd, 95 + c, probit=1) for c, pb in enumerate(probs)]
+    _say("tiny chains, redraw", redraw)
+    with sampler.BatchSampler(probs, m, nskip, nd, [95 + c for c in range(4)], probit=1, probit_redraw=redraw) as b:
+        _check(b, chs, "initial")
+        for k in range(nskip + nd):
+            b.run(1)
+            for ch in chs:
+                ch.sweeps(1)
+            _check(b, chs, k)

@@ -173,3 +173,21 @@ def test_gpu_probit_redraw_tiny_chains_match_oracle(oracle, redraw):
             for ch in chs:
                 ch.sweeps(1)
             _check(b, chs, k)
+
+
+@pytest.mark.parametrize("redraw", [1, 2])
+def test_gpu_probit_fx64_and_node_cap_match_oracle(oracle, redraw):
+    """Probit chains with fixed-point statistics and a small node cap (births refused at the cap), either redraw,
+    against the oracle's FX64 twin: every output."""
+    probs = [W.small_problem(n, 3, 20, 970 + n, probit=True, ntree=12) for n in (1500, 2600)]
+    m, nskip, nd, cap = 12, 8, 4, 5
+    _say("probit FX64, node cap", cap, "redraw", redraw)
+    with sampler.BatchSampler(probs, m, nskip, nd, [97, 98], probit=1, stats_mode=1, node_cap=cap,
+                              probit_redraw=redraw) as b:
+        b.run(nskip + nd)
+        outs = [b.fetch(c) for c in range(2)]
+    for c, pb in enumerate(probs):
+        e = oracle.fit(pb, m, nskip, nd, 97 + c, probit=1, stats_mode=1, node_cap=cap)
+        for k in ("sigma", "trmean", "trees", "stats"):
+            assert same_bits(outs[c][k], e[k]), (c, k)
+    assert any(o["stats"][2] > 0 for o in outs)  # the cap was reached

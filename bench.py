@@ -1,6 +1,6 @@
 """Benchmark: tree-updates/sec/GPU of the batched plain sampler, plus min-ESS/sec.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch512|fwbart_n4096|fpbart_n24576|maxshape64|learned512|predict600|predict_probit100]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch512|fwbart_n4096|fpbart_n24576|fpbart_n24576_fx64|maxshape64|learned512|predict600|predict_probit100]
 
 Default workload (the headline): batched fwbart -- 512 independent chains in one persistent launch, chain c with
 n_c ~ U{3091..4573} rows, p_c ~ U{15..19} predictors, x ~ U(0,1) with 255 cutpoints, y = 10 sin(pi x1 x2) +
@@ -14,6 +14,8 @@ learned512 comparison at 1,000 kept draws on the same chains (about a minute; --
 Other workloads (extra lines, not the headline):
   fwbart_n4096   one chain, n = 4,096, p = 16, 100 cutpoints, ntree 200, nskip 1000, ndpost 100
   fpbart_n24576  one probit chain, n = 24,576, p = 3, ntree 50, nskip 1000, ndpost 100
+  fpbart_n24576_fx64  the same chain with FX64 statistics and the automatic row split (8 workgroups), its latents
+                 redrawn over all of them by the device-wide kernels (results: the oracle's FX64 twin)
   maxshape64     64 chains, n = 65,536, p = 32 (24 continuous + 8 integer columns), node cap 64, FX64 statistics,
                  ntree 200, nskip 500, ndpost 100 (default here: 5 warmup + 40 timed sweeps)
   learned512     the 512-chain batch twice: the plain fit, then the learned schedule -- plain sweeps, then sweeps with
@@ -51,6 +53,7 @@ WORKLOADS = {
     "batch512": (512, 200, 1000, 100, 0, 0, 0, 20, None),
     "fwbart_n4096": (1, 200, 1000, 100, 0, 0, 0, 20, None),
     "fpbart_n24576": (1, 50, 1000, 100, 1, 0, 0, 20, None),
+    "fpbart_n24576_fx64": (1, 50, 1000, 100, 1, 1, 0, 20, None),
     "maxshape64": (64, 200, 500, 100, 0, 1, 64, 5, 40),
     "learned512": (512, 200, 1000, 100, 0, 0, 0, 20, None),
     "predict600": (6, 200, 1000, 100, 0, 0, 0, 3, 20),
@@ -69,7 +72,7 @@ def make_chain(workload: str, c: int) -> dict:
         return Wk.batch512_chain(c)
     if workload == "fwbart_n4096":
         return Wk.workload_fwbart_n4096()
-    if workload == "fpbart_n24576":
+    if workload in ("fpbart_n24576", "fpbart_n24576_fx64"):
         return Wk.workload_fpbart_n24576()
     if workload == "maxshape64":
         return Wk.maxshape64_chain(c)
@@ -84,6 +87,8 @@ def describe(workload: str, chains: int, nskip: int, ndpost: int) -> str:
                         f"REF16 statistics",
         "fpbart_n24576": f"fpbart probit, 1 chain, n 24576, p 3, ntree 50, nskip {nskip}, ndpost {ndpost}, "
                          f"REF16 statistics",
+        "fpbart_n24576_fx64": f"fpbart probit, 1 chain, n 24576, p 3, ntree 50, nskip {nskip}, ndpost {ndpost}, "
+                              f"FX64 statistics, automatic row split",
         "maxshape64": f"max-shape batched fwbart, {chains} chains, n 65536, p 32, 255 cutpoints, node cap 64, ntree 200, "
                       f"nskip {nskip}, ndpost {ndpost}, FX64 statistics",
         "learned512": f"batched fwbart with the learned birth proposal, {chains} chains, n_i~U{{3091..4573}}, "
@@ -680,7 +685,7 @@ def main() -> int:
     seeds = [1 + c for c in range(chains)]
     rows_total = sum(p["q"].shape[1] for p in probs)
     log(f"{wl}: generated {chains} chains in {time.perf_counter() - t0:.1f} s; total rows {rows_total}")
-    split = args.row_split if args.row_split is not None else (-1 if wl == "maxshape64" else 1)
+    split = args.row_split if args.row_split is not None else (-1 if wl in ("maxshape64", "fpbart_n24576_fx64") else 1)
     tune = args.packing == "auto"
     b = sampler.BatchSampler(probs, M, nskip, ndpost, seeds, probit=probit, keep_trees=1, stats_mode=stats_mode,
                              node_cap=node_cap, sweeps_per_launch=spl, row_split=split,

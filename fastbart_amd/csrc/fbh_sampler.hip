@@ -1227,7 +1227,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
     int n_birth = cs->n_birth, n_death = cs->n_death, n_cap = cs->n_cap;
     uint32_t xseq = ufl(cs->xseq);
     int xerr = (int)ufl((uint32_t)cs->xerr);
-    const bool split = FX && !PB && c.cd.split > 1;
+    const bool split = FX && c.cd.split > 1;
     for (int it = a.it0; it < a.it1; it++) {
       const bool keep = it >= a.nskip;
       if (lane == 0) {
@@ -1587,8 +1587,21 @@ __global__ void k_trmean(const double* TM, float* out, int n, int ndpost) {
 constexpr int kJumpLevels = 20;     // thread indices below 2^20: K below 2^26 slots
 constexpr int kPrSlots = 64;        // slots per generating thread (and between checkpoints)
 
+// A logical chain of the redraw and its rows in the row pools: one range, or, for a chain whose rows are split over W
+// workgroups, W ranges of `share` rows (a multiple of 64: no block of 64 rows straddles two) at a stride of share_pad.
+struct PrChain {
+  uint64_t row_off;  // pool position of row 0
+  int n, share, share_pad, W;
+  int dc0;           // device chain (workgroup) of member 0; the members are dc0 .. dc0 + W - 1
+  float offset;
+  __device__ __forceinline__ uint64_t operator()(int i) const {
+    return W == 1 ? row_off + (uint64_t)i
+                  : row_off + (uint64_t)(i / share) * (uint64_t)share_pad + (uint64_t)(i % share);
+  }
+};
+
 struct PrArgs {
-  const ChainDesc* desc;
+  const PrChain* pc;          // [CL] the logical chains
   ChainState* state;
   const double* AF;
   const uint8_t* ybin;
@@ -1681,14 +1694,13 @@ __device__ __forceinline__ double2 pr_block_scan(double e, double v, double& pe,
 // per row: expected rejections and variance; per 256 rows: their totals
 __global__ __launch_bounds__(256) void k_pr_rows(PrArgs A) {
   const int c = blockIdx.y;
-  const ChainDesc& d = A.desc[c];
-  if ((int)blockIdx.x * 256 >= d.n) return;
+  const PrChain& R = A.pc[c];
+  if ((int)blockIdx.x * 256 >= R.n) return;
   const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
-  const uint64_t ro = d.row_off;
   double e = 0., v = 0.;
-  if (i < d.n) {
-    pr_rejections(latent_row(A.AF[ro + i], A.ybin[ro + i], (double)d.offset), e, v);
-    A.ev[ro + i] = make_float2((float)e, (float)v);
+  if (i < R.n) {
+    pr_rejections(latent_row(A.AF[R(i)], A.ybin[R(i)], (double)R.offset), e, v);
+    A.ev[R(i)] = make_float2((float)e, (float)v);
   }
   double pe, pv;
   const double2 tot = pr_block_scan((double)(float)e, (double)(float)v, pe, pv);
@@ -1698,7 +1710,7 @@ __global__ __launch_bounds__(256) void k_pr_rows(PrArgs A) {
 // per chain: exclusive prefix sums of the 256-row totals
 __global__ __launch_bounds__(256) void k_pr_band(PrArgs A) {
   const int c = blockIdx.x;
-  const int nb = (A.desc[c].n + 255) / 256;
+  const int nb = (A.pc[c].n + 255) / 256;
   double2* T = A.btot + (size_t)c * A.nb256;
   double ce = 0., cv = 0.;
   for (int b0 = 0; b0 < nb; b0 += 256) {
@@ -1716,22 +1728,21 @@ __global__ __launch_bounds__(256) void k_pr_band(PrArgs A) {
 // per row: the predicted offset (its 256 rows' offset + the prefix within them) and the band around it
 __global__ __launch_bounds__(256) void k_pr_bandrows(PrArgs A) {
   const int c = blockIdx.y;
-  const ChainDesc& d = A.desc[c];
-  if ((int)blockIdx.x * 256 >= d.n) return;
+  const PrChain& R = A.pc[c];
+  if ((int)blockIdx.x * 256 >= R.n) return;
   const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
-  const uint64_t ro = d.row_off;
-  const float2 x = i < d.n ? A.ev[ro + i] : make_float2(0.f, 0.f);
+  const float2 x = i < R.n ? A.ev[R(i)] : make_float2(0.f, 0.f);
   double pe, pv;
   (void)pr_block_scan((double)x.x, (double)x.y, pe, pv);
-  if (i >= d.n) return;
+  if (i >= R.n) return;
   const double2 b = A.btot[(size_t)c * A.nb256 + blockIdx.x];
   pe += b.x;
   pv += b.y;
   const double h = 4.5 * sqrt(pv) + 64.;
   const int w = min(A.nwmax, max(3, (int)ceil(h / 32.)));
-  A.E[ro + i] = (float)pe;
-  A.lo[ro + i] = max(0, (int)floor(pe - 32. * w));
-  A.nw[ro + i] = (uint8_t)w;
+  A.E[R(i)] = (float)pe;
+  A.lo[R(i)] = max(0, (int)floor(pe - 32. * w));
+  A.nw[R(i)] = (uint8_t)w;
 }
 
 __global__ __launch_bounds__(256) void k_pr_gen(PrArgs A) {
@@ -1744,7 +1755,7 @@ __global__ __launch_bounds__(256) void k_pr_gen(PrArgs A) {
   for (int k = threadIdx.x; k < lev * 128; k += 256) J[k / 128][k % 128] = A.jump[k];
   __syncthreads();
   if (t >= nth) return;
-  const ChainState* cs = A.state + c;
+  const ChainState* cs = A.state + A.pc[c].dc0;
   uint32_t s0 = cs->rng[0], s1 = cs->rng[1], s2 = cs->rng[2], s3 = cs->rng[3];
   for (int k = 0; k < lev; k++) {
     uint32_t o0 = 0, o1 = 0, o2 = 0, o3 = 0;
@@ -1785,14 +1796,13 @@ __global__ __launch_bounds__(256) void k_pr_gen(PrArgs A) {
 
 __global__ __launch_bounds__(256) void k_pr_decide(PrArgs A) {
   const int c = blockIdx.y, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const ChainDesc& d = A.desc[c];
+  const PrChain& R = A.pc[c];
   const int i = (int)blockIdx.x * 4 + wave;
-  if (i >= d.n) return;
-  const uint64_t ro = d.row_off;
-  const LatentRow r = latent_row(A.AF[ro + i], A.ybin[ro + i], (double)d.offset);
-  const int lo = A.lo[ro + i], nw = A.nw[ro + i];
+  if (i >= R.n) return;
+  const LatentRow r = latent_row(A.AF[R(i)], A.ybin[R(i)], (double)R.offset);
+  const int lo = A.lo[R(i)], nw = A.nw[R(i)];
   const uint2* X = A.slots + (size_t)c * A.K;
-  unsigned long long* M = A.mask + (ro + i) * (uint64_t)A.nwmax;
+  unsigned long long* M = A.mask + R(i) * (uint64_t)A.nwmax;
   for (int w = 0; w < nw; w++) {
     const uint32_t slot = (uint32_t)(i + lo + 64 * w + lane);
     bool ok = false;
@@ -1816,24 +1826,24 @@ struct PrWin {
   unsigned long long w0, w1, w2;
   int vlo, vhi;
 };
-__device__ __forceinline__ PrMeta pr_meta(const PrArgs& A, uint64_t ro, int n, int i0, int lane) {
+__device__ __forceinline__ PrMeta pr_meta(const PrArgs& A, const PrChain& R, int n, int i0, int lane) {
   PrMeta M{0.f, 0, 3};
   const int i = i0 + lane;
   if (i < n) {
-    M.e = A.E[ro + i];
-    M.lo = A.lo[ro + i];
-    M.nw = A.nw[ro + i];
+    M.e = A.E[R(i)];
+    M.lo = A.lo[R(i)];
+    M.nw = A.nw[R(i)];
   }
   return M;
 }
-__device__ __forceinline__ PrWin pr_window(const PrArgs& A, const PrMeta& M, uint64_t ro, int n, int i0, uint32_t rb,
-                                           float eb, int lane) {
+__device__ __forceinline__ PrWin pr_window(const PrArgs& A, const PrMeta& M, const PrChain& R, int n, int i0,
+                                           uint32_t rb, float eb, int lane) {
   PrWin W{0, 0, 0, 0, 0};
   const int i = i0 + lane;
   if (i < n) {
     const float pred = (float)rb + (M.e - eb);
     const int q = min(max((int)floorf((pred - (float)M.lo) * (1.f / 64.f)) - 1, 0), M.nw - 3);
-    const unsigned long long* P = A.mask + (ro + i) * (uint64_t)A.nwmax + q;
+    const unsigned long long* P = A.mask + R(i) * (uint64_t)A.nwmax + q;
     W.w0 = P[0];
     W.w1 = P[1];
     W.w2 = P[2];
@@ -1845,9 +1855,9 @@ __device__ __forceinline__ PrWin pr_window(const PrArgs& A, const PrMeta& M, uin
 
 // row i's first accepting offset at or above `from`, decided here: 64 offsets at a time across the wave. The bound
 // only guarantees an exit (an attempt rejects with probability at most 1/2: 2^22 rejections in a row do not occur).
-__device__ uint32_t pr_spot(const PrArgs& A, const uint2* X, const uint4* CK, uint64_t ro, double off, int i,
+__device__ uint32_t pr_spot(const PrArgs& A, const uint2* X, const uint4* CK, const PrChain& R, double off, int i,
                             uint32_t from, int lane) {
-  const LatentRow r = latent_row(A.AF[ro + i], A.ybin[ro + i], off);
+  const LatentRow r = latent_row(A.AF[R(i)], A.ybin[R(i)], off);
   uint32_t o = from;
   for (int k = 0; k < (1 << 16); k++, o += 64) {
     const uint2 x = pr_slot(X, CK, A.K, (uint32_t)i + o + (uint32_t)lane);
@@ -1862,7 +1872,7 @@ __device__ uint32_t pr_spot(const PrArgs& A, const uint2* X, const uint4* CK, ui
 // one (from its words, or decided on the spot). Writes each row's accepting slot if `write`; returns the offset at
 // which the last row accepts (the next row's starting offset).
 __device__ __forceinline__ uint32_t pr_walk_window(const PrArgs& A, const PrWin& cur, const uint2* X, const uint4* CK,
-                                                   uint64_t ro, double off, int i0, int reff, uint32_t r,
+                                                   const PrChain& R, double off, int i0, int reff, uint32_t r,
                                                    unsigned& spot, bool write, int lane) {
   const unsigned long long live = reff >= 64 ? ~0ull : ((1ull << reff) - 1ull);
   int done = 0;
@@ -1890,13 +1900,13 @@ __device__ __forceinline__ uint32_t pr_walk_window(const PrArgs& A, const PrWin&
     if (nf >= 0) {
       r = (uint32_t)nf;
     } else {
-      r = pr_spot(A, X, CK, ro, off, i0 + f, (uint32_t)(-1 - nf), lane);
+      r = pr_spot(A, X, CK, R, off, i0 + f, (uint32_t)(-1 - nf), lane);
       spot++;
     }
     accv = lane == f ? r : accv;
     done = f + 1;
   }
-  if (write && lane < reff) A.acc[ro + i0 + lane] = (uint32_t)(i0 + lane) + accv;
+  if (write && lane < reff) A.acc[R(i0 + lane)] = (uint32_t)(i0 + lane) + accv;
   return r;
 }
 
@@ -1904,18 +1914,18 @@ __device__ __forceinline__ uint32_t pr_walk_window(const PrArgs& A, const PrWin&
 // row accepts, following the decision bits (-1 where the path leaves a row's band). Bands staged in LDS.
 __global__ __launch_bounds__(256) void k_pr_table(PrArgs A) {
   const int c = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
-  const ChainDesc& d = A.desc[c];
-  const int n = d.n, r0 = 64 * b;
+  const PrChain& R = A.pc[c];
+  const int n = R.n, r0 = 64 * b;
   if (r0 >= n) return;
   const int nr = min(64, n - r0), nwm = A.nwmax;
-  const uint64_t ro = d.row_off;
   extern __shared__ unsigned long long pr_sm[];  // [64][nwmax] band words, then lo[64], nw[64]
   int* slo = reinterpret_cast<int*>(pr_sm + 64 * nwm);
   int* snw = slo + 64;
-  for (int k = tid; k < nr * nwm; k += 256) pr_sm[k] = A.mask[(ro + r0) * (uint64_t)nwm + k];
+  // the block's rows are consecutive in the pools (a block never straddles two members of a split chain)
+  for (int k = tid; k < nr * nwm; k += 256) pr_sm[k] = A.mask[R(r0) * (uint64_t)nwm + k];
   for (int k = tid; k < nr; k += 256) {
-    slo[k] = A.lo[ro + r0 + k];
-    snw[k] = A.nw[ro + r0 + k];
+    slo[k] = A.lo[R(r0 + k)];
+    snw[k] = A.nw[R(r0 + k)];
   }
   __syncthreads();
   const int L = slo[0], Wd = 64 * snw[0];
@@ -1945,12 +1955,12 @@ __global__ __launch_bounds__(256) void k_pr_table(PrArgs A) {
 // Blocks [b0, b1) from entry offset x through the block tables, one dependent lookup a block (every block's band
 // start and width fetched first, lane j holding block b0 + j's); a block whose entry is outside its table, or whose
 // path leaves the bands, is walked row by row. Stores each block's entry offset if `entry`; returns the exit.
-__device__ __forceinline__ uint32_t pr_blocks_walk(const PrArgs& A, int c, int n, uint64_t ro, double off,
+__device__ __forceinline__ uint32_t pr_blocks_walk(const PrArgs& A, int c, int n, const PrChain& R, double off,
                                                    const uint2* X, const uint4* CK, int b0, int b1, uint32_t x,
                                                    uint32_t* entry, unsigned& spot, int lane) {
   const int* TB = A.tab + (size_t)c * A.nbmax * (size_t)A.wmax;
   const int bl = b0 + lane;
-  const int mL = lane < b1 - b0 ? A.lo[ro + 64 * bl] : 0, mW = lane < b1 - b0 ? 64 * (int)A.nw[ro + 64 * bl] : 0;
+  const int mL = lane < b1 - b0 ? A.lo[R(64 * bl)] : 0, mW = lane < b1 - b0 ? 64 * (int)A.nw[R(64 * bl)] : 0;
   for (int b = b0; b < b1; b++) {
     if (entry && lane == 0) entry[b] = x;
     const int t = (int)x - __builtin_amdgcn_readlane(mL, b - b0);
@@ -1959,8 +1969,8 @@ __device__ __forceinline__ uint32_t pr_blocks_walk(const PrArgs& A, int c, int n
       x = (uint32_t)v;
     } else {
       const int i0 = 64 * b;
-      const PrWin cur = pr_window(A, pr_meta(A, ro, n, i0, lane), ro, n, i0, x, A.E[ro + i0], lane);
-      x = pr_walk_window(A, cur, X, CK, ro, off, i0, min(64, n - i0), x, spot, false, lane);
+      const PrWin cur = pr_window(A, pr_meta(A, R, n, i0, lane), R, n, i0, x, A.E[R(i0)], lane);
+      x = pr_walk_window(A, cur, X, CK, R, off, i0, min(64, n - i0), x, spot, false, lane);
     }
   }
   return x;
@@ -1971,15 +1981,14 @@ __device__ __forceinline__ uint32_t pr_blocks_walk(const PrArgs& A, int c, int n
 constexpr int kPrGroup = 8;
 __global__ __launch_bounds__(256) void k_pr_compose(PrArgs A) {
   const int c = blockIdx.y, g = blockIdx.x, tid = threadIdx.x;
-  const ChainDesc& d = A.desc[c];
-  const int n = d.n, nb = (n + 63) / 64, b0 = kPrGroup * g;
+  const PrChain& R = A.pc[c];
+  const int n = R.n, nb = (n + 63) / 64, b0 = kPrGroup * g;
   if (b0 >= nb) return;
   const int b1 = min(nb, b0 + kPrGroup);
-  const uint64_t ro = d.row_off;
   __shared__ int sL[kPrGroup], sW[kPrGroup];
   if (tid < b1 - b0) {
-    sL[tid] = A.lo[ro + 64 * (b0 + tid)];
-    sW[tid] = 64 * (int)A.nw[ro + 64 * (b0 + tid)];
+    sL[tid] = A.lo[R(64 * (b0 + tid))];
+    sW[tid] = 64 * (int)A.nw[R(64 * (b0 + tid))];
   }
   __syncthreads();
   const int* TB = A.tab + (size_t)c * A.nbmax * (size_t)A.wmax;
@@ -2005,10 +2014,9 @@ constexpr int kPrWin = 256;   // table entries fetched around a group's predicte
 constexpr int kPrChunk = 256;
 __global__ __launch_bounds__(64) void k_pr_chain(PrArgs A) {
   const int c = blockIdx.x, lane = threadIdx.x;
-  const ChainDesc& d = A.desc[c];
-  const int n = d.n, nb = (n + 63) / 64, ng = (nb + kPrGroup - 1) / kPrGroup;
-  const uint64_t ro = d.row_off;
-  const double off = (double)d.offset;
+  const PrChain& R = A.pc[c];
+  const int n = R.n, nb = (n + 63) / 64, ng = (nb + kPrGroup - 1) / kPrGroup;
+  const double off = (double)R.offset;
   const uint2* X = A.slots + (size_t)c * A.K;
   const uint4* CK = A.ck + (size_t)c * (A.K / kPrSlots);
   const int* GT = A.gtab + (size_t)c * A.ngmax * (size_t)A.wmax;
@@ -2023,7 +2031,7 @@ __global__ __launch_bounds__(64) void k_pr_chain(PrArgs A) {
   auto stage = [&](int q) {
     for (int k = lane; k < kPrChunk + 2 * kPrAhead; k += 64) {
       const int gi = q + k;
-      const uint64_t r = ro + (uint64_t)64 * kPrGroup * gi;
+      const uint64_t r = gi < ng ? R(64 * kPrGroup * gi) : 0;
       sE[k] = gi < ng ? A.E[r] : 0.f;
       sL[k] = gi < ng ? A.lo[r] : 0;
       sW[k] = gi < ng ? 64 * (int)A.nw[r] : 0;
@@ -2087,15 +2095,16 @@ __global__ __launch_bounds__(64) void k_pr_chain(PrArgs A) {
         if (v >= 0)
           x = (uint32_t)v;
         else
-          x = pr_blocks_walk(A, c, n, ro, off, X, CK, kPrGroup * gi, min(nb, kPrGroup * gi + kPrGroup), x, nullptr,
+          x = pr_blocks_walk(A, c, n, R, off, X, CK, kPrGroup * gi, min(nb, kPrGroup * gi + kPrGroup), x, nullptr,
                              spot, lane);
       }
     }
     wave_sync();
   }
-  if (lane == 0) {
+  // every member of a split chain continues from the same generator state
+  if (lane < R.W) {
     const Rng g = pr_state_at(CK, A.K, (uint32_t)n + x);
-    ChainState* cs = A.state + c;
+    ChainState* cs = A.state + R.dc0 + lane;
     cs->rng[0] = g.s0;
     cs->rng[1] = g.s1;
     cs->rng[2] = g.s2;
@@ -2106,12 +2115,11 @@ __global__ __launch_bounds__(64) void k_pr_chain(PrArgs A) {
 // One wave per group: the entry offset of each of its blocks, from the group's.
 __global__ __launch_bounds__(64) void k_pr_expand(PrArgs A) {
   const int c = blockIdx.y, g = blockIdx.x, lane = threadIdx.x;
-  const ChainDesc& d = A.desc[c];
-  const int n = d.n, nb = (n + 63) / 64, b0 = kPrGroup * g;
+  const PrChain& R = A.pc[c];
+  const int n = R.n, nb = (n + 63) / 64, b0 = kPrGroup * g;
   if (b0 >= nb) return;
-  const uint64_t ro = d.row_off;
   unsigned spot = 0;
-  (void)pr_blocks_walk(A, c, n, ro, (double)d.offset, A.slots + (size_t)c * A.K,
+  (void)pr_blocks_walk(A, c, n, R, (double)R.offset, A.slots + (size_t)c * A.K,
                        A.ck + (size_t)c * (A.K / kPrSlots), b0, min(nb, b0 + kPrGroup),
                        A.gentry[(size_t)c * A.ngmax + g], A.entry + (size_t)c * A.nbmax, spot, lane);
 }
@@ -2119,27 +2127,25 @@ __global__ __launch_bounds__(64) void k_pr_expand(PrArgs A) {
 // One wave per block of 64 rows: each row's accepting slot, from the block's entry offset.
 __global__ __launch_bounds__(64) void k_pr_walk(PrArgs A) {
   const int c = blockIdx.y, b = blockIdx.x, lane = threadIdx.x;
-  const ChainDesc& d = A.desc[c];
-  const int n = d.n, i0 = 64 * b;
+  const PrChain& R = A.pc[c];
+  const int n = R.n, i0 = 64 * b;
   if (i0 >= n) return;
-  const uint64_t ro = d.row_off;
   const uint32_t r = A.entry[(size_t)c * A.nbmax + b];
-  const PrWin cur = pr_window(A, pr_meta(A, ro, n, i0, lane), ro, n, i0, r, A.E[ro + i0], lane);
+  const PrWin cur = pr_window(A, pr_meta(A, R, n, i0, lane), R, n, i0, r, A.E[R(i0)], lane);
   unsigned spot = 0;
-  (void)pr_walk_window(A, cur, A.slots + (size_t)c * A.K, A.ck + (size_t)c * (A.K / kPrSlots), ro,
-                       (double)d.offset, i0, min(64, n - i0), r, spot, true, lane);
+  (void)pr_walk_window(A, cur, A.slots + (size_t)c * A.K, A.ck + (size_t)c * (A.K / kPrSlots), R,
+                       (double)R.offset, i0, min(64, n - i0), r, spot, true, lane);
   if (lane == 0 && spot) atomicAdd(A.spot + c, spot);
 }
 
 __global__ __launch_bounds__(256) void k_pr_value(PrArgs A) {
   const int c = blockIdx.y;
-  const ChainDesc& d = A.desc[c];
+  const PrChain& R = A.pc[c];
   const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
-  if (i >= d.n) return;
-  const uint64_t ro = d.row_off;
-  const uint2 x = pr_slot(A.slots + (size_t)c * A.K, A.ck + (size_t)c * (A.K / kPrSlots), A.K, A.acc[ro + i]);
-  const LatentRow r = latent_row(A.AF[ro + i], A.ybin[ro + i], (double)d.offset);
-  A.Y[ro + i] = latent_value(r, x.x, x.y, A.tb);
+  if (i >= R.n) return;
+  const uint2 x = pr_slot(A.slots + (size_t)c * A.K, A.ck + (size_t)c * (A.K / kPrSlots), A.K, A.acc[R(i)]);
+  const LatentRow r = latent_row(A.AF[R(i)], A.ybin[R(i)], (double)R.offset);
+  A.Y[R(i)] = latent_value(r, x.x, x.y, A.tb);
 }
 
 // ------------------------------------------------------------------ launch variants
@@ -3400,6 +3406,7 @@ struct fbh_batch {
   uint2* d_slots = nullptr;
   uint4* d_ck = nullptr;
   float2* d_ev = nullptr;
+  PrChain* d_pc = nullptr;
   double2* d_btot = nullptr;
   float* d_E = nullptr;
   int* d_lo = nullptr;
@@ -3421,7 +3428,7 @@ static void free_batch(fbh_batch* b) {
                   b->d_img,  b->d_lps,   b->d_afx,   b->d_X,     b->d_rg,   b->d_px, b->d_prg, b->d_ptg,
                   b->d_pcount, b->d_idx, b->d_xbuf, b->d_xctr, b->d_order, b->d_pent, b->d_pq,
                   b->d_jump,  b->d_slots, b->d_ck,    b->d_E,     b->d_lo,   b->d_nw, b->d_mask, b->d_acc,
-                  b->d_spot,  b->d_ev,    b->d_tab,   b->d_entry, b->d_btot, b->d_gtab, b->d_gentry};
+                  b->d_spot,  b->d_ev,    b->d_tab,   b->d_entry, b->d_btot, b->d_gtab, b->d_gentry, b->d_pc};
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto g : b->gexecs) (void)hipGraphExecDestroy(g);
   for (auto g : b->graphs) (void)hipGraphDestroy(g);
@@ -3494,7 +3501,13 @@ static void host_jump(const std::vector<uint32_t>& J, const uint32_t s[4], uint6
   std::memcpy(out, st, 16);
 }
 
-static int pr_blocks(const fbh_batch* b) { return (std::max(b->nmax, 1) + 63) / 64; }
+// rows of the longest logical chain (a split chain's members together)
+static int pr_nmax(const fbh_batch* b) {
+  int m = 1;
+  for (int c = 0; c < b->CL; c++) m = std::max(m, b->hdesc[(size_t)c * b->W].n_total);
+  return m;
+}
+static int pr_blocks(const fbh_batch* b) { return (pr_nmax(b) + 63) / 64; }
 static int pr_groups(const fbh_batch* b) { return (pr_blocks(b) + kPrGroup - 1) / kPrGroup; }
 static size_t pr_table_lds(int nwmax) { return (size_t)64 * nwmax * 8 + 64 * 8; }
 
@@ -3502,8 +3515,9 @@ static size_t pr_table_lds(int nwmax) { return (size_t)64 * nwmax * 8 + 64 * 8; 
 // rejects with probability at most 1/2, so a chain uses about n + E_n <= 2 n slots); a smaller K (a multiple of 64)
 // sends the slots past it to the walk's own stepping
 static int pr_alloc(fbh_batch* b, uint32_t K) {
-  const int nwmax = std::min(255, (int)std::ceil((9.0 * std::sqrt(2.0 * std::max(b->nmax, 1)) + 128.0) / 64.0) + 1);
-  const uint64_t Kd = ((2ull * (uint64_t)b->nmax + 64ull * (uint64_t)nwmax + 4096ull) + 63ull) / 64ull * 64ull;
+  const int nm = pr_nmax(b);
+  const int nwmax = std::min(255, (int)std::ceil((9.0 * std::sqrt(2.0 * nm) + 128.0) / 64.0) + 1);
+  const uint64_t Kd = ((2ull * (uint64_t)nm + 64ull * (uint64_t)nwmax + 4096ull) + 63ull) / 64ull * 64ull;
   if (K == 0) K = (uint32_t)Kd;
   if (K % kPrSlots || K < (uint32_t)kPrSlots || (uint64_t)K / kPrSlots > (1ull << kJumpLevels)) {
     set_error(FBH_EINVAL, "probit redraw: slots must be a positive multiple of 64 below 2^26");
@@ -3521,7 +3535,9 @@ static int pr_alloc(fbh_batch* b, uint32_t K) {
   }
   if (!b->d_E) {
     const size_t ro = b->rows_total;
-    if ((rc = dalloc(&b->d_ev, ro)) || (rc = dalloc(&b->d_btot, (size_t)b->C * ((b->nmax + 255) / 256 + 1))) || (rc = dalloc(&b->d_E, ro)) || (rc = dalloc(&b->d_lo, ro)) || (rc = dalloc(&b->d_nw, ro)) ||
+    if ((rc = dalloc(&b->d_pc, (size_t)b->CL)) || (rc = dalloc(&b->d_ev, ro)) ||
+        (rc = dalloc(&b->d_btot, (size_t)b->C * ((nm + 255) / 256 + 1))) || (rc = dalloc(&b->d_E, ro)) ||
+        (rc = dalloc(&b->d_lo, ro)) || (rc = dalloc(&b->d_nw, ro)) ||
         (rc = dalloc(&b->d_mask, ro * (size_t)nwmax)) || (rc = dalloc(&b->d_acc, ro)) ||
         (rc = dalloc(&b->d_tab, (size_t)b->C * pr_blocks(b) * 64 * (size_t)nwmax)) ||
         (rc = dalloc(&b->d_entry, (size_t)b->C * pr_blocks(b))) ||
@@ -3534,6 +3550,24 @@ static int pr_alloc(fbh_batch* b, uint32_t K) {
       return FBH_EHIP;
     }
     b->pr_nwmax = nwmax;
+    // the logical chains: member 0's pool position; a split chain's members hold `share` rows each at a stride of
+    // share rounded up to 64 (the pools keep each member's rows 64-row aligned)
+    std::vector<PrChain> hpc((size_t)b->CL);
+    for (int c = 0; c < b->CL; c++) {
+      const ChainDesc& d0 = b->hdesc[(size_t)c * b->W];
+      PrChain& q = hpc[c];
+      q.row_off = d0.row_off;
+      q.n = d0.n_total;
+      q.share = b->W > 1 ? d0.n : d0.n_total;
+      q.share_pad = (q.share + 63) / 64 * 64;
+      q.W = b->W;
+      q.dc0 = c * b->W;
+      q.offset = d0.offset;
+    }
+    if (hipMemcpy(b->d_pc, hpc.data(), sizeof(PrChain) * b->CL, hipMemcpyHostToDevice) != hipSuccess) {
+      set_error(FBH_EHIP, "probit redraw: hipMemcpy failed");
+      return FBH_EHIP;
+    }
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(k_pr_table), hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)pr_table_lds(nwmax)) != hipSuccess) {
       set_error(FBH_EHIP, "probit redraw: hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
@@ -3555,7 +3589,7 @@ static int pr_alloc(fbh_batch* b, uint32_t K) {
 
 static PrArgs make_pr_args(const fbh_batch* b) {
   PrArgs A;
-  A.desc = b->d_desc;
+  A.pc = b->d_pc;
   A.state = b->d_state;
   A.AF = b->d_AF;
   A.ybin = b->d_ybin;
@@ -3566,7 +3600,7 @@ static PrArgs make_pr_args(const fbh_batch* b) {
   A.ck = b->d_ck;
   A.ev = b->d_ev;
   A.btot = b->d_btot;
-  A.nb256 = (std::max(b->nmax, 1) + 255) / 256;
+  A.nb256 = (pr_nmax(b) + 255) / 256;
   A.E = b->d_E;
   A.lo = b->d_lo;
   A.nw = b->d_nw;
@@ -3588,7 +3622,7 @@ static PrArgs make_pr_args(const fbh_batch* b) {
 // one sweep's redraw for every chain, on the batch's stream (after the sweep kernel has left the fit in HBM)
 static hipError_t pr_launch(const fbh_batch* b) {
   const PrArgs A = make_pr_args(b);
-  const unsigned C = (unsigned)b->C, nm = (unsigned)std::max(b->nmax, 1);
+  const unsigned C = (unsigned)b->CL, nm = (unsigned)pr_nmax(b);
   hipLaunchKernelGGL(k_pr_rows, dim3((nm + 255) / 256, C), dim3(256), 0, b->stream, A);
   hipLaunchKernelGGL(k_pr_band, dim3(C), dim3(256), 0, b->stream, A);
   hipLaunchKernelGGL(k_pr_bandrows, dim3((nm + 255) / 256, C), dim3(256), 0, b->stream, A);
@@ -3703,6 +3737,19 @@ static int configure(fbh_batch* b, int packing, int kind, bool dense) {
   return FBH_OK;
 }
 
+constexpr int kSplitRows = 3072;  // the automatic row split's fewest rows per member
+
+// rows per member of a row split: ceil(n / W); for probit chains rounded up to a multiple of 64, so that the latent
+// redraw's blocks of 64 rows never straddle two members (0 if the last member would then be empty)
+static int split_share(int n, int W, bool probit) {
+  int s = (n + W - 1) / W;
+  if (probit) {
+    s = (s + 63) / 64 * 64;
+    if ((long)(W - 1) * s >= n) return 0;
+  }
+  return s;
+}
+
 extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const fbh_opts* o, fbh_batch** out) {
   if (!out || !chains || !o || nchains < 1) {
     set_error(FBH_EINVAL, "fbh_batch_create: null argument or no chains");
@@ -3721,21 +3768,29 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
     return FBH_EINVAL;
   }
   int W = o->row_split > 1 ? o->row_split : 1;
-  if (o->row_split < 0 && o->stats_mode == FBH_STATS_FX64 && !o->probit) {
-    // automatic: as many workgroups per chain as leave one workgroup per CU, within 64 rows per member
+  const bool pb = o->probit != 0;
+  if (o->row_split < 0 && o->stats_mode == FBH_STATS_FX64 && !(pb && o->probit_redraw == 1)) {
+    // automatic: as many workgroups per chain as leave one workgroup per CU, with at least kSplitRows rows per
+    // member (fewer, and the exchange after every pass costs more than the shorter pass saves: measured on one
+    // chain, 4,096 wbart rows run fastest on one workgroup, 24,576 probit rows on 8)
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, o->device) != hipSuccess) ncu = 0;
     W = std::max(1, std::min(64, ncu / nchains));
-    for (int c = 0; c < nchains; c++) W = std::max(1, std::min(W, chains[c].n / 64));
+    for (int c = 0; c < nchains; c++) {
+      W = std::max(1, std::min(W, chains[c].n / kSplitRows));
+      while (W > 1 && !split_share(chains[c].n, W, pb)) W--;
+    }
   }
-  if (W > 1 && (o->stats_mode != FBH_STATS_FX64 || o->probit || W > 64)) {
-    set_error(FBH_EINVAL, "fbh_batch_create: row_split needs FX64 statistics and wbart chains (and at most 64)");
+  if (W > 1 && (o->stats_mode != FBH_STATS_FX64 || W > 64 || (pb && o->probit_redraw == 1))) {
+    set_error(FBH_EINVAL, "fbh_batch_create: row_split needs FX64 statistics (at most 64; for probit chains the "
+                          "latent redraw spread over the device)");
     return FBH_EINVAL;
   }
   for (int c = 0; W > 1 && c < nchains; c++)
-    if (chains[c].n < 64 * W) {
+    if (chains[c].n < 64 * W || !split_share(chains[c].n, W, pb)) {
       set_error(FBH_EINVAL, "fbh_batch_create: row_split " + std::to_string(W) + " needs n >= " +
-                                std::to_string(64 * W) + " in every chain");
+                                std::to_string(64 * W) + " in every chain (probit: rows for every member in "
+                                "multiples of 64)");
       return FBH_EINVAL;
     }
   for (int c = 0; c < nchains; c++) {
@@ -3784,7 +3839,8 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
   b->nskip = o->nskip;
   b->ndpost = o->ndpost;
   b->probit = o->probit ? 1 : 0;
-  b->pr_mode = o->probit_redraw ? o->probit_redraw : (b->C <= kPrAutoChains ? 2 : 1);
+  // a split chain's rows are spread over its members: only the device-wide redraw sees them all
+  b->pr_mode = W > 1 ? 2 : (o->probit_redraw ? o->probit_redraw : (b->C <= kPrAutoChains ? 2 : 1));
   b->keep_trees = o->keep_trees ? 1 : 0;
   b->stats_mode = o->stats_mode;
   b->node_cap = o->node_cap ? o->node_cap : kNodes;
@@ -3808,7 +3864,7 @@ extern "C" int fbh_batch_create(const fbh_chain_in* chains, int nchains, const f
   for (int dc = 0; dc < nchains; dc++) {
     const int c = dc / W, k = dc % W;
     const fbh_chain_in& ci = chains[c];
-    const int share = (ci.n + W - 1) / W;
+    const int share = W > 1 ? split_share(ci.n, W, b->probit != 0) : ci.n;
     const int r0 = std::min(ci.n, k * share), r1 = std::min(ci.n, (k + 1) * share);
     b->row0[dc] = r0;
     if (k == W - 1) b->row0[dc + 1] = ci.n;
@@ -4089,8 +4145,13 @@ extern "C" int fbh_batch_probit_redraw(fbh_batch* b, int mode, int slots) {
     set_error(FBH_EINVAL, "fbh_batch_probit_redraw: bad argument (a probit batch, mode 0..2, slots >= 0)");
     return FBH_EINVAL;
   }
+  if (mode == 1 && b->W > 1) {
+    set_error(FBH_EINVAL, "fbh_batch_probit_redraw: a chain split over workgroups redraws with the device-wide "
+                          "kernels only");
+    return FBH_EINVAL;
+  }
   FBH_HIP_CHECK(hipSetDevice(b->device));
-  const int m = mode ? mode : (b->C <= kPrAutoChains ? 2 : 1);
+  const int m = b->W > 1 ? 2 : (mode ? mode : (b->C <= kPrAutoChains ? 2 : 1));
   if (m == 2) {
     FBH_HIP_CHECK(hipStreamSynchronize(b->stream));  // the slot buffers may be replaced
     const int rc = pr_alloc(b, (uint32_t)slots);

@@ -85,13 +85,15 @@ typedef struct fbh_opts {
   int block_threads;        /* 0 or 320: one control wave + four data waves per chain (the only shape built) */
   int tile_rows;            /* 0 = automatic; 1024 or 1536 rows staged in LDS per tile (1536 needs the running fit
                                resident in LDS; 1024 alone, with resident = 0, streams it) */
-  int row_split;            /* 0 or 1 = one workgroup per chain. W > 1 (FX64 statistics, wbart only): the rows of
-                               every chain are split over W workgroups that run the same control steps and add their
-                               fixed-point pass totals after every pass -- exact, as fixed-point sums are order-free,
-                               so the results equal the one-workgroup FX64 chain bit for bit. Every chain needs
-                               n >= 64 W, and all C x W workgroups must fit on the device at once (FBH_EINVAL if not).
-                               -1 = automatic for FX64 wbart batches: the largest W with C x W <= the CU count and
-                               n >= 64 W (1 otherwise). */
+  int row_split;            /* 0 or 1 = one workgroup per chain. W > 1 (FX64 statistics): the rows of every chain
+                               are split over W workgroups that run the same control steps and add their fixed-point
+                               pass totals after every pass -- exact, as fixed-point sums are order-free, so the
+                               results equal the one-workgroup FX64 chain bit for bit. Every chain needs n >= 64 W,
+                               and all C x W workgroups must fit on the device at once (FBH_EINVAL if not). Probit
+                               chains split too, their members holding multiples of 64 rows (every member non-empty)
+                               and their latents redrawn by the device-wide kernels (probit_redraw 2; 1 is refused).
+                               -1 = automatic for FX64 batches: the largest W with C x W <= the CU count and at least
+                               3,072 rows per member (1 otherwise). */
   int packing;              /* workgroup -> chain assignment of the sweep kernel: 0 = batch order, 1 = serpentine by
                                rows (large and small chains share CUs), 2 = largest first. Timing only: results do not
                                depend on it. fbh_batch_autotune measures and picks. */

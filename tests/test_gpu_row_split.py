@@ -83,21 +83,25 @@ def test_gpu_row_split_rejects_what_it_cannot_run():
 
 
 def test_gpu_row_split_automatic():
-    """row_split = -1: one workgroup per CU for FX64 batches (here bounded by 64 rows per member), none for REF16."""
-    probs = [W.small_problem(n, 4, 25, 31 + n, "uniform", ntree=12) for n in (1100, 1900)]
+    """row_split = -1: up to one workgroup per CU for FX64 batches, with at least 3,072 rows per member (here the
+    shorter chain's 7,000 rows allow 2); none for REF16, nor for chains shorter than 6,144 rows."""
+    probs = [W.small_problem(n, 4, 25, 31 + n, "uniform", ntree=12) for n in (13000, 7000)]
     m, nskip, nd = 12, 6, 4
     _say("automatic split")
     outs = {}
     for split in (-1, 0):
         with sampler.BatchSampler(probs, m, nskip, nd, [5, 6], stats_mode=1, row_split=split) as b:
             if split == -1:
-                assert b.row_split == 1100 // 64
+                assert b.row_split == 2
             b.run(nskip + nd)
             outs[split] = [b.fetch(c) for c in range(2)]
     for c in range(2):
         for k in ("sigma", "trmean", "trees", "stats"):
             assert same_bits(outs[-1][c][k], outs[0][c][k]), (c, k)
     with sampler.BatchSampler(probs, m, nskip, nd, [5, 6], stats_mode=0, row_split=-1) as b:
+        assert b.row_split == 1
+    short = [W.small_problem(4096, 4, 25, 37, "uniform", ntree=12)]
+    with sampler.BatchSampler(short, m, nskip, nd, [5], stats_mode=1, row_split=-1) as b:
         assert b.row_split == 1
 
 
@@ -123,3 +127,40 @@ def test_gpu_row_split_with_launch_configuration(oracle):
             out = b.fetch(c)
             for k in ("sigma", "trmean", "trees", "stats"):
                 assert same_bits(out[k], e[k]), (c, k)
+
+
+@pytest.mark.parametrize("n,split", [(24576, 4), (5000, 3), (24576, 16)])
+def test_gpu_row_split_probit_fx64_matches_oracle(oracle, n, split):
+    """A probit chain with FX64 statistics split over workgroups (each member 64-row multiples of the chain's rows,
+    the pass totals exchanged after every pass) and its latents redrawn by the device-wide kernels over all members'
+    rows: latents and generator state after every sweep, and every output, against the oracle's one-chain FX64 twin."""
+    pb = W.small_problem(n, 3, 25, 1200 + n, probit=True, ntree=20)
+    m, nskip, nd = 20, 3, 2
+    ch = oracle.chain(pb, m, nskip, nd, 7, probit=1, stats_mode=1)
+    print("probit row split", n, split, flush=True)
+    with sampler.BatchSampler([pb], m, nskip, nd, [7], probit=1, stats_mode=1, row_split=split) as b:
+        assert b.row_split == split and b.probit_info["mode"] == 2
+        s0, e0 = b.state(0), ch.state()
+        assert same_bits(s0["latent"], e0["latent"])
+        for k in range(nskip + nd):
+            b.run(1)
+            ch.sweeps(1)
+            s, e = b.state(0), ch.state()
+            assert np.array_equal(s["rng"], e["rng"]), k
+            assert same_bits(s["latent"], e["latent"]), k
+        out = b.fetch(0)
+        with pytest.raises(FbhError):                   # the in-kernel redraw sees one member's rows only
+            b.probit_redraw(1)
+    assert same_bits(out["trees"].reshape(-1), ch.trees_out)
+    assert same_bits(out["trmean"], ch.trmean())
+
+
+def test_gpu_row_split_probit_rejects_what_it_cannot_run():
+    pb = W.small_problem(3000, 3, 20, 1301, probit=True, ntree=5)
+    with pytest.raises(FbhError):                       # REF16
+        sampler.BatchSampler([pb], 5, 1, 1, [1], probit=1, stats_mode=0, row_split=2)
+    with pytest.raises(FbhError):                       # the in-kernel redraw asked for
+        sampler.BatchSampler([pb], 5, 1, 1, [1], probit=1, stats_mode=1, row_split=2, probit_redraw=1)
+    with pytest.raises(FbhError):                       # 200 rows in 128-row members: the third would be empty
+        sampler.BatchSampler([W.small_problem(200, 3, 20, 1302, probit=True, ntree=5)], 5, 1, 1, [1], probit=1,
+                             stats_mode=1, row_split=3)

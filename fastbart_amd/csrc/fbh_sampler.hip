@@ -1567,7 +1567,7 @@ __global__ void k_trmean(const double* TM, float* out, int n, int ndpost) {
 //   k_pr_rows, k_pr_band, k_pr_bandrows: per row, the expected rejections and their variance (geometric, from the
 //                row's truncation point); prefix sums (per 256 rows, over those, then within them) give the predicted
 //                offset E_i and variance V_i of every row, and its band of offsets [lo_i, lo_i + 64 nw_i), centred on
-//                E_i, half-width 4.5 sqrt(V_i) + 64;
+//                E_i, half-width 3.5 sqrt(V_i) + 32;
 //   k_pr_gen:    the stream's first K slots: thread t jumps to slot 64 t with the GF(2) matrices T^(128 2^k) of the
 //                generator's state transition (T is linear over GF(2)^128) and steps through 64 slots;
 //   k_pr_decide: one wave per row decides every offset of its band (latent_accepts: device math, the host-libm
@@ -1738,7 +1738,7 @@ __global__ __launch_bounds__(256) void k_pr_bandrows(PrArgs A) {
   const double2 b = A.btot[(size_t)c * A.nb256 + blockIdx.x];
   pe += b.x;
   pv += b.y;
-  const double h = 4.5 * sqrt(pv) + 64.;
+  const double h = 3.5 * sqrt(pv) + 32.;
   const int w = min(A.nwmax, max(3, (int)ceil(h / 32.)));
   A.E[R(i)] = (float)pe;
   A.lo[R(i)] = max(0, (int)floor(pe - 32. * w));
@@ -3516,7 +3516,7 @@ static size_t pr_table_lds(int nwmax) { return (size_t)64 * nwmax * 8 + 64 * 8; 
 // sends the slots past it to the walk's own stepping
 static int pr_alloc(fbh_batch* b, uint32_t K) {
   const int nm = pr_nmax(b);
-  const int nwmax = std::min(255, (int)std::ceil((9.0 * std::sqrt(2.0 * nm) + 128.0) / 64.0) + 1);
+  const int nwmax = std::min(255, (int)std::ceil((7.0 * std::sqrt(2.0 * nm) + 64.0) / 64.0) + 1);
   const uint64_t Kd = ((2ull * (uint64_t)nm + 64ull * (uint64_t)nwmax + 4096ull) + 63ull) / 64ull * 64ull;
   if (K == 0) K = (uint32_t)Kd;
   if (K % kPrSlots || K < (uint32_t)kPrSlots || (uint64_t)K / kPrSlots > (1ull << kJumpLevels)) {

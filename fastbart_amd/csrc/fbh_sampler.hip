@@ -1745,37 +1745,47 @@ __global__ __launch_bounds__(256) void k_pr_bandrows(PrArgs A) {
   A.nw[R(i)] = (uint8_t)w;
 }
 
+// A quad of lanes per generating thread t (slots 64 t .. 64 t + 63): lane j of the quad applies the jump tables'
+// columns of state word j, the quad's XOR combines the four parts; then every lane steps through the 64 slots and
+// writes its quarter of them.
 __global__ __launch_bounds__(256) void k_pr_gen(PrArgs A) {
-  const int c = blockIdx.y;
-  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  const int c = blockIdx.y, j = threadIdx.x & 3;
+  const uint32_t t = blockIdx.x * 64u + (threadIdx.x >> 2);
   const uint32_t nth = A.K / kPrSlots;  // generating threads of a chain
-  // levels used by any thread of the chain: the bits of nth - 1; their tables staged in LDS (read as broadcasts)
+  // levels used by any thread of the chain: the bits of nth - 1; their tables staged in LDS
   const int lev = nth > 1 ? 32 - __builtin_clz(nth - 1) : 0;
   __shared__ uint4 J[kJumpLevels][128];
   for (int k = threadIdx.x; k < lev * 128; k += 256) J[k / 128][k % 128] = A.jump[k];
   __syncthreads();
-  if (t >= nth) return;
+  if (t >= nth) return;  // a quad's four lanes share t: they leave together, and its shuffles see only live lanes
   const ChainState* cs = A.state + A.pc[c].dc0;
   uint32_t s0 = cs->rng[0], s1 = cs->rng[1], s2 = cs->rng[2], s3 = cs->rng[3];
   for (int k = 0; k < lev; k++) {
+    const uint32_t x = j == 0 ? s0 : (j == 1 ? s1 : (j == 2 ? s2 : s3));
     uint32_t o0 = 0, o1 = 0, o2 = 0, o3 = 0;
-    // columns in groups of 16, every read of a group issued before its use (one wave per SIMD: nothing else hides
-    // the LDS latency)
 #pragma unroll
-    for (int g = 0; g < 8; g++) {
+    for (int g = 0; g < 2; g++) {
       uint4 col[16];
 #pragma unroll
-      for (int q = 0; q < 16; q++) col[q] = J[k][16 * g + q];
-      const uint32_t x = (g >> 1) == 0 ? s0 : ((g >> 1) == 1 ? s1 : ((g >> 1) == 2 ? s2 : s3));
+      for (int q = 0; q < 16; q++) col[q] = J[k][32 * j + 16 * g + q];
 #pragma unroll
       for (int q = 0; q < 16; q++) {
-        const uint32_t msk = 0u - ((x >> (16 * (g & 1) + q)) & 1u);
+        const uint32_t msk = 0u - ((x >> (16 * g + q)) & 1u);
         o0 ^= col[q].x & msk;
         o1 ^= col[q].y & msk;
         o2 ^= col[q].z & msk;
         o3 ^= col[q].w & msk;
       }
     }
+    // the quad's four parts
+    o0 ^= __shfl_xor(o0, 1);
+    o1 ^= __shfl_xor(o1, 1);
+    o2 ^= __shfl_xor(o2, 1);
+    o3 ^= __shfl_xor(o3, 1);
+    o0 ^= __shfl_xor(o0, 2);
+    o1 ^= __shfl_xor(o1, 2);
+    o2 ^= __shfl_xor(o2, 2);
+    o3 ^= __shfl_xor(o3, 2);
     const bool take = (t >> k) & 1u;
     s0 = take ? o0 : s0;
     s1 = take ? o1 : s1;
@@ -1783,14 +1793,14 @@ __global__ __launch_bounds__(256) void k_pr_gen(PrArgs A) {
     s3 = take ? o3 : s3;
   }
   Rng g{s0, s1, s2, s3};
-  A.ck[(size_t)c * nth + t] = make_uint4(g.s0, g.s1, g.s2, g.s3);
+  if (j == 0) A.ck[(size_t)c * nth + t] = make_uint4(g.s0, g.s1, g.s2, g.s3);
   uint2* out = A.slots + (size_t)c * A.K + (size_t)t * kPrSlots;
   for (int k = 0; k < kPrSlots; k++) {
     const uint32_t x1 = starstar(g.s1);
     rng_step(g, 11u, 21u);
     const uint32_t x2 = starstar(g.s1);
     rng_step(g, 11u, 21u);
-    out[k] = make_uint2(x1, x2);
+    if ((k >> 4) == j) out[k] = make_uint2(x1, x2);
   }
 }
 
@@ -3626,7 +3636,7 @@ static hipError_t pr_launch(const fbh_batch* b) {
   hipLaunchKernelGGL(k_pr_rows, dim3((nm + 255) / 256, C), dim3(256), 0, b->stream, A);
   hipLaunchKernelGGL(k_pr_band, dim3(C), dim3(256), 0, b->stream, A);
   hipLaunchKernelGGL(k_pr_bandrows, dim3((nm + 255) / 256, C), dim3(256), 0, b->stream, A);
-  hipLaunchKernelGGL(k_pr_gen, dim3((A.K / kPrSlots + 255) / 256, C), dim3(256), 0, b->stream, A);
+  hipLaunchKernelGGL(k_pr_gen, dim3((A.K / kPrSlots + 63) / 64, C), dim3(256), 0, b->stream, A);
   hipLaunchKernelGGL(k_pr_decide, dim3((nm + 3) / 4, C), dim3(256), 0, b->stream, A);
   hipLaunchKernelGGL(k_pr_table, dim3((unsigned)A.nbmax, C), dim3(256), pr_table_lds(A.nwmax), b->stream, A);
   hipLaunchKernelGGL(k_pr_compose, dim3((unsigned)A.ngmax, C), dim3(256), 0, b->stream, A);

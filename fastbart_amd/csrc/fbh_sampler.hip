@@ -1940,25 +1940,31 @@ __global__ __launch_bounds__(256) void k_pr_table(PrArgs A) {
   __syncthreads();
   const int L = slo[0], Wd = 64 * snw[0];
   int* T = A.tab + ((size_t)c * A.nbmax + b) * (size_t)A.wmax;
-  for (int u = tid; u < Wd; u += 256) {
-    int x = L + u;
+  // kPrTabE entries per thread walked together: independent chains of dependent LDS reads, issued side by side
+  constexpr int kPrTabE = 4;
+  for (int u0 = tid; u0 < Wd; u0 += 256 * kPrTabE) {
+    int x[kPrTabE];
+#pragma unroll
+    for (int e = 0; e < kPrTabE; e++) x[e] = u0 + 256 * e < Wd ? L + u0 + 256 * e : -1;
     for (int l = 0; l < nr; l++) {
       const int lo = slo[l], nw = snw[l];
-      const int rel = x - lo;
-      if (rel < 0 || rel >= 64 * nw) {
-        x = -1;
-        break;
+#pragma unroll
+      for (int e = 0; e < kPrTabE; e++) {
+        if (x[e] < 0) continue;
+        const int rel = x[e] - lo;
+        if (rel < 0 || rel >= 64 * nw) {
+          x[e] = -1;
+          continue;
+        }
+        int wi = rel >> 6;
+        unsigned long long m = pr_sm[l * nwm + wi] & (~0ull << (rel & 63));
+        while (!m && ++wi < nw) m = pr_sm[l * nwm + wi];
+        x[e] = m ? lo + 64 * wi + (int)__builtin_ctzll(m) : -1;
       }
-      int wi = rel >> 6;
-      unsigned long long m = pr_sm[l * nwm + wi] & (~0ull << (rel & 63));
-      while (!m && ++wi < nw) m = pr_sm[l * nwm + wi];
-      if (!m) {
-        x = -1;
-        break;
-      }
-      x = lo + 64 * wi + (int)__builtin_ctzll(m);
     }
-    T[u] = x;
+#pragma unroll
+    for (int e = 0; e < kPrTabE; e++)
+      if (u0 + 256 * e < Wd) T[u0 + 256 * e] = x[e];
   }
 }
 

@@ -645,8 +645,8 @@ def main() -> int:
                          "conditional, or each accepted birth with BART's rule")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build runs every chain on a single MI355X "
-              "(chains shard across GPUs with no exchange; see the design notes). Use --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: every chain runs on a single MI355X (chains would shard across GPUs "
+              "with no exchange between them). Use --gpus 1.", file=sys.stderr)
         return 2
     wl = args.workload
     if wl.startswith("predict"):

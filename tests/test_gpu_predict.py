@@ -1,5 +1,6 @@
 """GPU predictor: compile_trees / predict_draw / batched draws against the oracle's restatement of src/predict.cpp,
-and the invariant that re-predicting the training rows from the text reproduces yhat.train (the reference's README.md:88).
+and the invariant that re-predicting the training rows from the text reproduces yhat.train (the reference's
+README.md:88).
 
 Two device kernels serve a prediction: the rank form (sorted, NaN-free cutpoints: rows as 16-bit ranks, fixed-depth
 walks) and the general form (any cutpoint table, any draw size). Both are compared with the oracle bit for bit."""

@@ -305,10 +305,19 @@ __device__ __forceinline__ double latent_value(const LatentRow& r, uint32_t x1, 
 }
 
 // ------------------------------------------------------------------ control state of one chain's workgroup
-struct ThR {  // a slot's leaf value and, for a used leaf, its rank among the tree's leaves (a row's task tag)
+struct ThR {  // a slot's leaf value and, for a used leaf, its code (a row's task tag, leaf_code below)
   double th;
   uint32_t rank, pad_;
 };
+
+// A used leaf's code in the row tags: its rank among the tree's leaves, or, in the one-chain-per-CU build (oh) for a
+// tree of at most kOneHotLeaves leaves, one bit at the rank, which lets a summing thread test its task with a single
+// bit-field extract (bits 6 and 7 of the tag are the birth proposal's halves). The dense build keeps the ranks: its
+// second summing loop would cost more in spilled registers than it saves.
+constexpr int kOneHotLeaves = 6;
+__device__ __forceinline__ uint32_t leaf_code(int rank, int nleaf, bool oh) {
+  return (oh && nleaf <= kOneHotLeaves) ? (1u << rank) : (uint32_t)rank;
+}
 struct Ctl {
   TreeDev tree[2];               // tree j (being updated) and tree j+1 (proposal prepared during pass j)
   ChainDesc cd;
@@ -409,7 +418,7 @@ __device__ __forceinline__ void copy_tree(TreeDev* dst, const TreeDev* src, int 
 // ------------------------------------------------------------------ control wave pieces
 // Proposal for tree buffer b (:413-464). All 64 lanes run it (lane s looks at slot s); the generator state and every
 // decision are wave-uniform.
-__device__ void ctl_propose(Ctl& c, int b, Rng& g, int cap, int lane) {
+__device__ void ctl_propose(Ctl& c, int b, Rng& g, int cap, int lane, bool oh) {
   const TreeDev& t = c.tree[b];
   const double pb = 0.5;
   const int p = c.cd.p;
@@ -423,7 +432,7 @@ __device__ void ctl_propose(Ctl& c, int b, Rng& g, int cap, int lane) {
   c.thr[b][s].th = (double)t.th[s];
   if (BITM(lm, s)) {
     const int rk = __popcll(lm & ((1ull << s) - 1ull));
-    c.thr[b][s].rank = (uint32_t)rk;
+    c.thr[b][s].rank = leaf_code(rk, __popcll(lm), oh);
     c.slot_of[b][rk] = (uint8_t)s;
   } else {
     c.thr[b][s].rank = 0;
@@ -550,13 +559,16 @@ struct RowPtrs {
 };
 
 // Walk one lane's rows in row order, adding src[i] to the task's sum where the task matches the row's tag: bit f of the
-// task's pattern, f being the tag's field (sh, wd) -- the leaf rank (bits 0-4) or the birth-split side (bits 6-7). Two
-// bit-field extracts and a conversion make the 0.0 / 1.0 factor: no compare, so no lane mask and no wait states on
-// the sequential adds. Rows go in blocks of 16 whose LDS reads are all issued before the adds.
+// task's pattern, f being the tag's field (sh, wd) -- the leaf rank (bits 0-4) or the birth-split side (bits 6-7); with
+// one-bit leaf codes (OH), bit sh of the tag itself. Bit-field extracts and a conversion make the 0.0 / 1.0 factor: no
+// compare, so no lane mask and no wait states on the sequential adds. Rows go in blocks of 16 whose LDS reads are all
+// issued before the adds.
+template <bool OH>
 __device__ __forceinline__ uint32_t tag_match(uint32_t word, uint32_t off, uint32_t wd, uint32_t pat) {
+  if (OH) return __builtin_amdgcn_ubfe(word, off, 1u);
   return __builtin_amdgcn_ubfe(pat, __builtin_amdgcn_ubfe(word, off, wd), 1u);
 }
-template <bool FX>
+template <bool FX, bool OH>
 __device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, const double* __restrict__ src, int cnt,
                                            uint32_t sh, uint32_t wd, uint32_t pat, double acc) {
   int i = 0;
@@ -571,7 +583,7 @@ __device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, cons
 #pragma unroll
       for (int u = 0; u < 16; u++) x[u] = v[i + u];
 #pragma unroll
-      for (int u = 0; u < 16; u++) s += tag_match(t4[u >> 2], 8 * (u & 3) + sh, wd, pat) ? x[u] : 0ll;
+      for (int u = 0; u < 16; u++) s += tag_match<OH>(t4[u >> 2], 8 * (u & 3) + sh, wd, pat) ? x[u] : 0ll;
     }
     for (; i < cnt; i += 8) {
       // the last rows in blocks of 8: rows at or past cnt hold earlier values and are masked out
@@ -583,7 +595,7 @@ __device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, cons
 #pragma unroll
       for (int u = 0; u < 8; u++) x[u] = v[i + u];
 #pragma unroll
-      for (int u = 0; u < 8; u++) s += (u < rem && tag_match(t2[u >> 2], 8 * (u & 3) + sh, wd, pat)) ? x[u] : 0ll;
+      for (int u = 0; u < 8; u++) s += (u < rem && tag_match<OH>(t2[u >> 2], 8 * (u & 3) + sh, wd, pat)) ? x[u] : 0ll;
     }
     return __longlong_as_double(s);
   }
@@ -596,7 +608,7 @@ __device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, cons
 #pragma unroll
     for (int u = 0; u < 16; u++) x[u] = src[i + u];
 #pragma unroll
-    for (int u = 0; u < 16; u++) s = fma((double)tag_match(t4[u >> 2], 8 * (u & 3) + sh, wd, pat), x[u], s);
+    for (int u = 0; u < 16; u++) s = fma((double)tag_match<OH>(t4[u >> 2], 8 * (u & 3) + sh, wd, pat), x[u], s);
   }
   for (; i < cnt; i += 8) {
     // the last rows in blocks of 8: rows at or past cnt hold earlier values (any bit pattern) and add +0.0
@@ -610,7 +622,7 @@ __device__ __forceinline__ double lane_sum(const uint8_t* __restrict__ tag, cons
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const bool in = u < rem;
-      s = fma(in ? (double)tag_match(t2[u >> 2], 8 * (u & 3) + sh, wd, pat) : 0.0, in ? x[u] : 0.0, s);
+      s = fma(in ? (double)tag_match<OH>(t2[u >> 2], 8 * (u & 3) + sh, wd, pat) : 0.0, in ? x[u] : 0.0, s);
     }
   }
   return s;
@@ -691,6 +703,7 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
   const int cmode = stats ? (int)ufl(c.cmode[cb]) : 0, cnx = ufl(c.cnx[cb]), cc = ufl(c.cc[cb]), cv = ufl(c.cv[cb]);
   const int nleaf = ufl(c.nleaf[cb]);
   const int ntask = rss ? 2 : 2 * (nleaf + (cmode == 1 ? 2 : 0));
+  const bool oh = !D && !rss && nleaf <= kOneHotLeaves;  // the tree's leaf codes are single bits (leaf_code)
   const size_t ns = (size_t)P.ns;
   uint8_t* __restrict__ lprev = apply ? P.LEAF + (size_t)jp * ns : nullptr;
   const uint8_t* __restrict__ lcur = stats ? P.LEAF + (size_t)j * ns : nullptr;
@@ -816,7 +829,7 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
             } else {
               const double rr = y - af + thc_r[k];
               const double wr = w * rr;
-              // tag: the leaf rank, bit 6 / 7 for the left / right half of the birth proposal's node
+              // tag: the leaf's code (leaf_code), bit 6 / 7 for the left / right half of the birth proposal's node
               uint32_t tag = rank_r[k];
               if (cmode == 1) {
                 const bool split = ((T.lb[s] >> (8 * u)) & 255u) == (uint32_t)cnx && i < (uint32_t)n;
@@ -839,8 +852,9 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
     }
     wave_sync();
     // ---- phase B: this thread's tasks walk lane kb's rows of the tile in row order. task = 2 rank + kind: kind 0
-    // sums w, kind 1 sums w*r; ranks below nleaf are leaves (rank field), then the split halves (side field 1 and
-    // 2); the closing pass sums every row (an empty field matches bit 0). A slot past ntask sums nothing.
+    // sums w, kind 1 sums w*r; ranks below nleaf are leaves (rank field, or in the one-chain-per-CU build the rank's
+    // bit for trees of at most kOneHotLeaves leaves), then the split halves (side field 1 and 2, bits 6 and 7); the
+    // closing pass sums every row (an empty field matches bit 0). A slot past ntask sums nothing.
     // the next tile's loads go out now and arrive during phase B
     if (tb + G::kTile < n) tile_load<R, RES>(T, P, la, lb, qa, qb, ii0 + G::kTW4, tq, qlast);
     const int cnt = min(G::kTW4, max(0, nper_kb - ii0));
@@ -856,10 +870,16 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
       const int task = rtask + kTaskSlots * g;
       if (task < ntask && (!rp || g == 0)) {
         const int q = task >> 1;
-        const uint32_t sh = (!rss && q >= nleaf) ? 6u : 0u;
-        const uint32_t wd = rss ? 0u : (q < nleaf ? 5u : 2u);
-        const uint32_t pat = rss ? 1u : (q < nleaf ? (1u << q) : (q == nleaf ? 2u : 4u));
-        acc[g] = lane_sum<FX>(tgb + r0, ((task & 1) ? v1b : v0b) + r0, rn, sh, wd, pat, acc[g]);
+        const double* src = ((task & 1) ? v1b : v0b) + r0;
+        if (!D && oh) {
+          const uint32_t bit = q < nleaf ? (uint32_t)q : (q == nleaf ? 6u : 7u);
+          acc[g] = lane_sum<FX, !D>(tgb + r0, src, rn, bit, 0u, 0u, acc[g]);
+        } else {
+          const uint32_t sh = (!rss && q >= nleaf) ? 6u : 0u;
+          const uint32_t wd = rss ? 0u : (q < nleaf ? 5u : 2u);
+          const uint32_t pat = rss ? 1u : (q < nleaf ? (1u << q) : (q == nleaf ? 2u : 4u));
+          acc[g] = lane_sum<FX, false>(tgb + r0, src, rn, sh, wd, pat, acc[g]);
+        }
       }
     }
     wave_sync();
@@ -1237,7 +1257,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       }
       copy_tree(&c.tree[0], &TR[0], lane);
       wave_sync();
-      ctl_propose(c, 0, g, cap, lane);
+      ctl_propose(c, 0, g, cap, lane, !D);
       __syncthreads();
       __builtin_amdgcn_s_setprio(0);
       int cb = 0;
@@ -1286,7 +1306,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
           tdst[lane] = tw0;
           if (lane < kTreeWords - 64) tdst[64 + lane] = tw1;
           wave_sync();
-          ctl_propose(c, cb ^ 1, g, cap, lane);
+          ctl_propose(c, cb ^ 1, g, cap, lane, !D);
         }
         if (want) c.z[lane] = __builtin_sqrtf(-2.f * lf) * cf;
         if (lane == 0) c.lu = luv;

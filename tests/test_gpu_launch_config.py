@@ -2,6 +2,7 @@
 CU at 168 VGPRs or two at 128), graph capture and the autotuner that picks among them. Chains are independent and the state between launches is complete in device memory,
 so none of these may change a result: every configuration, and a switch between configurations at any launch
 boundary, must reproduce the oracle's chains bit for bit."""
+import numpy as np
 import pytest
 
 from conftest import same_bits
@@ -45,6 +46,25 @@ def test_gpu_packing_and_graph_match_oracle(oracle, packing, graph, per_cu):
         avg, nl, _ = b.timing()
         assert nl == 2 + 4 and avg > 0  # 7 sweeps in 2 launches, 13 in 4 (a graph is timed as a whole)
         _check_oracle(oracle, b, probs, m, nskip, nd, seeds)
+
+
+@pytest.mark.parametrize("per_cu", [1, 2])
+def test_gpu_trees_past_six_leaves_match_oracle(oracle, per_cu):
+    """Two trees on 4,000 rows grow past six leaves. The one-chain-per-CU build tags a row with one bit at its leaf's
+    rank while the tree has at most six leaves and with the rank itself beyond that; the dense build always with the
+    rank. Every output against the oracle in both builds, with kept trees on both sides of six leaves."""
+    pb = W.small_problem(4000, 5, 40, 977, ntree=2)
+    m, nskip, nd = 2, 0, 60  # kept from the first sweep: 2 to 17 leaves
+    _say("trees past six leaves, chains per CU", per_cu)
+    with sampler.BatchSampler([pb], m, nskip, nd, [7], chains_per_cu=per_cu, sweeps_per_launch=10) as b:
+        b.run(nskip + nd)
+        out = b.fetch(0)
+    e = oracle.fit(pb, m, nskip, nd, 7)
+    for k in ("sigma", "trmean", "trees", "stats"):
+        assert same_bits(out[k], e[k]), k
+    w = np.asarray(out["trees"]).view(np.uint64).reshape(-1, 64)
+    leaves = (((w & 255) != 255) & (((w >> 16) & 255) == 0) & (((w >> 24) & 255) == 0)).sum(axis=1)
+    assert leaves.max() > 6 and leaves.min() <= 6, (leaves.min(), leaves.max())
 
 
 def test_gpu_probit_packing_and_graph_match_oracle(oracle):

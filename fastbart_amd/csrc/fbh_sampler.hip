@@ -310,10 +310,10 @@ struct ThR {  // a slot's leaf value and, for a used leaf, its code (a row's tas
   uint32_t rank, pad_;
 };
 
-// A used leaf's code in the row tags: its rank among the tree's leaves, or, in the one-chain-per-CU build (oh) for a
-// tree of at most kOneHotLeaves leaves, one bit at the rank, which lets a summing thread test its task with a single
-// bit-field extract (bits 6 and 7 of the tag are the birth proposal's halves). The dense build keeps the ranks: its
-// second summing loop would cost more in spilled registers than it saves.
+// A used leaf's code in the row tags: its rank among the tree's leaves, or, in the one-chain-per-CU build with REF16
+// statistics (oh) for a tree of at most kOneHotLeaves leaves, one bit at the rank, which lets a summing thread test
+// its task with a single bit-field extract (bits 6 and 7 of the tag are the birth proposal's halves). The dense build
+// and FX64 statistics keep the ranks: measured, the one-bit loop is slower there.
 constexpr int kOneHotLeaves = 6;
 __device__ __forceinline__ uint32_t leaf_code(int rank, int nleaf, bool oh) {
   return (oh && nleaf <= kOneHotLeaves) ? (1u << rank) : (uint32_t)rank;
@@ -703,7 +703,7 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
   const int cmode = stats ? (int)ufl(c.cmode[cb]) : 0, cnx = ufl(c.cnx[cb]), cc = ufl(c.cc[cb]), cv = ufl(c.cv[cb]);
   const int nleaf = ufl(c.nleaf[cb]);
   const int ntask = rss ? 2 : 2 * (nleaf + (cmode == 1 ? 2 : 0));
-  const bool oh = !D && !rss && nleaf <= kOneHotLeaves;  // the tree's leaf codes are single bits (leaf_code)
+  const bool oh = !D && !FX && !rss && nleaf <= kOneHotLeaves;  // the tree's leaf codes are single bits (leaf_code)
   const size_t ns = (size_t)P.ns;
   uint8_t* __restrict__ lprev = apply ? P.LEAF + (size_t)jp * ns : nullptr;
   const uint8_t* __restrict__ lcur = stats ? P.LEAF + (size_t)j * ns : nullptr;
@@ -852,9 +852,9 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
     }
     wave_sync();
     // ---- phase B: this thread's tasks walk lane kb's rows of the tile in row order. task = 2 rank + kind: kind 0
-    // sums w, kind 1 sums w*r; ranks below nleaf are leaves (rank field, or in the one-chain-per-CU build the rank's
-    // bit for trees of at most kOneHotLeaves leaves), then the split halves (side field 1 and 2, bits 6 and 7); the
-    // closing pass sums every row (an empty field matches bit 0). A slot past ntask sums nothing.
+    // sums w, kind 1 sums w*r; ranks below nleaf are leaves (rank field, or in the one-chain-per-CU REF16 build the
+    // rank's bit for trees of at most kOneHotLeaves leaves), then the split halves (side field 1 and 2, bits 6 and
+    // 7); the closing pass sums every row (an empty field matches bit 0). A slot past ntask sums nothing.
     // the next tile's loads go out now and arrive during phase B
     if (tb + G::kTile < n) tile_load<R, RES>(T, P, la, lb, qa, qb, ii0 + G::kTW4, tq, qlast);
     const int cnt = min(G::kTW4, max(0, nper_kb - ii0));
@@ -871,9 +871,9 @@ __device__ void data_pass(Ctl& c, unsigned char* scratch, const RowPtrs& P, int 
       if (task < ntask && (!rp || g == 0)) {
         const int q = task >> 1;
         const double* src = ((task & 1) ? v1b : v0b) + r0;
-        if (!D && oh) {
+        if (!D && !FX && oh) {
           const uint32_t bit = q < nleaf ? (uint32_t)q : (q == nleaf ? 6u : 7u);
-          acc[g] = lane_sum<FX, !D>(tgb + r0, src, rn, bit, 0u, 0u, acc[g]);
+          acc[g] = lane_sum<FX, !D && !FX>(tgb + r0, src, rn, bit, 0u, 0u, acc[g]);
         } else {
           const uint32_t sh = (!rss && q >= nleaf) ? 6u : 0u;
           const uint32_t wd = rss ? 0u : (q < nleaf ? 5u : 2u);
@@ -1257,7 +1257,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
       }
       copy_tree(&c.tree[0], &TR[0], lane);
       wave_sync();
-      ctl_propose(c, 0, g, cap, lane, !D);
+      ctl_propose(c, 0, g, cap, lane, !D && !FX);
       __syncthreads();
       __builtin_amdgcn_s_setprio(0);
       int cb = 0;
@@ -1306,7 +1306,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_wa
           tdst[lane] = tw0;
           if (lane < kTreeWords - 64) tdst[64 + lane] = tw1;
           wave_sync();
-          ctl_propose(c, cb ^ 1, g, cap, lane, !D);
+          ctl_propose(c, cb ^ 1, g, cap, lane, !D && !FX);
         }
         if (want) c.z[lane] = __builtin_sqrtf(-2.f * lf) * cf;
         if (lane == 0) c.lu = luv;
